@@ -1,0 +1,260 @@
+/*
+ * vjx -- MI355X-native transient core for vajax-style circuit simulation.
+ *
+ * Plain C ABI (extern "C", plain pointers and sizes).  Every function returns
+ * an int status: 0 = ok, > 0 = warning (result usable), < 0 = error; the text
+ * of the last error or warning on the calling thread is vjx_last_error().
+ *
+ * Pointer conventions: arguments named *_host are host memory borrowed for the
+ * duration of the call; arguments named *_dev are device (HBM) memory on the
+ * current HIP device, also borrowed.  The library owns everything behind its
+ * handles and frees it in the matching *_destroy call.
+ */
+#ifndef VJX_H
+#define VJX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VJX_OK 0
+#define VJX_WARN_PIVOT 1      /* numeric pivot degraded; result still returned */
+#define VJX_WARN_NOCONV 2     /* Newton iteration did not converge */
+#define VJX_ERR_ARG (-1)      /* invalid argument / malformed CSR */
+#define VJX_ERR_HIP (-2)      /* HIP runtime error */
+#define VJX_ERR_SINGULAR (-3) /* structurally singular matrix */
+#define VJX_ERR_STATE (-4)    /* call out of sequence (e.g. solve before refactor) */
+
+#define VJX_PREC_F64 0 /* factorise and solve in FP64 */
+#define VJX_PREC_F32 1 /* factorise in FP32, FP64 residual + refinement */
+
+typedef struct vjx_pattern vjx_pattern;
+typedef struct vjx_circuit vjx_circuit;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int vjx_version(void);
+/* Text of the last error/warning on this thread ("" if none). */
+const char* vjx_last_error(void);
+/* Number of visible HIP devices (0 when none; never fails). */
+int vjx_device_count(void);
+
+/* ---------------------------------------------------------------------------
+ * Sparse LU of an n x n CSR matrix with a fixed pattern.
+ * Replaces the reference's XLA FFI linear solver
+ * (vajax/sparse/umfpack_ffi.cpp:189-338 solve, :345-370 dot) and its
+ * pattern-keyed symbolic cache (:140-179): one handle per pattern, symbolic
+ * analysis and pivot order chosen once, numeric refactorisation per call.
+ * ------------------------------------------------------------------------- */
+
+/* Validate the CSR pattern (indptr[0]=0, monotonic, indptr[n]=nnz, columns
+ * strictly increasing in [0,n)) and create a handle.  umfpack_ffi.cpp:234-251. */
+int vjx_pattern_create(int32_t n, int32_t nnz, const int32_t* indptr_host,
+                       const int32_t* indices_host, vjx_pattern** out);
+void vjx_pattern_destroy(vjx_pattern* pat);
+
+/* Ordering + threshold-pivoted symbolic/numeric analysis from host values.
+ * Called implicitly by the first vjx_lu_refactor; call again to re-pivot.
+ * (umfpack_ffi.cpp:266-290, umfpack_di_symbolic + first numeric.) */
+int vjx_lu_analyze(vjx_pattern* pat, const double* data_host);
+
+/* Numeric refactorisation on the GPU with the pivot order fixed by the last
+ * analysis (umfpack_ffi.cpp:292-313, umfpack_di_numeric).  precision is
+ * VJX_PREC_F64 or VJX_PREC_F32.  Returns VJX_WARN_PIVOT when pivot growth
+ * suggests calling vjx_lu_analyze again. */
+int vjx_lu_refactor(vjx_pattern* pat, const double* data_dev, int32_t precision);
+
+/* Solve A x = b with the last factorisation.  For VJX_PREC_F32, refine_steps
+ * FP64 residual corrections r = b - A x use data_dev (the FP64 matrix), as in
+ * vajax/analysis/solver_factories.py:895-922 (refine_steps = 1 there). */
+int vjx_lu_solve(vjx_pattern* pat, const double* data_dev, const double* b_dev, double* x_dev,
+                 int32_t refine_steps);
+
+/* y = A x (umfpack_ffi.cpp:345-370, UmfpackDotF64Impl). */
+int vjx_spmv(vjx_pattern* pat, const double* data_dev, const double* x_dev, double* y_dev);
+
+/* One-shot host-buffer solve with the reference FFI's semantics:
+ * solve(indptr, indices, data, b) -> x (umfpack_ffi.cpp:329-338).  Keeps a
+ * handle cached on the (n, nnz, pattern) key like the reference's global
+ * symbolic cache; vjx_clear_cache() drops it. */
+int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr_host, const int32_t* indices_host,
+                   const double* data_host, const double* b_host, double* x_host);
+void vjx_clear_cache(void);
+
+typedef struct {
+  int32_t n;
+  int32_t nnz_a;
+  int64_t nnz_l;       /* strictly lower */
+  int64_t nnz_u;       /* upper incl. diagonal */
+  int64_t n_updates;   /* multiply-add contributions per refactorisation */
+  double flops;        /* sum_k (|L_k| + 2 |L_k| |U_k|) */
+  int32_t fact_levels, fwd_levels, bwd_levels;
+  int32_t fact_launches, fwd_launches, bwd_launches;
+  int32_t n_dense_rows;
+  int32_t n_offdiag_pivots;
+  double min_pivot_ratio;
+} vjx_lu_info;
+
+int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
+
+/* Host-side copies of the analysis (for tests): permutations, LU pattern. */
+int vjx_lu_get_perm(const vjx_pattern* pat, int32_t* p_host, int32_t* q_host);
+int vjx_lu_get_pattern(const vjx_pattern* pat, int32_t* rowptr_host, int32_t* col_host);
+/* Factorise / solve on the CPU with the same numeric program as the GPU path. */
+int vjx_lu_refactor_cpu(vjx_pattern* pat, const double* data_host, int32_t precision,
+                        double* lu_host);
+int vjx_lu_solve_cpu(vjx_pattern* pat, const double* data_host, int32_t precision,
+                     const double* b_host, double* x_host);
+/* Copy the device LU values (FP64 or FP32 widened) to the host. */
+int vjx_lu_download(vjx_pattern* pat, double* lu_host);
+
+/* ---------------------------------------------------------------------------
+ * Circuit description and system build.
+ * Replaces make_mna_build_system_fn / _build_system_csr_direct /
+ * _build_residual (vajax/analysis/mna_builder.py:258-1078) and the vmapped
+ * device eval contract (openvaf_jax/codegen/function_builder.py:967-996).
+ * ------------------------------------------------------------------------- */
+
+#define VJX_MODEL_RESISTOR 0
+#define VJX_MODEL_CAPACITOR 1
+#define VJX_MODEL_MOSFET 2
+#define VJX_MAX_MODELS 8
+
+/* One model group.  All arrays are column-major [k][count] (structure of
+ * arrays): column k of v_node1 is the first node of branch voltage k, etc. */
+typedef struct {
+  int32_t model_id;
+  int32_t count;
+  const int32_t* v_node1; /* [n_v][count] */
+  const int32_t* v_node2; /* [n_v][count] */
+  const int32_t* jac_dst; /* [n_jac][count] contribution slot, -1 dropped */
+  const int32_t* res_dst; /* [n_res][count] contribution slot, -1 ground */
+  const double* params;   /* [n_par][count] model parameters / init cache */
+} vjx_model_desc;
+
+typedef struct {
+  int32_t n_total;    /* nodes incl. ground */
+  int32_t n_vsources; /* branch unknowns */
+  int32_t n_isources;
+  int32_t nse;        /* CSR nnz */
+  const int32_t* indptr;  /* [n_aug+1] */
+  const int32_t* indices; /* [nse] */
+  /* per-slot constant added after device contributions: diag (gmin+gshunt)
+   * flag and B/B^T value, in the reference's stamping order */
+  const int32_t* slot_diag; /* [nse] 1 for node-row diagonals */
+  const double* slot_b;     /* [nse] +-1 for B/B^T slots else 0 */
+  const int32_t* vs_node_p; /* [m] */
+  const int32_t* vs_node_n; /* [m] */
+  const int32_t* vs_wave;   /* [m] 0 dc, 1 pulse */
+  const double* vs_params;  /* [8][m] dc, val0, val1, rise, fall, width, period, delay */
+  const int32_t* is_wave;   /* [k] */
+  const double* is_params;  /* [8][k] */
+  const int32_t* is_dst;    /* [2][k] residual contribution slots of +I at p, -I at n */
+  int32_t n_jac_contrib;
+  int32_t n_res_contrib;
+  const int32_t* jac_seg;   /* [nse+1] */
+  const int32_t* res_seg;   /* [n_unknowns+1] */
+  /* branch-current KCL terms per node row, in segment_sum order
+   * (all +I_k at p, then all -I_k at n): [n_unknowns+1] ptr, entries = k or ~k */
+  const int32_t* br_ptr;
+  const int32_t* br_ent;
+  const int32_t* conv_mask; /* [n_unknowns] 1 = checked in residual test (may be NULL) */
+  int32_t n_models;
+  vjx_model_desc models[VJX_MAX_MODELS];
+} vjx_circuit_desc;
+
+int vjx_circuit_create(const vjx_circuit_desc* desc, vjx_circuit** out);
+void vjx_circuit_destroy(vjx_circuit* c);
+/* Sparse LU handle bound to the circuit's CSR pattern (owned by the circuit). */
+vjx_pattern* vjx_circuit_pattern(vjx_circuit* c);
+
+/* build_system (mna_builder.py:528-596): evaluate all devices at X and
+ * assemble csr_data, f (augmented), Q, I_vsource and max_res_contrib.
+ * Device pointers; csr_data_dev may be NULL to skip the Jacobian. */
+int vjx_build_system(vjx_circuit* c, const double* X_dev, const double* vsrc_dev,
+                     const double* isrc_dev, const double* Q_prev_dev, const double* dQdt_prev_dev,
+                     const double* Q_prev2_dev, double c0, double c1, double d1, double c2,
+                     double gmin, double gshunt, double* csr_data_dev, double* f_dev, double* Q_dev,
+                     double* I_vsource_dev, double* max_res_contrib_dev);
+
+/* Vectorised device eval for one model group (the openvaf_jax eval_fn
+ * contract restricted to these models): branch voltages in, residuals and
+ * Jacobians out, row-major per instance: res [count][n_res],
+ * jac [count][n_jac].  params as in vjx_model_desc. */
+int vjx_model_eval(int32_t model_id, int32_t count, const double* vbranch_dev /*[n_v][count]*/,
+                   const double* params_dev, double* res_resist_dev, double* res_react_dev,
+                   double* jac_resist_dev, double* jac_react_dev);
+
+/* Evaluate source waveforms at time t into vals_dev. */
+int vjx_eval_sources(vjx_circuit* c, double t, double* vsrc_dev, double* isrc_dev);
+
+/* ---------------------------------------------------------------------------
+ * Transient analysis on the GPU.
+ * Replaces FullMNAStrategy.run and its while-loop body
+ * (vajax/analysis/transient/full_mna.py:780-1196, 1840-2194), the fixed-step
+ * loop (:1198-1441) and nr_solve (vajax/analysis/solver_factories.py:140-727).
+ * ------------------------------------------------------------------------- */
+
+#define VJX_METHOD_BE 0
+#define VJX_METHOD_TRAP 1
+#define VJX_METHOD_GEAR2 2
+
+typedef struct {
+  double t_stop;
+  double dt;          /* initial step as requested (tran_fs applied inside) */
+  int32_t max_steps;  /* output capacity incl. t=0 */
+  int32_t method;     /* VJX_METHOD_* */
+  int32_t fixed_step; /* 1: fixed-step loop */
+  int32_t uic;        /* 1: icmode uic, else DC operating point */
+  int32_t precision;  /* VJX_PREC_* for the LU */
+  int32_t refine_steps;
+  int32_t tran_itl;   /* NR iteration cap */
+  int32_t use_graph;  /* capture the Newton iteration as a hipGraph */
+  /* options (options.py / adaptive.py defaults shown) */
+  double reltol;      /* 1e-3 */
+  double abstol;      /* 1e-12 */
+  double vntol;       /* 1e-6 */
+  double gmin;        /* 1e-12 netlist option: diagonal regularisation */
+  double nr_damping;  /* 1.0 */
+  double lte_ratio;   /* 3.5 */
+  double redo_factor; /* 2.5 */
+  double min_dt;      /* 1e-15 */
+  double max_dt;      /* inf */
+  int32_t tran_minpts;/* 50 */
+  int32_t warmup_steps; /* 2 */
+  int32_t max_order;  /* 2 */
+  double grow_factor; /* 2.0 */
+  double gshunt_init, gshunt_target;
+  int32_t gshunt_steps; /* 5 */
+  double tran_fs;     /* 0.25 */
+  double tran_ft;     /* 0.25 */
+  int32_t max_consecutive_rejects; /* 5 */
+  const double* v_init_host; /* [n_total] initial node voltages for the DC solve */
+} vjx_tran_config;
+
+typedef struct {
+  /* outputs, host buffers sized by the caller: [max_steps], [max_steps][n_out], ... */
+  double* times;
+  double* V_out;     /* node voltages (n_total columns) */
+  double* I_out;     /* vsource currents (max(m,1) columns) */
+  /* statistics */
+  int32_t n_steps;   /* valid rows incl. t=0 */
+  int32_t rejected_steps;
+  int64_t total_nr_iterations;
+  int64_t total_evals;
+  int32_t dc_converged;
+  int32_t dc_iterations;
+  double min_dt_used, max_dt_used;
+  double wall_time;  /* seconds, time loop only (setup and DC excluded) */
+  double dc_time;
+  int32_t repivots;
+} vjx_tran_result;
+
+int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx_tran_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VJX_H */

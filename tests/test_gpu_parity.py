@@ -1,0 +1,237 @@
+"""GPU parity: the HIP path against the oracle on the same seeded inputs.
+
+Bars: bit-exact for the CSR pattern and for resistor/capacitor stamping (same
+FP64 operation order, no FMA contraction); FP64 LU bit-exact against the CPU
+execution of the same numeric program; MOSFET eval within 1e-13 relative
+(device exp/log1p vs numpy); node-voltage waveforms within 1e-9 V absolute
+for FP64 and 1e-6 V for FP32-LU + FP64 refinement.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import mna as omna
+from oracle import models as omod
+from vajax_amd import generators as G
+
+from .helpers import run_gpu, run_oracle, waveform_error
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+
+    assert _t.cuda.is_available()
+    return _t
+
+
+def _dev(torch, a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def _engine(net):
+    from vajax_amd.engine import CircuitEngine
+
+    e = CircuitEngine(net)
+    e.parse()
+    return e
+
+
+@pytest.mark.parametrize("model", ["resistor", "capacitor", "mosfet"])
+def test_model_eval_contract(torch, model):
+    import ctypes as C
+
+    from vajax_amd import _lib as L
+    from vajax_amd.models import param_block
+
+    rng = np.random.default_rng(11)
+    n = 4099
+    if model == "mosfet":
+        typ = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+        p = {"type": typ, "w": rng.uniform(0.2e-6, 2e-6, n), "l": rng.uniform(45e-9, 90e-9, n),
+             "vto": rng.uniform(0.2, 0.5, n), "gamma": rng.uniform(0.2, 0.7, n),
+             "phi": rng.uniform(0.5, 0.9, n), "kp": rng.uniform(5e-5, 3e-4, n),
+             "theta": rng.uniform(0.0, 0.3, n), "cox": np.full(n, 0.02),
+             "cov": np.full(n, 2e-10), "cj": np.full(n, 5e-10)}
+        vb = rng.uniform(-1.5, 1.5, (3, n))
+        vb[:, :8] = [[0.0] * 8, [0.0] * 8, [0.0] * 8]
+        ref = omod.eval_mosfet(vb[0], vb[1], vb[2], omod.mosfet_cache(p))
+        nv, nr, nj = 3, 4, 16
+    else:
+        key = "r" if model == "resistor" else "c"
+        p = {key: rng.uniform(1e2, 1e5, n) if model == "resistor" else rng.uniform(1e-15, 1e-12, n)}
+        vb = rng.uniform(-2, 2, (1, n))
+        fn = omod.eval_resistor if model == "resistor" else omod.eval_capacitor
+        ref = fn(vb[0], p[key])
+        nv, nr, nj = 1, 2, 4
+    par = _dev(torch, param_block(model, p))
+    vbd = _dev(torch, vb)
+    outs = [torch.empty((n, nr), dtype=torch.float64, device="cuda") for _ in range(2)]
+    outs += [torch.empty((n, nj), dtype=torch.float64, device="cuda") for _ in range(2)]
+    rc = L.lib.vjx_model_eval(L.MODEL_IDS[model], n, C.c_void_p(vbd.data_ptr()),
+                              C.c_void_p(par.data_ptr()), *[C.c_void_p(o.data_ptr()) for o in outs])
+    assert rc == 0, L.last_error()
+    for got, want in zip(outs, ref):
+        got = got.cpu().numpy()
+        if model == "mosfet":
+            # entries such as gb = -(gg+gs+gd) cancel: compare against the
+            # largest magnitude of the same instance's output row
+            scale = np.max(np.abs(want), axis=1, keepdims=True)
+            assert np.all(np.abs(got - want) <= 1e-13 * scale + 1e-300), model
+        else:
+            assert np.array_equal(got, want), model
+
+
+def _random_state(cir, rng, spread=1.0):
+    X = np.zeros(cir.n_total + cir.m)
+    X[1:cir.n_total] = rng.uniform(-spread, spread, cir.n_unk)
+    X[cir.n_total:] = rng.uniform(-1e-3, 1e-3, cir.m)
+    return X
+
+
+@pytest.mark.parametrize("which", ["rc", "ring", "dag"])
+def test_build_system_matches_oracle(torch, which):
+    net = {"rc": lambda: G.rc_ladder(200, seed=2),
+           "ring": lambda: G.ring_oscillator(11, seed=2),
+           "dag": lambda: G.gate_dag(300, seed=2)}[which]()
+    cir = omna.setup(net)
+    eng = _engine(net)
+    topo = eng.topology
+    assert np.array_equal(topo.indptr, cir.indptr) and np.array_equal(topo.indices, cir.indices)
+    rng = np.random.default_rng(5)
+    X = _random_state(cir, rng)
+    vs = rng.uniform(0, 1, cir.m)
+    isv = rng.uniform(-1e-4, 1e-4, net.isources.count)
+    Qp, dQ, Q2 = (rng.uniform(-1e-13, 1e-13, cir.n_unk) for _ in range(3))
+    c0, c1, d1, c2, gsh = 1.5e12, -2e12, 0.0, 0.5e12, 1e-9
+    want = omna.build_system(cir, X, vs, isv, Qp, c0, gsh, c1, d1, dQ, c2, Q2)
+    out = eng.device.build_system(_dev(torch, X), _dev(torch, vs),
+                                  _dev(torch, isv) if len(isv) else None,
+                                  _dev(torch, Qp), _dev(torch, dQ), _dev(torch, Q2),
+                                  c0=c0, c1=c1, d1=d1, c2=c2, gmin=1e-12, gshunt=gsh)
+    names = ["csr_data", "f", "Q", "I_vsource", "max_res_contrib"]
+    for name, w in zip(names, want):
+        g = out[name].cpu().numpy()[: len(w)]
+        if which == "rc":
+            assert np.array_equal(g, w), name
+        else:
+            scale = np.max(np.abs(w)) if len(w) else 1.0
+            assert np.max(np.abs(g - w)) <= 1e-12 * scale, name
+
+
+def test_lu_gpu_matches_cpu_program(torch):
+    import scipy.sparse as sp
+
+    net = G.gate_dag(2000, seed=4)
+    cir = omna.setup(net)
+    rng = np.random.default_rng(1)
+    X = _random_state(cir, rng, 0.5)
+    csr, *_ = omna.build_system(cir, X, np.zeros(cir.m), np.zeros(0), np.zeros(cir.n_unk), 1e13)
+    eng = _engine(net)
+    lu = eng.device.lu
+    lu.analyze(csr)
+    d = _dev(torch, csr)
+    assert lu.refactor(d, 0) == 0
+    got = lu.download()
+    want = lu.refactor_cpu(csr, 0)
+    assert np.array_equal(got, want)
+    A = sp.csr_matrix((csr, cir.indices, cir.indptr), shape=(cir.n_aug, cir.n_aug))
+    b = rng.standard_normal(cir.n_aug)
+    bd = _dev(torch, b)
+    x = torch.empty(cir.n_aug, dtype=torch.float64, device="cuda")
+    lu.solve(d, bd, x, 0)
+    xg = x.cpu().numpy()
+    assert np.array_equal(xg, lu.solve_cpu(csr, b, 0))
+    assert np.linalg.norm(A @ xg - b) / np.linalg.norm(b) < 1e-12
+    # FP32 factorisation + one FP64 refinement step
+    assert lu.refactor(d, 1) == 0
+    lu.solve(d, bd, x, 1)
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(A @ xg - b) / np.linalg.norm(b) < 1e-9
+
+
+def test_solve_host_matches_scipy():
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from vajax_amd.sparse import solve
+
+    net = G.ring_oscillator(21, seed=1)
+    cir = omna.setup(net)
+    rng = np.random.default_rng(3)
+    csr, *_ = omna.build_system(cir, _random_state(cir, rng), np.ones(cir.m), np.zeros(1),
+                                np.zeros(cir.n_unk), 1e12)
+    b = rng.standard_normal(cir.n_aug)
+    x = solve(cir.indptr, cir.indices, csr, b)
+    A = sp.csr_matrix((csr, cir.indices, cir.indptr), shape=(cir.n_aug, cir.n_aug)).tocsc()
+    xs = spla.spsolve(A, b)
+    assert np.allclose(x, xs, rtol=1e-10, atol=1e-14)
+
+
+def test_transient_rc_fixed_step():
+    net = G.rc_ladder(200, seed=0)
+    _, ref = run_oracle(net, 1e-9, 1e-12, fixed=True, tran_fs=1.0)
+    _, res = run_gpu(net, 1e-9, 1e-12, fixed=True, tran_fs=1.0)
+    assert res.stats["n_steps"] == len(ref.times)
+    assert np.array_equal(res.times, ref.times)
+    V = res.stats["V_out"]
+    assert np.max(np.abs(V - ref.V)) <= 1e-12
+    assert res.stats["total_nr_iterations"] == ref.stats["total_nr_iterations"]
+
+
+def test_transient_rc_adaptive():
+    net = G.rc_ladder(200, seed=0)
+    _, ref = run_oracle(net, 10e-9, 1e-12)
+    _, res = run_gpu(net, 10e-9, 1e-12)
+    assert res.stats["n_steps"] == len(ref.times)
+    assert res.stats["rejected_steps"] == ref.stats["rejected_steps"]
+    assert np.allclose(res.times, ref.times, rtol=1e-12, atol=0)
+    assert np.max(np.abs(res.stats["V_out"] - ref.V)) <= 1e-9
+
+
+def test_transient_ring_uic():
+    net = G.ring_oscillator(11, seed=0)
+    _, ref = run_oracle(net, 0.3e-9, 1e-12)
+    _, res = run_gpu(net, 0.3e-9, 1e-12)
+    err = waveform_error(ref.times, ref.V, res.times, res.stats["V_out"])
+    assert err <= 1e-6, err
+    assert abs(res.stats["n_steps"] - len(ref.times)) <= 2
+
+
+def test_transient_gate_dag_fixed_step_fp64_and_mixed():
+    """Fixed 1 ps grid through the supply ramp: same time points by construction."""
+    net = G.gate_dag(200, seed=7)
+    _, ref = run_oracle(net, 0.15e-9, 1e-12, fixed=True, tran_fs=1.0)
+    _, r64 = run_gpu(net, 0.15e-9, 1e-12, fixed=True, tran_fs=1.0)
+    assert np.array_equal(r64.times, ref.times)
+    e64 = float(np.max(np.abs(r64.stats["V_out"] - ref.V)))
+    assert e64 <= 1e-9, e64
+    _, r32 = run_gpu(net, 0.15e-9, 1e-12, fixed=True, tran_fs=1.0, precision="f32")
+    e32 = float(np.max(np.abs(r32.stats["V_out"] - ref.V)))
+    assert e32 <= 1e-6, e32
+
+
+def test_transient_gate_dag_adaptive():
+    """Adaptive BDF2 with LTE rejections.
+
+    Accept/reject and the Newton stopping test are discontinuous in the
+    solution, so two correct implementations whose LU round differently
+    follow the same step sequence only until the first decision that lies
+    within rounding of its threshold.  Bars: the common prefix of accepted
+    time points covers at least 30 % of the run with waveforms within
+    1e-9 V on it, and the whole waveforms agree to the LTE tolerance scale
+    (reltol * lte_ratio * 1 V = 3.5e-3 V) after interpolation."""
+    net = G.gate_dag(200, seed=7)
+    _, ref = run_oracle(net, 0.2e-9, 1e-12)
+    _, res = run_gpu(net, 0.2e-9, 1e-12)
+    n = min(len(ref.times), len(res.times))
+    same = np.isclose(res.times[:n], ref.times[:n], rtol=1e-9, atol=0)
+    k = n if same.all() else int(np.argmin(same))
+    assert k >= 0.3 * len(ref.times), (k, len(ref.times), len(res.times))
+    err = float(np.max(np.abs(res.stats["V_out"][:k] - ref.V[:k])))
+    assert err <= 1e-9, err
+    e_all = waveform_error(ref.times, ref.V, res.times, res.stats["V_out"])
+    assert e_all <= 3.5e-3, e_all
+    assert res.times[-1] == pytest.approx(0.2e-9, rel=1e-12)

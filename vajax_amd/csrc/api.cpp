@@ -1,0 +1,304 @@
+// C-ABI layer: sparse LU handles, circuit handles, system build.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "circuit.h"
+#include "internal.h"
+
+namespace vjx {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+int pattern_upload(vjx_pattern* pat);
+
+}  // namespace vjx
+
+using namespace vjx;
+
+extern "C" {
+
+int vjx_version(void) { return 100; }
+
+const char* vjx_last_error(void) { return g_err.c_str(); }
+
+int vjx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int vjx_pattern_create(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* indices,
+                       vjx_pattern** out) {
+  if (!out || !indptr || (nnz > 0 && !indices)) {
+    set_error("vjx_pattern_create: null argument");
+    return VJX_ERR_ARG;
+  }
+  std::string err;
+  if (!csr_validate(n, nnz, indptr, indices, err)) {
+    set_error("vjx_pattern_create: " + err);
+    return VJX_ERR_ARG;
+  }
+  auto* p = new vjx_pattern();
+  p->n = n;
+  p->nnz = nnz;
+  p->indptr.assign(indptr, indptr + n + 1);
+  p->indices.assign(indices, indices + nnz);
+  *out = p;
+  return VJX_OK;
+}
+
+void vjx_pattern_destroy(vjx_pattern* pat) {
+  if (!pat) return;
+  if (pat->stream) (void)hipStreamDestroy(pat->stream);
+  delete pat;
+}
+
+int vjx_lu_analyze(vjx_pattern* pat, const double* data_host) {
+  if (!pat || (!data_host && pat->nnz)) {
+    set_error("vjx_lu_analyze: null argument");
+    return VJX_ERR_ARG;
+  }
+  std::string err;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!lu_analyze(pat->n, pat->indptr.data(), pat->indices.data(), data_host, pat->opts,
+                  pat->lu.sym, err)) {
+    set_error("vjx_lu_analyze: " + err);
+    pat->lu.analyzed = false;
+    return err.find("singular") != std::string::npos ? VJX_ERR_SINGULAR : VJX_ERR_ARG;
+  }
+  pat->analyze_seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  pat->lu.analyzed = true;
+  pat->lu.factored_prec = -1;
+  pat->uploaded = false;
+  pat->n_analyses++;
+  return VJX_OK;
+}
+
+int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
+  if (!pat || !o) {
+    set_error("vjx_lu_get_info: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (!pat->lu.analyzed) {
+    set_error("vjx_lu_get_info: no analysis yet");
+    return VJX_ERR_STATE;
+  }
+  const LuSymbolic& s = pat->lu.sym;
+  o->n = s.n;
+  o->nnz_a = s.nnz_a;
+  o->nnz_l = s.nnz_l;
+  o->nnz_u = s.nnz_u;
+  o->n_updates = s.n_updates;
+  o->flops = s.flops;
+  o->fact_levels = (int32_t)s.fact.lvl_ptr.size() - 1;
+  o->fwd_levels = (int32_t)s.fwd.lvl_ptr.size() - 1;
+  o->bwd_levels = (int32_t)s.bwd.lvl_ptr.size() - 1;
+  o->fact_launches = (int32_t)s.fact.groups.size();
+  o->fwd_launches = (int32_t)s.fwd.groups.size();
+  o->bwd_launches = (int32_t)s.bwd.groups.size();
+  o->n_dense_rows = s.n_dense;
+  o->n_offdiag_pivots = s.n_offdiag_pivots;
+  o->min_pivot_ratio = s.min_pivot_ratio;
+  return VJX_OK;
+}
+
+int vjx_lu_get_perm(const vjx_pattern* pat, int32_t* p, int32_t* q) {
+  if (!pat || !pat->lu.analyzed) {
+    set_error("vjx_lu_get_perm: no analysis yet");
+    return VJX_ERR_STATE;
+  }
+  if (p) std::memcpy(p, pat->lu.sym.p.data(), sizeof(int32_t) * pat->n);
+  if (q) std::memcpy(q, pat->lu.sym.q.data(), sizeof(int32_t) * pat->n);
+  return VJX_OK;
+}
+
+int vjx_lu_get_pattern(const vjx_pattern* pat, int32_t* rowptr, int32_t* col) {
+  if (!pat || !pat->lu.analyzed) {
+    set_error("vjx_lu_get_pattern: no analysis yet");
+    return VJX_ERR_STATE;
+  }
+  const auto& s = pat->lu.sym;
+  if (rowptr) std::memcpy(rowptr, s.rowptr.data(), sizeof(int32_t) * (s.n + 1));
+  if (col) std::memcpy(col, s.col.data(), sizeof(int32_t) * s.col.size());
+  return VJX_OK;
+}
+
+int vjx_lu_refactor_cpu(vjx_pattern* pat, const double* data, int32_t prec, double* lu_out) {
+  if (!pat || !data) {
+    set_error("vjx_lu_refactor_cpu: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (!pat->lu.analyzed) {
+    int rc = vjx_lu_analyze(pat, data);
+    if (rc < 0) return rc;
+  }
+  const auto& s = pat->lu.sym;
+  const size_t nnz = s.col.size();
+  if (prec == VJX_PREC_F32) {
+    std::vector<float> lu(nnz);
+    lu_refactor_host<float>(s, data, lu.data());
+    if (lu_out)
+      for (size_t e = 0; e < nnz; ++e) lu_out[e] = lu[e];
+  } else {
+    std::vector<double> lu(nnz);
+    lu_refactor_host<double>(s, data, lu.data());
+    if (lu_out) std::memcpy(lu_out, lu.data(), nnz * sizeof(double));
+  }
+  return VJX_OK;
+}
+
+int vjx_lu_solve_cpu(vjx_pattern* pat, const double* data, int32_t prec, const double* b,
+                     double* x) {
+  if (!pat || !data || !b || !x) {
+    set_error("vjx_lu_solve_cpu: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (!pat->lu.analyzed) {
+    int rc = vjx_lu_analyze(pat, data);
+    if (rc < 0) return rc;
+  }
+  const auto& s = pat->lu.sym;
+  const size_t nnz = s.col.size();
+  if (prec == VJX_PREC_F32) {
+    std::vector<float> lu(nnz);
+    lu_refactor_host<float>(s, data, lu.data());
+    lu_solve_host<float>(s, lu.data(), b, x);
+  } else {
+    std::vector<double> lu(nnz);
+    lu_refactor_host<double>(s, data, lu.data());
+    lu_solve_host<double>(s, lu.data(), b, x);
+  }
+  return VJX_OK;
+}
+
+int vjx_lu_refactor(vjx_pattern* pat, const double* data_dev, int32_t prec) {
+  if (!pat || (!data_dev && pat->nnz)) {
+    set_error("vjx_lu_refactor: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (prec != VJX_PREC_F64 && prec != VJX_PREC_F32) {
+    set_error("vjx_lu_refactor: precision must be VJX_PREC_F64 or VJX_PREC_F32");
+    return VJX_ERR_ARG;
+  }
+  int rc = pattern_ensure_analyzed(pat, data_dev);
+  if (rc < 0) return rc;
+  rc = pattern_refactor_async(pat, data_dev, prec, true);
+  if (rc < 0) return rc;
+  VJX_TRY_HIP(hipStreamSynchronize(pat->stream));
+  return pattern_check_pivots(pat);
+}
+
+int vjx_lu_solve(vjx_pattern* pat, const double* data_dev, const double* b_dev, double* x_dev,
+                 int32_t refine_steps) {
+  if (!pat || !b_dev || !x_dev) {
+    set_error("vjx_lu_solve: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (pat->lu.factored_prec < 0) {
+    set_error("vjx_lu_solve: call vjx_lu_refactor first");
+    return VJX_ERR_STATE;
+  }
+  if (pat->lu.factored_prec == VJX_PREC_F32 && refine_steps > 0 && !data_dev) {
+    set_error("vjx_lu_solve: refinement needs the FP64 matrix");
+    return VJX_ERR_ARG;
+  }
+  int rc = pattern_solve_async(pat, data_dev, b_dev, +1.0, x_dev, refine_steps);
+  if (rc < 0) return rc;
+  VJX_TRY_HIP(hipStreamSynchronize(pat->stream));
+  return VJX_OK;
+}
+
+int vjx_spmv(vjx_pattern* pat, const double* data_dev, const double* x_dev, double* y_dev) {
+  if (!pat || !data_dev || !x_dev || !y_dev) {
+    set_error("vjx_spmv: null argument");
+    return VJX_ERR_ARG;
+  }
+  int rc = pattern_upload_csr(pat);
+  if (rc < 0) return rc;
+  launch_spmv(pat->n, pat->d_indptr.p, pat->d_indices.p, data_dev, x_dev, nullptr, 1.0, y_dev,
+              pat->stream);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(pat->stream));
+  return VJX_OK;
+}
+
+int vjx_lu_download(vjx_pattern* pat, double* lu_host) {
+  if (!pat || !lu_host || pat->lu.factored_prec < 0) {
+    set_error("vjx_lu_download: nothing factorised");
+    return VJX_ERR_STATE;
+  }
+  const size_t nnz = pat->lu.sym.col.size();
+  if (pat->lu.factored_prec == VJX_PREC_F32) {
+    std::vector<float> tmp(nnz);
+    VJX_TRY_HIP(hipMemcpy(tmp.data(), pat->lu.lu32.p, nnz * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < nnz; ++e) lu_host[e] = tmp[e];
+  } else {
+    VJX_TRY_HIP(hipMemcpy(lu_host, pat->lu.lu64.p, nnz * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return VJX_OK;
+}
+
+// ---- host-buffer solve with a pattern-keyed cache ----
+static std::mutex g_cache_mu;
+static std::map<std::tuple<int, int, std::vector<int32_t>, std::vector<int32_t>>,
+                std::unique_ptr<vjx_pattern, void (*)(vjx_pattern*)>>
+    g_cache;
+
+int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* indices,
+                   const double* data, const double* b, double* x) {
+  if (!indptr || !indices || !data || !b || !x) {
+    set_error("vjx_solve_host: null argument");
+    return VJX_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  auto key = std::make_tuple((int)n, (int)nnz, std::vector<int32_t>(indptr, indptr + n + 1),
+                             std::vector<int32_t>(indices, indices + nnz));
+  auto it = g_cache.find(key);
+  vjx_pattern* pat = nullptr;
+  if (it == g_cache.end()) {
+    int rc = vjx_pattern_create(n, nnz, indptr, indices, &pat);
+    if (rc < 0) return rc;
+    rc = vjx_lu_analyze(pat, data);
+    if (rc < 0) {
+      vjx_pattern_destroy(pat);
+      return rc;
+    }
+    g_cache.emplace(key, std::unique_ptr<vjx_pattern, void (*)(vjx_pattern*)>(pat, vjx_pattern_destroy));
+  } else {
+    pat = it->second.get();
+  }
+  DBuf<double> d_data, d_b, d_x;
+  VJX_TRY_HIP(d_data.upload(data, nnz));
+  VJX_TRY_HIP(d_b.upload(b, n));
+  VJX_TRY_HIP(d_x.alloc(n));
+  int rc = vjx_lu_refactor(pat, d_data.p, VJX_PREC_F64);
+  if (rc < 0) return rc;
+  if (rc == VJX_WARN_PIVOT) {
+    // pivot order from the first matrix no longer fits: re-pivot on this one
+    rc = vjx_lu_analyze(pat, data);
+    if (rc < 0) return rc;
+    rc = vjx_lu_refactor(pat, d_data.p, VJX_PREC_F64);
+    if (rc < 0) return rc;
+  }
+  int rc2 = vjx_lu_solve(pat, d_data.p, d_b.p, d_x.p, 0);
+  if (rc2 < 0) return rc2;
+  VJX_TRY_HIP(hipMemcpy(x, d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  return rc;
+}
+
+void vjx_clear_cache(void) {
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  g_cache.clear();
+}
+
+}  // extern "C"

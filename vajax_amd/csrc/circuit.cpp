@@ -1,0 +1,236 @@
+// Circuit handle: device copies of the topology, and the system build
+// (device eval + stamping + residual) behind vjx_build_system.
+#include <cstring>
+
+#include "circuit.h"
+
+vjx_circuit::~vjx_circuit() {
+  if (h_sc) (void)hipHostFree(h_sc);
+  if (pat) vjx_pattern_destroy(pat);
+}
+
+namespace vjx {
+
+void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
+                         const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
+                         const double* res_floor, double* csr, double* f, double* Q, double* mrc,
+                         double* I_vs, DevStatus* st, hipStream_t s) {
+  EvalOut out{csr ? c->jac_contrib.p : nullptr, c->rf.p, c->rq.p};
+  for (auto& m : c->models) launch_eval(*m, X, c->d_sc.p, out, s);
+  launch_isrc_contrib(c->k, isrc, c->is_dst.p, c->rf.p, c->rq.p, s);
+  if (csr)
+    launch_reduce_jac(c->nse, c->jac_seg.p, c->jac_contrib.p, c->slot_diag.p, c->slot_b.p,
+                      c->d_sc.p, csr, s);
+  ResidualArgs a;
+  a.n_unk = c->n_unk;
+  a.m = c->m;
+  a.n_total = c->n_total;
+  a.res_seg = c->res_seg.p;
+  a.rf = c->rf.p;
+  a.rq = c->rq.p;
+  a.br_ptr = c->br_ptr.p;
+  a.br_ent = c->br_ent.p;
+  a.vs_p = c->vs_p.p;
+  a.vs_n = c->vs_n.p;
+  a.conv_mask = c->has_conv_mask ? c->conv_mask.p : nullptr;
+  a.X = X;
+  a.vsrc = vsrc;
+  a.Q_prev = Q_prev;
+  a.dQdt_prev = dQdt_prev;
+  a.Q_prev2 = Q_prev2;
+  a.res_floor = res_floor;
+  a.sc = c->d_sc.p;
+  a.f = f;
+  a.Q = Q;
+  a.mrc = mrc;
+  a.I_vs = I_vs;
+  a.st = st;
+  launch_residual(a, s);
+}
+
+}  // namespace vjx
+
+using namespace vjx;
+
+extern "C" {
+
+int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
+  if (!d || !out) {
+    set_error("vjx_circuit_create: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (d->n_total < 1 || d->n_vsources < 0 || d->n_isources < 0 || d->nse < 0 ||
+      d->n_models < 0 || d->n_models > VJX_MAX_MODELS) {
+    set_error("vjx_circuit_create: invalid sizes");
+    return VJX_ERR_ARG;
+  }
+  std::unique_ptr<vjx_circuit> c(new vjx_circuit());
+  c->n_total = d->n_total;
+  c->n_unk = d->n_total - 1;
+  c->m = d->n_vsources;
+  c->k = d->n_isources;
+  c->n_aug = c->n_unk + c->m;
+  c->nse = d->nse;
+  int rc = vjx_pattern_create(c->n_aug, c->nse, d->indptr, d->indices, &c->pat);
+  if (rc < 0) return rc;
+  rc = pattern_upload_csr(c->pat);
+  if (rc < 0) return rc;
+  const int nu = c->n_unk, m = c->m, k = c->k;
+  if (d->jac_seg[0] != 0 || d->jac_seg[c->nse] != d->n_jac_contrib || d->res_seg[0] != 0 ||
+      d->res_seg[nu] != d->n_res_contrib) {
+    set_error("vjx_circuit_create: reduction segments do not match contribution counts");
+    return VJX_ERR_ARG;
+  }
+  VJX_TRY_HIP(c->slot_diag.upload(d->slot_diag, c->nse));
+  VJX_TRY_HIP(c->slot_b.upload(d->slot_b, c->nse));
+  VJX_TRY_HIP(c->jac_seg.upload(d->jac_seg, c->nse + 1));
+  VJX_TRY_HIP(c->res_seg.upload(d->res_seg, nu + 1));
+  VJX_TRY_HIP(c->br_ptr.upload(d->br_ptr, nu + 1));
+  VJX_TRY_HIP(c->br_ent.upload(d->br_ent, d->br_ptr[nu]));
+  VJX_TRY_HIP(c->vs_p.upload(d->vs_node_p, m));
+  VJX_TRY_HIP(c->vs_n.upload(d->vs_node_n, m));
+  VJX_TRY_HIP(c->vs_wave.upload(d->vs_wave, m));
+  VJX_TRY_HIP(c->vs_par.upload(d->vs_params, (size_t)8 * m));
+  VJX_TRY_HIP(c->is_wave.upload(d->is_wave, k));
+  VJX_TRY_HIP(c->is_par.upload(d->is_params, (size_t)8 * k));
+  VJX_TRY_HIP(c->is_dst.upload(d->is_dst, (size_t)2 * k));
+  c->h_vs_wave.assign(d->vs_wave, d->vs_wave + m);
+  c->h_is_wave.assign(d->is_wave, d->is_wave + k);
+  c->h_vs_par.assign(d->vs_params, d->vs_params + (size_t)8 * m);
+  c->h_is_par.assign(d->is_params, d->is_params + (size_t)8 * k);
+  if (d->conv_mask) {
+    VJX_TRY_HIP(c->conv_mask.upload(d->conv_mask, nu));
+    c->has_conv_mask = true;
+  }
+  c->n_jac_contrib = d->n_jac_contrib;
+  c->n_res_contrib = d->n_res_contrib;
+  VJX_TRY_HIP(c->jac_contrib.alloc(d->n_jac_contrib));
+  VJX_TRY_HIP(c->rf.alloc(d->n_res_contrib));
+  VJX_TRY_HIP(c->rq.alloc(d->n_res_contrib));
+  for (int i = 0; i < d->n_models; ++i) {
+    const vjx_model_desc& md = d->models[i];
+    if (md.model_id < 0 || md.model_id > VJX_MODEL_MOSFET || md.count < 0) {
+      set_error("vjx_circuit_create: unknown model id");
+      return VJX_ERR_ARG;
+    }
+    auto dm = std::make_unique<DevModel>();
+    dm->model_id = md.model_id;
+    dm->count = md.count;
+    dm->n_v = model_n_v(md.model_id);
+    dm->n_jac = model_n_jac(md.model_id);
+    dm->n_res = model_n_res(md.model_id);
+    dm->n_par = model_n_par(md.model_id);
+    const size_t N = (size_t)md.count;
+    // bounds checks on every index the kernels will follow
+    for (size_t e = 0; e < N * dm->n_v; ++e)
+      if (md.v_node1[e] < 0 || md.v_node1[e] >= c->n_total || md.v_node2[e] < 0 ||
+          md.v_node2[e] >= c->n_total) {
+        set_error("vjx_circuit_create: branch-voltage node index out of range");
+        return VJX_ERR_ARG;
+      }
+    for (size_t e = 0; e < N * dm->n_jac; ++e)
+      if (md.jac_dst[e] < -1 || md.jac_dst[e] >= d->n_jac_contrib) {
+        set_error("vjx_circuit_create: Jacobian contribution slot out of range");
+        return VJX_ERR_ARG;
+      }
+    for (size_t e = 0; e < N * dm->n_res; ++e)
+      if (md.res_dst[e] < -1 || md.res_dst[e] >= d->n_res_contrib) {
+        set_error("vjx_circuit_create: residual contribution slot out of range");
+        return VJX_ERR_ARG;
+      }
+    VJX_TRY_HIP(dm->vn1.upload(md.v_node1, N * dm->n_v));
+    VJX_TRY_HIP(dm->vn2.upload(md.v_node2, N * dm->n_v));
+    VJX_TRY_HIP(dm->jac_dst.upload(md.jac_dst, N * dm->n_jac));
+    VJX_TRY_HIP(dm->res_dst.upload(md.res_dst, N * dm->n_res));
+    VJX_TRY_HIP(dm->par.upload(md.params, N * dm->n_par));
+    c->models.push_back(std::move(dm));
+  }
+  for (int e = 0; e < 2 * k; ++e)
+    if (d->is_dst[e] < -1 || d->is_dst[e] >= d->n_res_contrib) {
+      set_error("vjx_circuit_create: current-source slot out of range");
+      return VJX_ERR_ARG;
+    }
+  for (int i = 0; i < m; ++i)
+    if (d->vs_node_p[i] < 0 || d->vs_node_p[i] >= c->n_total || d->vs_node_n[i] < 0 ||
+        d->vs_node_n[i] >= c->n_total) {
+      set_error("vjx_circuit_create: voltage-source node out of range");
+      return VJX_ERR_ARG;
+    }
+  for (int e = 0; e < d->br_ptr[nu]; ++e) {
+    int kk = d->br_ent[e] >= 0 ? d->br_ent[e] : ~d->br_ent[e];
+    if (kk >= m) {
+      set_error("vjx_circuit_create: branch entry out of range");
+      return VJX_ERR_ARG;
+    }
+  }
+  VJX_TRY_HIP(c->d_sc.alloc(1));
+  VJX_TRY_HIP(hipHostMalloc((void**)&c->h_sc, sizeof(DevScalars), hipHostMallocDefault));
+  std::memset(c->h_sc, 0, sizeof(DevScalars));
+  *out = c.release();
+  return VJX_OK;
+}
+
+void vjx_circuit_destroy(vjx_circuit* c) { delete c; }
+
+vjx_pattern* vjx_circuit_pattern(vjx_circuit* c) { return c ? c->pat : nullptr; }
+
+int vjx_build_system(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
+                     const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
+                     double c0, double c1, double d1, double c2, double gmin, double gshunt,
+                     double* csr, double* f, double* Q, double* I_vs, double* mrc) {
+  if (!c || !X || !f || (c->m && (!vsrc || !I_vs)) || (c->k && !isrc)) {
+    set_error("vjx_build_system: null argument");
+    return VJX_ERR_ARG;
+  }
+  hipStream_t s = c->pat->stream;
+  DevScalars* h = c->h_sc;
+  h->c0 = c0;
+  h->c1 = c1;
+  h->c2 = c2;
+  h->d1 = d1;
+  h->gmin = gmin;
+  h->gshunt = gshunt;
+  h->reltol = 1e-3;
+  h->vntol = 1e-6;
+  h->abstol = 1e-12;
+  h->nr_damping = 1.0;
+  h->max_step = 1e30;
+  h->iteration = 0;
+  VJX_TRY_HIP(hipMemcpyAsync(c->d_sc.p, h, sizeof(DevScalars), hipMemcpyHostToDevice, s));
+  circuit_build_async(c, X, vsrc, isrc, Q_prev, dQdt_prev, Q_prev2, nullptr, csr, f, Q, mrc, I_vs,
+                      nullptr, s);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(s));
+  return VJX_OK;
+}
+
+int vjx_model_eval(int32_t model_id, int32_t count, const double* vb, const double* par,
+                   double* rr, double* rc, double* jr, double* jc) {
+  if (model_id < 0 || model_id > VJX_MODEL_MOSFET || count < 0) {
+    set_error("vjx_model_eval: bad model id or count");
+    return VJX_ERR_ARG;
+  }
+  if (count && (!vb || !par || !rr || !rc || !jr || !jc)) {
+    set_error("vjx_model_eval: null argument");
+    return VJX_ERR_ARG;
+  }
+  launch_model_eval_raw(model_id, count, vb, par, rr, rc, jr, jc, 0);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipDeviceSynchronize());
+  return VJX_OK;
+}
+
+int vjx_eval_sources(vjx_circuit* c, double t, double* vsrc, double* isrc) {
+  if (!c) {
+    set_error("vjx_eval_sources: null circuit");
+    return VJX_ERR_ARG;
+  }
+  hipStream_t s = c->pat->stream;
+  if (c->m && vsrc) launch_sources(c->m, c->vs_wave.p, c->vs_par.p, t, vsrc, s);
+  if (c->k && isrc) launch_sources(c->k, c->is_wave.p, c->is_par.p, t, isrc, s);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(s));
+  return VJX_OK;
+}
+
+}  // extern "C"

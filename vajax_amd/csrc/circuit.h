@@ -1,0 +1,68 @@
+// Handle types behind the C ABI and the helpers shared by api.cpp,
+// circuit.cpp and transient.cpp.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "internal.h"
+
+struct vjx_pattern {
+  int n = 0;
+  int nnz = 0;
+  std::vector<int32_t> indptr, indices;
+  vjx::AnalyzeOptions opts;
+  vjx::DevLU lu;
+  bool uploaded = false;
+  vjx::DBuf<int32_t> d_indptr, d_indices;
+  hipStream_t stream = nullptr;
+  vjx::DBuf<vjx::DevStatus> d_status;
+  vjx::DevStatus* h_status = nullptr;  // pinned
+  double analyze_seconds = 0.0;
+  int n_analyses = 0;
+  ~vjx_pattern() {
+    if (h_status) (void)hipHostFree(h_status);
+  }
+};
+
+struct vjx_circuit {
+  int n_total = 0, n_unk = 0, m = 0, k = 0, n_aug = 0, nse = 0;
+  std::vector<std::unique_ptr<vjx::DevModel>> models;
+  vjx::DBuf<int32_t> slot_diag, jac_seg, res_seg, br_ptr, br_ent, vs_p, vs_n, vs_wave, is_wave,
+      is_dst, conv_mask;
+  vjx::DBuf<double> slot_b, vs_par, is_par;
+  std::vector<int32_t> h_vs_wave, h_is_wave;
+  std::vector<double> h_vs_par, h_is_par;
+  int n_jac_contrib = 0, n_res_contrib = 0;
+  vjx::DBuf<double> jac_contrib, rf, rq;
+  vjx_pattern* pat = nullptr;
+  vjx::DBuf<vjx::DevScalars> d_sc;
+  vjx::DevScalars* h_sc = nullptr;  // pinned
+  bool has_conv_mask = false;
+  ~vjx_circuit();
+};
+
+namespace vjx {
+
+int pattern_stream(vjx_pattern* pat);
+int pattern_upload_csr(vjx_pattern* pat);
+int pattern_ensure_analyzed(vjx_pattern* pat, const double* data_dev);
+int pattern_upload(vjx_pattern* pat);
+int pattern_refactor_async(vjx_pattern* pat, const double* data_dev, int prec, bool reset_status);
+int pattern_check_pivots(vjx_pattern* pat);
+// Solve A x = scale*rhs (scale = +1 or -1) with the current factors; for FP32
+// factors run refine_steps FP64 residual corrections.
+int pattern_solve_async(vjx_pattern* pat, const double* data_dev, const double* rhs, double scale,
+                        double* x, int refine_steps);
+
+void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const double* data,
+                 const double* x, const double* base, double sgn, double* out, hipStream_t s);
+void launch_status_reset(vjx::DevStatus* st, hipStream_t s);
+
+// Enqueue one system build: eval, stamping reductions, residual.
+void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
+                         const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
+                         const double* res_floor, double* csr, double* f, double* Q, double* mrc,
+                         double* I_vs, vjx::DevStatus* st, hipStream_t s);
+
+}  // namespace vjx

@@ -1,0 +1,161 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vjx.h"
+#include "symbolic.h"
+
+namespace vjx {
+
+void set_error(const std::string& msg);
+
+#define VJX_TRY_HIP(call)                                                       \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      ::vjx::set_error(std::string(#call) + " failed: " + hipGetErrorString(e_)); \
+      return VJX_ERR_HIP;                                                       \
+    }                                                                           \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, count * sizeof(T));
+  }
+  hipError_t upload(const T* host, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice);
+  }
+  hipError_t upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
+};
+
+// Per-NR-iteration scalars read by kernels from device memory, so captured
+// graphs replay with current values.
+struct DevScalars {
+  double c0, c1, c2, d1;
+  double gmin, gshunt;
+  double reltol, vntol, abstol, nr_damping, max_step;
+  int32_t iteration;
+  int32_t pad;
+};
+
+// Flags and reductions written by kernels (atomics on integer bit patterns of
+// non-negative doubles: order-independent, hence deterministic).
+struct DevStatus {
+  unsigned long long maxf_bits;
+  unsigned long long maxdv_bits;
+  unsigned long long maxdelta_bits;
+  unsigned long long maxl_bits;
+  unsigned int res_fail;
+  unsigned int delta_fail;
+  unsigned int nonfinite;
+  unsigned int pad;
+};
+
+struct DevModel {
+  int32_t model_id = 0;
+  int32_t count = 0;
+  int32_t n_v = 0, n_jac = 0, n_res = 0, n_par = 0;
+  DBuf<int32_t> vn1, vn2, jac_dst, res_dst;
+  DBuf<double> par;
+};
+
+// Level program on the device: item ids per level plus host-side launch plan.
+struct DevSchedule {
+  DBuf<int32_t> items;
+  DBuf<int32_t> lvl_dev;         // device copy of lvl_ptr
+  std::vector<int32_t> lvl_ptr;  // host copy
+  std::vector<LaunchGroup> groups;
+};
+
+struct DevLU {
+  LuSymbolic sym;
+  bool analyzed = false;
+  int factored_prec = -1;  // precision of the current device factors
+  DBuf<int32_t> rowptr, col, diag, src, tgt_pos, tgt_seg, upd_l, upd_u, upd_d, l_slot, l_diag, p, q;
+  DevSchedule fact, fwd, bwd;
+  DBuf<double> lu64;
+  DBuf<float> lu32;
+  DBuf<double> y64, z64, r64, x64;
+  DBuf<float> y32, z32;
+  int narrow_block = 512;
+  int wide_block = 256;
+};
+
+// ----- kernel launchers (kernels.hip) -----
+struct EvalOut {
+  double* jac;  // contribution arrays (sorted slots), may be null
+  double* rf;
+  double* rq;
+};
+
+void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalOut out,
+                 hipStream_t s);
+void launch_isrc_contrib(int k, const double* isrc, const int32_t* is_dst, double* rf, double* rq,
+                         hipStream_t s);
+void launch_reduce_jac(int nse, const int32_t* seg, const double* contrib, const int32_t* slot_diag,
+                       const double* slot_b, const DevScalars* sc, double* csr, hipStream_t s);
+struct ResidualArgs {
+  int n_unk, m, n_total;
+  const int32_t* res_seg;
+  const double* rf;
+  const double* rq;
+  const int32_t* br_ptr;
+  const int32_t* br_ent;
+  const int32_t* vs_p;
+  const int32_t* vs_n;
+  const int32_t* conv_mask;
+  const double* X;
+  const double* vsrc;
+  const double* Q_prev;
+  const double* dQdt_prev;
+  const double* Q_prev2;
+  const double* res_floor;  // may be null -> abstol
+  const DevScalars* sc;
+  double* f;
+  double* Q;
+  double* mrc;
+  double* I_vs;
+  DevStatus* st;  // may be null
+};
+void launch_residual(const ResidualArgs& a, hipStream_t s);
+
+template <typename T>
+void launch_lu_refactor(DevLU& lu, const double* csr, T* lu_vals, DevStatus* st, hipStream_t s);
+template <typename T>
+void launch_lu_solve(DevLU& lu, const T* lu_vals, const double* rhs, double rhs_scale, double* x,
+                     T* y, T* z, hipStream_t s);
+void launch_axpy(int n, double* x, const double* d, hipStream_t s);
+
+void launch_model_eval_raw(int model_id, int count, const double* vb, const double* par,
+                           double* rr, double* rc, double* jr, double* jc, hipStream_t s);
+void launch_sources(int m, const int32_t* wave, const double* params, double t, double* out,
+                    hipStream_t s);
+
+int model_n_v(int model_id);
+int model_n_res(int model_id);
+int model_n_jac(int model_id);
+int model_n_par(int model_id);
+
+}  // namespace vjx

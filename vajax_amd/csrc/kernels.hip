@@ -1,0 +1,725 @@
+// HIP kernels of the transient core for gfx950 (MI355X).
+//
+// Stamping: one lane per device instance reads structure-of-arrays node
+// indices and parameters (coalesced), evaluates the model and writes every
+// Jacobian / residual contribution to its slot in a contribution list that is
+// pre-sorted by CSR slot (resp. residual row).  A second kernel reduces each
+// slot's contiguous segment in a fixed order, so csr_data and f are
+// bit-identical run to run without atomics.
+//
+// LU: right-looking numeric refactorisation over a level schedule computed
+// on the host.  Every item of a level is one update target whose segment of
+// contributions L(i,k)*U(k,j)/U(k,k) is summed in fixed order; wide levels run
+// as a grid, runs of narrow levels as one block with barriers between levels.
+// Triangular solves use row level sets in the same way.
+//
+// Compiled with -ffp-contract=off so every FP64 expression rounds exactly as
+// written (and as the numpy oracle evaluates it).
+#include "circuit.h"
+#include "internal.h"
+
+#include <cmath>
+
+namespace vjx {
+
+namespace {
+
+constexpr double KB = 1.380649e-23;
+constexpr double QE = 1.602176634e-19;
+constexpr double TEMP_K = 300.15;
+constexpr double VT = KB * TEMP_K / QE;
+
+__device__ __forceinline__ double nan0(double x) { return x != x ? 0.0 : x; }
+
+__device__ __forceinline__ unsigned long long dbits(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+
+// ---- models -------------------------------------------------------------
+
+template <int MID>
+struct ModelTraits;
+template <>
+struct ModelTraits<VJX_MODEL_RESISTOR> {
+  static constexpr int NV = 1, NR = 2, NJ = 4, NP = 1;
+};
+template <>
+struct ModelTraits<VJX_MODEL_CAPACITOR> {
+  static constexpr int NV = 1, NR = 2, NJ = 4, NP = 1;
+};
+template <>
+struct ModelTraits<VJX_MODEL_MOSFET> {
+  static constexpr int NV = 3, NR = 4, NJ = 16, NP = 11;
+};
+
+// resistor.va:23-27 -- I(A,B) <+ V(A,B)/r
+__device__ __forceinline__ void eval_resistor(const double* vb, const double* par, int i, int N,
+                                              double* rr, double* rc, double* jr, double* jc) {
+  const double r = par[i];
+  const double cur = vb[0] / r;
+  const double g = 1.0 / r;
+  rr[0] = cur;
+  rr[1] = -cur;
+  rc[0] = 0.0;
+  rc[1] = 0.0;
+  jr[0] = g;
+  jr[1] = -g;
+  jr[2] = -g;
+  jr[3] = g;
+  jc[0] = jc[1] = jc[2] = jc[3] = 0.0;
+}
+
+// capacitor.va:20-23 -- I(A,B) <+ ddt(c*V(A,B))
+__device__ __forceinline__ void eval_capacitor(const double* vb, const double* par, int i, int N,
+                                               double* rr, double* rc, double* jr, double* jc) {
+  const double c = par[i];
+  const double q = c * vb[0];
+  rr[0] = 0.0;
+  rr[1] = 0.0;
+  rc[0] = q;
+  rc[1] = -q;
+  jr[0] = jr[1] = jr[2] = jr[3] = 0.0;
+  jc[0] = c;
+  jc[1] = -c;
+  jc[2] = -c;
+  jc[3] = c;
+}
+
+__device__ __forceinline__ double softplus(double u) {
+  return fmax(u, 0.0) + log1p(exp(-fabs(u)));
+}
+__device__ __forceinline__ double sigmoid(double u) {
+  const double e = exp(-fabs(u));
+  return u >= 0.0 ? 1.0 / (1.0 + e) : e / (1.0 + e);
+}
+
+// EKV 2.6 long-channel drain current (ekv_longchannel.va:58-96) in the NMOS
+// frame with polarity sgn, analytic derivatives, linear terminal charges.
+// Parameter columns: sgn, vgoff, phi, gamma, beta0, theta, cgs, cgd, cgb, cdb, csb.
+// Branch voltages: vb[0] = V(g,b), vb[1] = V(s,b), vb[2] = V(d,b).
+// Residual order d, g, s, b; Jacobian keys row-major over (d, g, s, b).
+__device__ __forceinline__ void eval_mosfet(const double* vb, const double* par, int i, int N,
+                                            double* rr, double* rc, double* jr, double* jc) {
+  const double sgn = par[0 * (size_t)N + i];
+  const double vgoff = par[1 * (size_t)N + i];
+  const double phi = par[2 * (size_t)N + i];
+  const double gamma = par[3 * (size_t)N + i];
+  const double beta0 = par[4 * (size_t)N + i];
+  const double theta = par[5 * (size_t)N + i];
+  const double vgb = vb[0], vsb = vb[1], vdb = vb[2];
+  const double VG = sgn * vgb;
+  const double VS = sgn * vsb;
+  const double VD = sgn * vdb;
+  const double vgp = VG + vgoff;
+  const double g2 = 0.5 * gamma;
+  const double t = vgp + g2 * g2;
+  double vp, dvp;
+  if (t > 1e-12) {
+    const double s = sqrt(t);
+    vp = vgp - phi - gamma * (s - g2);
+    dvp = 1.0 - gamma / (2.0 * s);
+  } else {
+    vp = vgp - phi - gamma * gamma / 2.0;
+    dvp = 1.0;
+  }
+  const double arg = phi + vp + 4.0 * VT;
+  double n, dn;
+  if (arg > 1e-24) {
+    const double sa = sqrt(arg);
+    n = 1.0 + gamma / (2.0 * sa);
+    dn = -gamma / (4.0 * arg * sa);
+  } else {
+    n = 1.0 + gamma / 2.0 * 1e48;
+    dn = 0.0;
+  }
+  const double den = 1.0 + theta * vp;
+  const double beta = beta0 / den;
+  const double dbeta = -beta0 * theta / (den * den);
+  const double uf = 0.5 * ((vp - VS) / VT);
+  const double ur = 0.5 * ((vp - VD) / VT);
+  const double lf = softplus(uf);
+  const double lr = softplus(ur);
+  const double fpf = lf * sigmoid(uf);
+  const double fpr = lr * sigmoid(ur);
+  const double vt2 = VT * VT;
+  const double ispec = 2.0 * n * beta * vt2;
+  const double idiff = lf * lf - lr * lr;
+  const double i_d = ispec * idiff;
+  const double dispec = 2.0 * vt2 * (dn * beta + n * dbeta);
+  const double gvp = dispec * idiff + ispec * (fpf - fpr) / VT;
+  const double gg = gvp * dvp;
+  const double gs = -ispec * fpf / VT;
+  const double gd = ispec * fpr / VT;
+  const double gb = -(gg + gs + gd);
+  const double iext = sgn * i_d;
+  rr[0] = iext;
+  rr[1] = 0.0;
+  rr[2] = -iext;
+  rr[3] = 0.0;
+  jr[0] = gd;
+  jr[1] = gg;
+  jr[2] = gs;
+  jr[3] = gb;
+  jr[4] = jr[5] = jr[6] = jr[7] = 0.0;
+  jr[8] = -gd;
+  jr[9] = -gg;
+  jr[10] = -gs;
+  jr[11] = -gb;
+  jr[12] = jr[13] = jr[14] = jr[15] = 0.0;
+  const double cgs = par[6 * (size_t)N + i];
+  const double cgd = par[7 * (size_t)N + i];
+  const double cgb = par[8 * (size_t)N + i];
+  const double cdb = par[9 * (size_t)N + i];
+  const double csb = par[10 * (size_t)N + i];
+  const double vgs = vgb - vsb;
+  const double vgd = vgb - vdb;
+  rc[1] = cgs * vgs + cgd * vgd + cgb * vgb;
+  rc[0] = -cgd * vgd + cdb * vdb;
+  rc[2] = -cgs * vgs + csb * vsb;
+  rc[3] = -(cgb * vgb + cdb * vdb + csb * vsb);
+  jc[0] = cgd + cdb;
+  jc[1] = -cgd;
+  jc[2] = 0.0;
+  jc[3] = -cdb;
+  jc[4] = -cgd;
+  jc[5] = cgs + cgd + cgb;
+  jc[6] = -cgs;
+  jc[7] = -cgb;
+  jc[8] = 0.0;
+  jc[9] = -cgs;
+  jc[10] = cgs + csb;
+  jc[11] = -csb;
+  jc[12] = -cdb;
+  jc[13] = -cgb;
+  jc[14] = -csb;
+  jc[15] = cdb + csb + cgb;
+}
+
+template <int MID>
+__device__ __forceinline__ void eval_model(const double* vb, const double* par, int i, int N,
+                                           double* rr, double* rc, double* jr, double* jc) {
+  if constexpr (MID == VJX_MODEL_RESISTOR) eval_resistor(vb, par, i, N, rr, rc, jr, jc);
+  if constexpr (MID == VJX_MODEL_CAPACITOR) eval_capacitor(vb, par, i, N, rr, rc, jr, jc);
+  if constexpr (MID == VJX_MODEL_MOSFET) eval_mosfet(vb, par, i, N, rr, rc, jr, jc);
+}
+
+// Eval + stamp: lane = device instance.
+template <int MID>
+__global__ void __launch_bounds__(256) k_eval(int N, const int32_t* __restrict__ vn1,
+                                              const int32_t* __restrict__ vn2,
+                                              const int32_t* __restrict__ jdst,
+                                              const int32_t* __restrict__ rdst,
+                                              const double* __restrict__ par,
+                                              const double* __restrict__ X,
+                                              const DevScalars* __restrict__ sc, double* jac,
+                                              double* rf, double* rq) {
+  using Tr = ModelTraits<MID>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double vb[Tr::NV];
+#pragma unroll
+  for (int k = 0; k < Tr::NV; ++k)
+    vb[k] = X[vn1[(size_t)k * N + i]] - X[vn2[(size_t)k * N + i]];
+  double rr[Tr::NR], rc[Tr::NR], jr[Tr::NJ], jc[Tr::NJ];
+  eval_model<MID>(vb, par, i, N, rr, rc, jr, jc);
+  if (jac) {
+    const double c0 = sc->c0;
+#pragma unroll
+    for (int e = 0; e < Tr::NJ; ++e) {
+      const int d = jdst[(size_t)e * N + i];
+      if (d >= 0) jac[d] = nan0(jr[e] + c0 * jc[e]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < Tr::NR; ++r) {
+    const int d = rdst[(size_t)r * N + i];
+    if (d >= 0) {
+      rf[d] = nan0(rr[r]);
+      rq[d] = nan0(rc[r]);
+    }
+  }
+}
+
+// Raw eval (openvaf_jax contract), row-major outputs per instance.
+template <int MID>
+__global__ void k_model_eval(int N, const double* __restrict__ vbr, const double* __restrict__ par,
+                             double* rro, double* rco, double* jro, double* jco) {
+  using Tr = ModelTraits<MID>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double vb[Tr::NV];
+#pragma unroll
+  for (int k = 0; k < Tr::NV; ++k) vb[k] = vbr[(size_t)k * N + i];
+  double rr[Tr::NR], rc[Tr::NR], jr[Tr::NJ], jc[Tr::NJ];
+  eval_model<MID>(vb, par, i, N, rr, rc, jr, jc);
+#pragma unroll
+  for (int r = 0; r < Tr::NR; ++r) {
+    rro[(size_t)i * Tr::NR + r] = rr[r];
+    rco[(size_t)i * Tr::NR + r] = rc[r];
+  }
+#pragma unroll
+  for (int e = 0; e < Tr::NJ; ++e) {
+    jro[(size_t)i * Tr::NJ + e] = jr[e];
+    jco[(size_t)i * Tr::NJ + e] = jc[e];
+  }
+}
+
+__global__ void k_isrc(int k, const double* __restrict__ isrc, const int32_t* __restrict__ dst,
+                       double* rf, double* rq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const double v = isrc[i];
+  const int dp = dst[2 * i], dn = dst[2 * i + 1];
+  if (dp >= 0) {
+    rf[dp] = nan0(v * 1.0);
+    rq[dp] = 0.0;
+  }
+  if (dn >= 0) {
+    rf[dn] = nan0(v * -1.0);
+    rq[dn] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_reduce_jac(int nse, const int32_t* __restrict__ seg,
+                                                    const double* __restrict__ contrib,
+                                                    const int32_t* __restrict__ slot_diag,
+                                                    const double* __restrict__ slot_b,
+                                                    const DevScalars* __restrict__ sc,
+                                                    double* __restrict__ csr) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nse) return;
+  double acc = 0.0;
+  for (int u = seg[s], e = seg[s + 1]; u < e; ++u) acc += contrib[u];
+  if (slot_diag[s]) acc += sc->gmin + sc->gshunt;
+  const double b = slot_b[s];
+  if (b != 0.0) acc += b;
+  csr[s] = acc;
+}
+
+__device__ __forceinline__ void block_max_u64(unsigned long long v, unsigned long long* out) {
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
+}
+
+__device__ __forceinline__ void block_count(bool flag, unsigned int* out) {
+  unsigned long long m = __ballot(flag);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (unsigned int)__popcll(m));
+}
+
+__global__ void __launch_bounds__(256) k_residual(ResidualArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const DevScalars* sc = a.sc;
+  bool fail = false;
+  double fchk = 0.0;
+  if (i < a.n_unk) {
+    double f = 0.0, q = 0.0, mx = 0.0;
+    for (int u = a.res_seg[i], e = a.res_seg[i + 1]; u < e; ++u) {
+      const double v = a.rf[u];
+      f += v;
+      q += a.rq[u];
+      mx = fmax(mx, fabs(v));
+    }
+    double fr = f - 0.0;
+    if (a.m > 0) {
+      double br = 0.0;
+      for (int e = a.br_ptr[i]; e < a.br_ptr[i + 1]; ++e) {
+        const int k = a.br_ent[e];
+        if (k >= 0) {
+          a.I_vs[k] = -fr;
+          br += a.X[a.n_total + k];
+        } else {
+          br += -a.X[a.n_total + ~k];
+        }
+      }
+      fr = fr + br;
+    }
+    const double qp = a.Q_prev ? a.Q_prev[i] : 0.0;
+    const double dq = a.dQdt_prev ? a.dQdt_prev[i] : 0.0;
+    const double q2 = a.Q_prev2 ? a.Q_prev2[i] : 0.0;
+    double t = fr - 0.0;
+    t = t + sc->c0 * (q - 0.0);
+    t = t + sc->c1 * qp;
+    t = t + sc->d1 * dq;
+    t = t + sc->c2 * q2;
+    const double eff = sc->gmin + sc->gshunt;
+    t = t + eff * a.X[i + 1];
+    a.f[i] = t;
+    if (a.Q) a.Q[i] = q;
+    if (a.mrc) a.mrc[i] = mx;
+    const bool checked = a.conv_mask == nullptr || a.conv_mask[i] != 0;
+    if (checked) {
+      const double floor = a.res_floor ? a.res_floor[i] : sc->abstol;
+      const double tol = fmax(mx * sc->reltol, floor);
+      fchk = fabs(t);
+      fail = !(fchk < tol);
+    }
+  } else if (i < a.n_unk + a.m) {
+    const int k = i - a.n_unk;
+    const int p = a.vs_p[k], nn = a.vs_n[k];
+    const double t = a.X[p] - a.X[nn] - a.vsrc[k];
+    a.f[i] = t;
+    if (p == 0) a.I_vs[k] = 0.0;
+    fchk = fabs(t);
+    fail = !(fchk < sc->vntol);
+  }
+  if (a.st) {
+    block_max_u64(dbits(fchk), &a.st->maxf_bits);
+    block_count(fail, &a.st->res_fail);
+  }
+}
+
+// ---- LU refactor ------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_gather(int nnz, const int32_t* __restrict__ src,
+                                                   const double* __restrict__ csr, T* lu) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const int s = src[e];
+  lu[e] = s >= 0 ? (T)csr[s] : (T)0;
+}
+
+template <typename T>
+__device__ __forceinline__ void lu_item(int t, const int32_t* __restrict__ tpos,
+                                        const int32_t* __restrict__ tseg,
+                                        const int32_t* __restrict__ ul,
+                                        const int32_t* __restrict__ uu,
+                                        const int32_t* __restrict__ ud, T* lu) {
+  T acc = (T)0;
+  for (int u = tseg[t], e = tseg[t + 1]; u < e; ++u) acc += lu[ul[u]] * lu[uu[u]] / lu[ud[u]];
+  lu[tpos[t]] -= acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_wide(int lo, int hi, const int32_t* __restrict__ items,
+                                                 const int32_t* __restrict__ tpos,
+                                                 const int32_t* __restrict__ tseg,
+                                                 const int32_t* __restrict__ ul,
+                                                 const int32_t* __restrict__ uu,
+                                                 const int32_t* __restrict__ ud, T* lu) {
+  const int g = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= hi) return;
+  lu_item<T>(items[g], tpos, tseg, ul, uu, ud, lu);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_lu_narrow(int l0, int l1, const int32_t* __restrict__ lvl,
+                                                    const int32_t* __restrict__ items,
+                                                    const int32_t* __restrict__ tpos,
+                                                    const int32_t* __restrict__ tseg,
+                                                    const int32_t* __restrict__ ul,
+                                                    const int32_t* __restrict__ uu,
+                                                    const int32_t* __restrict__ ud, T* lu) {
+  for (int l = l0; l < l1; ++l) {
+    const int lo = lvl[l], hi = lvl[l + 1];
+    for (int g = lo + (int)threadIdx.x; g < hi; g += blockDim.x)
+      lu_item<T>(items[g], tpos, tseg, ul, uu, ud, lu);
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_scale(int n, const int32_t* __restrict__ ls,
+                                                  const int32_t* __restrict__ ld, T* lu,
+                                                  DevStatus* st) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double mag = 0.0;
+  bool bad = false;
+  if (e < n) {
+    const T v = lu[ls[e]] / lu[ld[e]];
+    lu[ls[e]] = v;
+    mag = fabs((double)v);
+    bad = !(mag <= 1.7976931348623157e308);
+  }
+  if (st) {
+    block_max_u64(dbits(bad ? 0.0 : mag), &st->maxl_bits);
+    block_count(bad, &st->nonfinite);
+  }
+}
+
+// ---- triangular solves ------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_solve_init(int n, const int32_t* __restrict__ p,
+                                                    const double* __restrict__ rhs, double scale,
+                                                    T* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (scale == -1.0)
+    y[i] = -(T)rhs[p[i]];
+  else
+    y[i] = (T)rhs[p[i]];
+}
+
+template <typename T>
+__device__ __forceinline__ void fwd_row(int i, const int32_t* __restrict__ rp,
+                                        const int32_t* __restrict__ col,
+                                        const int32_t* __restrict__ dg, const T* __restrict__ lu,
+                                        T* y) {
+  T acc = (T)0;
+  for (int e = rp[i], d = dg[i]; e < d; ++e) acc += lu[e] * y[col[e]];
+  y[i] = y[i] - acc;
+}
+
+template <typename T>
+__device__ __forceinline__ void bwd_row(int i, const int32_t* __restrict__ rp,
+                                        const int32_t* __restrict__ col,
+                                        const int32_t* __restrict__ dg, const T* __restrict__ lu,
+                                        const T* __restrict__ y, T* z) {
+  T acc = (T)0;
+  const int d = dg[i];
+  for (int e = d + 1, end = rp[i + 1]; e < end; ++e) acc += lu[e] * z[col[e]];
+  z[i] = (y[i] - acc) / lu[d];
+}
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(256) k_tri_wide(int lo, int hi, const int32_t* __restrict__ items,
+                                                  const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ dg,
+                                                  const T* __restrict__ lu, T* y, T* z) {
+  const int g = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= hi) return;
+  if (FWD)
+    fwd_row<T>(items[g], rp, col, dg, lu, y);
+  else
+    bwd_row<T>(items[g], rp, col, dg, lu, y, z);
+}
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(1024) k_tri_narrow(int l0, int l1, const int32_t* __restrict__ lvl,
+                                                     const int32_t* __restrict__ items,
+                                                     const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ dg,
+                                                     const T* __restrict__ lu, T* y, T* z) {
+  for (int l = l0; l < l1; ++l) {
+    const int lo = lvl[l], hi = lvl[l + 1];
+    for (int g = lo + (int)threadIdx.x; g < hi; g += blockDim.x) {
+      if (FWD)
+        fwd_row<T>(items[g], rp, col, dg, lu, y);
+      else
+        bwd_row<T>(items[g], rp, col, dg, lu, y, z);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_solve_finish(int n, const int32_t* __restrict__ q,
+                                                      const T* __restrict__ z, double* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[q[i]] = (double)z[i];
+}
+
+// out = base + sgn * (A x)   (base may be null)
+__global__ void __launch_bounds__(256) k_spmv(int n, const int32_t* __restrict__ rp,
+                                              const int32_t* __restrict__ ci,
+                                              const double* __restrict__ a,
+                                              const double* __restrict__ x,
+                                              const double* __restrict__ base, double sgn,
+                                              double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int e = rp[i], end = rp[i + 1]; e < end; ++e) acc += a[e] * x[ci[e]];
+  if (sgn != 1.0) acc = sgn * acc;
+  out[i] = base ? base[i] + acc : acc;
+}
+
+__global__ void k_status_reset(DevStatus* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->maxf_bits = 0;
+    st->maxdv_bits = 0;
+    st->maxdelta_bits = 0;
+    st->maxl_bits = 0;
+    st->res_fail = 0;
+    st->delta_fail = 0;
+    st->nonfinite = 0;
+    st->pad = 0;
+  }
+}
+
+__global__ void k_axpy(int n, double* x, const double* d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + d[i];
+}
+
+// vajax/analysis/sources.py:31-63 (pulse) and :25-28 (dc)
+__device__ __forceinline__ double wave_value(int w, const double* p, int i, int m, double t) {
+  const double dc = p[0 * (size_t)m + i];
+  if (w == 0) return dc;
+  const double v0 = p[1 * (size_t)m + i], v1 = p[2 * (size_t)m + i];
+  const double r = p[3 * (size_t)m + i], f = p[4 * (size_t)m + i];
+  const double wd = p[5 * (size_t)m + i], per = p[6 * (size_t)m + i];
+  const double d = p[7 * (size_t)m + i];
+  if (t < d) return v0;
+  double tin = fmod(t - d, per);
+  if (tin < 0.0) tin += per;
+  if (tin < r) return v0 + (v1 - v0) * tin / r;
+  if (tin < r + wd) return v1;
+  if (tin < r + wd + f) return v1 - (v1 - v0) * (tin - r - wd) / f;
+  return v0;
+}
+
+__global__ void k_sources(int m, const int32_t* __restrict__ wave, const double* __restrict__ par,
+                          double t, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = wave_value(wave[i], par, i, m, t);
+}
+
+inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+int model_n_v(int id) {
+  return id == VJX_MODEL_MOSFET ? 3 : 1;
+}
+int model_n_res(int id) { return id == VJX_MODEL_MOSFET ? 4 : 2; }
+int model_n_jac(int id) { return id == VJX_MODEL_MOSFET ? 16 : 4; }
+int model_n_par(int id) { return id == VJX_MODEL_MOSFET ? 11 : 1; }
+
+void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalOut out,
+                 hipStream_t s) {
+  if (m.count == 0) return;
+  const int B = 256;
+  const int G = nblk(m.count, B);
+  switch (m.model_id) {
+    case VJX_MODEL_RESISTOR:
+      k_eval<VJX_MODEL_RESISTOR><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
+                                                 m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
+                                                 out.rq);
+      break;
+    case VJX_MODEL_CAPACITOR:
+      k_eval<VJX_MODEL_CAPACITOR><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
+                                                  m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
+                                                  out.rq);
+      break;
+    case VJX_MODEL_MOSFET:
+      k_eval<VJX_MODEL_MOSFET><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
+                                               m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
+                                               out.rq);
+      break;
+  }
+}
+
+void launch_model_eval_raw(int id, int count, const double* vb, const double* par, double* rr,
+                           double* rc, double* jr, double* jc, hipStream_t s) {
+  if (count == 0) return;
+  const int B = 256, G = nblk(count, B);
+  switch (id) {
+    case VJX_MODEL_RESISTOR:
+      k_model_eval<VJX_MODEL_RESISTOR><<<G, B, 0, s>>>(count, vb, par, rr, rc, jr, jc);
+      break;
+    case VJX_MODEL_CAPACITOR:
+      k_model_eval<VJX_MODEL_CAPACITOR><<<G, B, 0, s>>>(count, vb, par, rr, rc, jr, jc);
+      break;
+    case VJX_MODEL_MOSFET:
+      k_model_eval<VJX_MODEL_MOSFET><<<G, B, 0, s>>>(count, vb, par, rr, rc, jr, jc);
+      break;
+  }
+}
+
+void launch_isrc_contrib(int k, const double* isrc, const int32_t* is_dst, double* rf, double* rq,
+                         hipStream_t s) {
+  if (k == 0) return;
+  k_isrc<<<nblk(k, 64), 64, 0, s>>>(k, isrc, is_dst, rf, rq);
+}
+
+void launch_reduce_jac(int nse, const int32_t* seg, const double* contrib, const int32_t* slot_diag,
+                       const double* slot_b, const DevScalars* sc, double* csr, hipStream_t s) {
+  k_reduce_jac<<<nblk(nse, 256), 256, 0, s>>>(nse, seg, contrib, slot_diag, slot_b, sc, csr);
+}
+
+void launch_residual(const ResidualArgs& a, hipStream_t s) {
+  const int n = a.n_unk + a.m;
+  if (n == 0) return;
+  k_residual<<<nblk(n, 256), 256, 0, s>>>(a);
+}
+
+template <typename T>
+static void run_fact_groups(DevLU& L, T* lu, hipStream_t s, const int32_t* lvl_dev) {
+  for (const auto& g : L.fact.groups) {
+    if (g.wide) {
+      const int lo = L.fact.lvl_ptr[g.lvl_begin], hi = L.fact.lvl_ptr[g.lvl_end];
+      k_lu_wide<T><<<nblk(hi - lo, L.wide_block), L.wide_block, 0, s>>>(
+          lo, hi, L.fact.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p, lu);
+    } else {
+      k_lu_narrow<T><<<1, L.narrow_block, 0, s>>>(g.lvl_begin, g.lvl_end, lvl_dev, L.fact.items.p,
+                                                  L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p,
+                                                  L.upd_d.p, lu);
+    }
+  }
+}
+
+template <typename T>
+void launch_lu_refactor(DevLU& L, const double* csr, T* lu, DevStatus* st, hipStream_t s) {
+  const int nnz = (int)L.col.n;
+  k_lu_gather<T><<<nblk(nnz, 256), 256, 0, s>>>(nnz, L.src.p, csr, lu);
+  run_fact_groups<T>(L, lu, s, L.fact.lvl_dev.p);
+  const int nl = (int)L.l_slot.n;
+  if (nl > 0) k_lu_scale<T><<<nblk(nl, 256), 256, 0, s>>>(nl, L.l_slot.p, L.l_diag.p, lu, st);
+}
+
+template <typename T>
+void launch_lu_solve(DevLU& L, const T* lu, const double* rhs, double scale, double* x, T* y, T* z,
+                     hipStream_t s) {
+  const int n = L.sym.n;
+  if (n == 0) return;
+  k_solve_init<T><<<nblk(n, 256), 256, 0, s>>>(n, L.p.p, rhs, scale, y);
+  for (const auto& g : L.fwd.groups) {
+    if (g.wide) {
+      const int lo = L.fwd.lvl_ptr[g.lvl_begin], hi = L.fwd.lvl_ptr[g.lvl_end];
+      k_tri_wide<T, true><<<nblk(hi - lo, L.wide_block), L.wide_block, 0, s>>>(
+          lo, hi, L.fwd.items.p, L.rowptr.p, L.col.p, L.diag.p, lu, y, z);
+    } else {
+      k_tri_narrow<T, true><<<1, L.narrow_block, 0, s>>>(g.lvl_begin, g.lvl_end, L.fwd.lvl_dev.p,
+                                                         L.fwd.items.p, L.rowptr.p, L.col.p,
+                                                         L.diag.p, lu, y, z);
+    }
+  }
+  for (const auto& g : L.bwd.groups) {
+    if (g.wide) {
+      const int lo = L.bwd.lvl_ptr[g.lvl_begin], hi = L.bwd.lvl_ptr[g.lvl_end];
+      k_tri_wide<T, false><<<nblk(hi - lo, L.wide_block), L.wide_block, 0, s>>>(
+          lo, hi, L.bwd.items.p, L.rowptr.p, L.col.p, L.diag.p, lu, y, z);
+    } else {
+      k_tri_narrow<T, false><<<1, L.narrow_block, 0, s>>>(g.lvl_begin, g.lvl_end, L.bwd.lvl_dev.p,
+                                                          L.bwd.items.p, L.rowptr.p, L.col.p,
+                                                          L.diag.p, lu, y, z);
+    }
+  }
+  k_solve_finish<T><<<nblk(n, 256), 256, 0, s>>>(n, L.q.p, z, x);
+}
+
+template void launch_lu_refactor<double>(DevLU&, const double*, double*, DevStatus*, hipStream_t);
+template void launch_lu_refactor<float>(DevLU&, const double*, float*, DevStatus*, hipStream_t);
+template void launch_lu_solve<double>(DevLU&, const double*, const double*, double, double*,
+                                      double*, double*, hipStream_t);
+template void launch_lu_solve<float>(DevLU&, const float*, const double*, double, double*, float*,
+                                     float*, hipStream_t);
+
+void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const double* data,
+                 const double* x, const double* base, double sgn, double* out, hipStream_t s) {
+  if (n == 0) return;
+  k_spmv<<<nblk(n, 256), 256, 0, s>>>(n, indptr, indices, data, x, base, sgn, out);
+}
+
+void launch_status_reset(DevStatus* st, hipStream_t s) { k_status_reset<<<1, 64, 0, s>>>(st); }
+
+void launch_axpy(int n, double* x, const double* d, hipStream_t s) {
+  if (n == 0) return;
+  k_axpy<<<nblk(n, 256), 256, 0, s>>>(n, x, d);
+}
+
+void launch_sources(int m, const int32_t* wave, const double* params, double t, double* out,
+                    hipStream_t s) {
+  if (m == 0) return;
+  k_sources<<<nblk(m, 64), 64, 0, s>>>(m, wave, params, t, out);
+}
+
+}  // namespace vjx

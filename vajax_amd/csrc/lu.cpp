@@ -1,0 +1,138 @@
+// Sparse LU handle helpers: device upload of the symbolic analysis and the
+// asynchronous refactor / solve sequences used by the C ABI and the
+// transient driver.
+#include <cmath>
+#include <cstring>
+
+#include "circuit.h"
+
+namespace vjx {
+
+int pattern_stream(vjx_pattern* pat) {
+  if (pat->stream) return VJX_OK;
+  VJX_TRY_HIP(hipStreamCreateWithFlags(&pat->stream, hipStreamNonBlocking));
+  VJX_TRY_HIP(pat->d_status.alloc(1));
+  VJX_TRY_HIP(hipHostMalloc((void**)&pat->h_status, sizeof(DevStatus), hipHostMallocDefault));
+  std::memset(pat->h_status, 0, sizeof(DevStatus));
+  return VJX_OK;
+}
+
+int pattern_upload_csr(vjx_pattern* pat) {
+  int rc = pattern_stream(pat);
+  if (rc < 0) return rc;
+  if (pat->d_indptr.p == nullptr) {
+    VJX_TRY_HIP(pat->d_indptr.upload(pat->indptr));
+    VJX_TRY_HIP(pat->d_indices.upload(pat->indices));
+  }
+  return VJX_OK;
+}
+
+static hipError_t upload_sched(DevSchedule& d, const LevelSchedule& h) {
+  hipError_t e = d.items.upload(h.items);
+  if (e != hipSuccess) return e;
+  e = d.lvl_dev.upload(h.lvl_ptr);
+  d.lvl_ptr = h.lvl_ptr;
+  d.groups = h.groups;
+  return e;
+}
+
+int pattern_upload(vjx_pattern* pat) {
+  if (pat->uploaded) return VJX_OK;
+  int rc = pattern_upload_csr(pat);
+  if (rc < 0) return rc;
+  DevLU& L = pat->lu;
+  const LuSymbolic& s = L.sym;
+  VJX_TRY_HIP(L.rowptr.upload(s.rowptr));
+  VJX_TRY_HIP(L.col.upload(s.col));
+  VJX_TRY_HIP(L.diag.upload(s.diag));
+  VJX_TRY_HIP(L.src.upload(s.src));
+  VJX_TRY_HIP(L.tgt_pos.upload(s.tgt_pos));
+  VJX_TRY_HIP(L.tgt_seg.upload(s.tgt_seg));
+  VJX_TRY_HIP(L.upd_l.upload(s.upd_l));
+  VJX_TRY_HIP(L.upd_u.upload(s.upd_u));
+  VJX_TRY_HIP(L.upd_d.upload(s.upd_d));
+  VJX_TRY_HIP(L.l_slot.upload(s.l_slot));
+  VJX_TRY_HIP(L.l_diag.upload(s.l_diag));
+  VJX_TRY_HIP(L.p.upload(s.p));
+  VJX_TRY_HIP(L.q.upload(s.q));
+  VJX_TRY_HIP(upload_sched(L.fact, s.fact));
+  VJX_TRY_HIP(upload_sched(L.fwd, s.fwd));
+  VJX_TRY_HIP(upload_sched(L.bwd, s.bwd));
+  const size_t nnz = s.col.size();
+  const size_t n = (size_t)s.n;
+  VJX_TRY_HIP(L.lu64.alloc(nnz));
+  VJX_TRY_HIP(L.lu32.alloc(nnz));
+  VJX_TRY_HIP(L.y64.alloc(n));
+  VJX_TRY_HIP(L.z64.alloc(n));
+  VJX_TRY_HIP(L.r64.alloc(n));
+  VJX_TRY_HIP(L.x64.alloc(n));
+  VJX_TRY_HIP(L.y32.alloc(n));
+  VJX_TRY_HIP(L.z32.alloc(n));
+  pat->uploaded = true;
+  return VJX_OK;
+}
+
+int pattern_ensure_analyzed(vjx_pattern* pat, const double* data_dev) {
+  int rc = pattern_stream(pat);
+  if (rc < 0) return rc;
+  if (!pat->lu.analyzed) {
+    std::vector<double> h(pat->nnz);
+    if (pat->nnz)
+      VJX_TRY_HIP(hipMemcpy(h.data(), data_dev, sizeof(double) * pat->nnz, hipMemcpyDeviceToHost));
+    rc = vjx_lu_analyze(pat, h.data());
+    if (rc < 0) return rc;
+  }
+  return pattern_upload(pat);
+}
+
+int pattern_refactor_async(vjx_pattern* pat, const double* data_dev, int prec, bool reset_status) {
+  DevLU& L = pat->lu;
+  if (reset_status) launch_status_reset(pat->d_status.p, pat->stream);
+  if (prec == VJX_PREC_F32)
+    launch_lu_refactor<float>(L, data_dev, L.lu32.p, pat->d_status.p, pat->stream);
+  else
+    launch_lu_refactor<double>(L, data_dev, L.lu64.p, pat->d_status.p, pat->stream);
+  VJX_TRY_HIP(hipGetLastError());
+  L.factored_prec = prec;
+  return VJX_OK;
+}
+
+int pattern_check_pivots(vjx_pattern* pat) {
+  VJX_TRY_HIP(hipMemcpy(pat->h_status, pat->d_status.p, sizeof(DevStatus), hipMemcpyDeviceToHost));
+  double maxl;
+  std::memcpy(&maxl, &pat->h_status->maxl_bits, sizeof(double));
+  if (pat->h_status->nonfinite > 0 || maxl > 1e12) {
+    set_error("vjx_lu_refactor: pivot growth (max |L| = " + std::to_string(maxl) +
+              ", non-finite = " + std::to_string(pat->h_status->nonfinite) +
+              "); call vjx_lu_analyze to re-pivot");
+    return VJX_WARN_PIVOT;
+  }
+  return VJX_OK;
+}
+
+int pattern_solve_async(vjx_pattern* pat, const double* data_dev, const double* rhs, double scale,
+                        double* x, int refine_steps) {
+  DevLU& L = pat->lu;
+  hipStream_t s = pat->stream;
+  const int n = pat->n;
+  if (L.factored_prec == VJX_PREC_F64) {
+    launch_lu_solve<double>(L, L.lu64.p, rhs, scale, x, L.y64.p, L.z64.p, s);
+  } else {
+    launch_lu_solve<float>(L, L.lu32.p, rhs, scale, x, L.y32.p, L.z32.p, s);
+    if (refine_steps > 0) {
+      int rc = pattern_upload_csr(pat);
+      if (rc < 0) return rc;
+    }
+    for (int r = 0; r < refine_steps; ++r) {
+      // residual of A x = scale*rhs, in the reference's form
+      // (solver_factories.py:911-921): r = rhs + A x for scale -1.
+      launch_spmv(n, pat->d_indptr.p, pat->d_indices.p, data_dev, x, rhs, -scale, L.r64.p, s);
+      launch_lu_solve<float>(L, L.lu32.p, L.r64.p, scale, L.x64.p, L.y32.p, L.z32.p, s);
+      launch_axpy(n, x, L.x64.p, s);
+    }
+  }
+  VJX_TRY_HIP(hipGetLastError());
+  return VJX_OK;
+}
+
+}  // namespace vjx

@@ -1,0 +1,92 @@
+// Host-side symbolic analysis for the sparse LU of the MNA Jacobian.
+//
+// Replaces the symbolic half of the reference's UMFPACK FFI
+// (vajax/sparse/umfpack_ffi.cpp:140-179, 213-267): ordering and the pivot
+// sequence are chosen once per topology from the first numeric matrix; every
+// later Newton iteration only runs the numeric refactorisation on the GPU.
+//
+//  * column order q: minimum degree on the pattern of A + A^T, with rows
+//    whose degree exceeds max(16, 10 sqrt(n)) (supply rails) ordered last;
+//  * row pivots p: left-looking Gilbert-Peierls LU of A(:, q) with threshold
+//    partial pivoting (tolerance 0.1, the UMFPACK setting at
+//    umfpack_ffi.cpp:153), preferring the structural diagonal;
+//  * LU pattern of P A Q in CSR with the fill, and for every LU slot the A
+//    entry it is initialised from (or -1 for fill);
+//  * a right-looking level schedule: pivot k may run once every pivot k' with
+//    U(k',k) != 0 or L(k,k') != 0 has run.  Each level is a list of update
+//    targets (i,j) with the fixed-order list of contributions
+//    L(i,k)*U(k,j)/U(k,k) from that level's pivots, so the numeric phase is
+//    deterministic and needs no atomics;
+//  * row level sets for the forward (L) and backward (U) triangular solves.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vjx {
+
+// One launch of a level program: either a single wide level (grid of blocks)
+// or a run of consecutive narrow levels done by one block with barriers.
+struct LaunchGroup {
+  int lvl_begin;
+  int lvl_end;  // exclusive
+  int wide;     // 1: one level over a grid; 0: levels looped in one block
+  int items;    // number of items over the group
+};
+
+struct LevelSchedule {
+  std::vector<int32_t> items;    // item ids grouped by level
+  std::vector<int32_t> lvl_ptr;  // level l covers items[lvl_ptr[l] .. lvl_ptr[l+1])
+  std::vector<LaunchGroup> groups;
+  int max_width = 0;
+};
+
+struct LuSymbolic {
+  int n = 0;
+  int nnz_a = 0;
+  std::vector<int32_t> q, qinv;  // column order: permuted col k = original col q[k]
+  std::vector<int32_t> p, pinv;  // row order:    permuted row k = original row p[k]
+  // LU pattern of P A Q, CSR, columns sorted; L strictly left of diag, U right.
+  std::vector<int32_t> rowptr, col, diag;
+  std::vector<int32_t> src;  // per LU slot: index into A's CSR data, -1 = fill
+  // right-looking update program
+  std::vector<int32_t> tgt_pos;  // per target: LU slot updated
+  std::vector<int32_t> tgt_seg;  // per target: contributions [tgt_seg[t], tgt_seg[t+1])
+  std::vector<int32_t> upd_l;    // per contribution: slot of L(i,k)
+  std::vector<int32_t> upd_u;    // per contribution: slot of U(k,j)
+  std::vector<int32_t> upd_d;    // per contribution: slot of U(k,k)
+  LevelSchedule fact;            // items = target ids
+  std::vector<int32_t> l_slot;   // every L slot (i,k) ...
+  std::vector<int32_t> l_diag;   // ... and the slot of its pivot U(k,k)
+  LevelSchedule fwd;             // items = rows, L solve
+  LevelSchedule bwd;             // items = rows, U solve
+  // statistics
+  int64_t nnz_l = 0, nnz_u = 0, n_updates = 0;
+  double flops = 0.0;
+  int n_dense = 0;
+  int n_offdiag_pivots = 0;
+  double min_pivot_ratio = 1.0;
+};
+
+struct AnalyzeOptions {
+  double pivot_tol = 0.1;
+  int wide_threshold = 4096;  // levels with at least this many items get a grid
+  int ordering = 0;           // 0 = minimum degree, 1 = natural
+};
+
+// Full analysis from a CSR matrix (host arrays).  Returns false and sets err on
+// malformed input.  Values are used only to choose pivots.
+bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                const AnalyzeOptions& opt, LuSymbolic& out, std::string& err);
+
+// CSR structure validation as in vajax/sparse/umfpack_ffi.cpp:234-251.
+bool csr_validate(int n, int nnz, const int32_t* indptr, const int32_t* indices, std::string& err);
+
+// CPU execution of the same numeric program (reference for the GPU kernels).
+template <typename T>
+void lu_refactor_host(const LuSymbolic& s, const double* a, T* lu);
+template <typename T>
+void lu_solve_host(const LuSymbolic& s, const T* lu, const double* b, double* x);
+
+}  // namespace vjx

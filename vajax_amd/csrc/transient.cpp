@@ -1,0 +1,479 @@
+// Transient driver: Newton iterations and the adaptive / fixed-step time loop.
+//
+// All vectors (solution, charges, histories, Jacobian, LU factors) stay in
+// HBM.  The host runs only the scalar logic of the reference's step control
+// -- integration coefficients, predictor weights, accept/reject and the next
+// dt (vajax/analysis/transient/full_mna.py:1840-2194,
+// vajax/analysis/transient/adaptive.py:94-339) -- in the same FP64
+// arithmetic, reading back one small status record per Newton iteration.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "circuit.h"
+#include "transient.h"
+
+namespace vjx {
+
+namespace {
+
+double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+// sources.py:31-63 (pulse) / :25-28 (dc)
+double wave_value(int w, const std::vector<double>& p, int i, int m, double t) {
+  const double dc = p[0 * (size_t)m + i];
+  if (w == 0) return dc;
+  const double v0 = p[1 * (size_t)m + i], v1 = p[2 * (size_t)m + i];
+  const double r = p[3 * (size_t)m + i], f = p[4 * (size_t)m + i];
+  const double wd = p[5 * (size_t)m + i], per = p[6 * (size_t)m + i];
+  const double d = p[7 * (size_t)m + i];
+  if (t < d) return v0;
+  double tin = std::fmod(t - d, per);
+  if (tin < 0.0) tin += per;
+  if (tin < r) return v0 + (v1 - v0) * tin / r;
+  if (tin < r + wd) return v1;
+  if (tin < r + wd + f) return v1 - (v1 - v0) * (tin - r - wd) / f;
+  return v0;
+}
+
+struct Work {
+  DBuf<double> Xa, Xw, Xsave, Vp, Qp, dQ, Q2, Qn, dQn, mrc, hmrc, floor, Vmax, f, csr, delta, vsrc, isrc,
+      Ivs, hist, vout, iout, zeros;
+  DBuf<DevStatus> st;
+  DevStatus* h_st = nullptr;
+  ~Work() {
+    if (h_st) (void)hipHostFree(h_st);
+  }
+};
+
+struct Driver {
+  vjx_circuit* c;
+  const vjx_tran_config* cfg;
+  hipStream_t s;
+  Work w;
+  int n_total, n_unk, m, k, n_aug;
+  int64_t total_evals = 0;
+  int repivots = 0;
+
+  int upload_scalars(double c0, double c1, double d1, double c2, double gshunt, int it) {
+    DevScalars* h = c->h_sc;
+    h->c0 = c0;
+    h->c1 = c1;
+    h->c2 = c2;
+    h->d1 = d1;
+    h->gmin = cfg->gmin;
+    h->gshunt = gshunt;
+    h->reltol = cfg->reltol;
+    h->vntol = cfg->vntol;
+    h->abstol = cfg->abstol;
+    h->nr_damping = cfg->nr_damping;
+    h->max_step = 1e30;
+    h->iteration = it;
+    VJX_TRY_HIP(hipMemcpyAsync(c->d_sc.p, h, sizeof(DevScalars), hipMemcpyHostToDevice, s));
+    return VJX_OK;
+  }
+
+  int set_sources(double t) {
+    std::vector<double> v(std::max(m, 1)), i(std::max(k, 1));
+    for (int j = 0; j < m; ++j) v[j] = wave_value(c->h_vs_wave[j], c->h_vs_par, j, m, t);
+    for (int j = 0; j < k; ++j) i[j] = wave_value(c->h_is_wave[j], c->h_is_par, j, k, t);
+    if (m) VJX_TRY_HIP(hipMemcpyAsync(w.vsrc.p, v.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    if (k) VJX_TRY_HIP(hipMemcpyAsync(w.isrc.p, i.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    // the host vectors die at return: make the copies complete first
+    VJX_TRY_HIP(hipStreamSynchronize(s));
+    return VJX_OK;
+  }
+
+  int read_status() {
+    VJX_TRY_HIP(hipMemcpyAsync(w.h_st, w.st.p, sizeof(DevStatus), hipMemcpyDeviceToHost, s));
+    VJX_TRY_HIP(hipStreamSynchronize(s));
+    return VJX_OK;
+  }
+
+  // One Newton iteration (solver_factories.py:230-325) on Xw.
+  int nr_iteration(const double* Qp, const double* dQ, const double* Q2, const double* floor) {
+    launch_status_reset(w.st.p, s);
+    circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, floor, w.csr.p, w.f.p, w.Qn.p,
+                        w.mrc.p, w.Ivs.p, w.st.p, s);
+    ++total_evals;
+    vjx_pattern* pat = c->pat;
+    // analysis on first use (pivots from the first numeric matrix)
+    if (!pat->lu.analyzed) {
+      VJX_TRY_HIP(hipStreamSynchronize(s));
+      int rc = pattern_ensure_analyzed(pat, w.csr.p);
+      if (rc < 0) return rc;
+    }
+    pat->stream = s;
+    int rc = pattern_refactor_async(pat, w.csr.p, cfg->precision, false);
+    if (rc < 0) return rc;
+    // pivot growth is checked from the status of this iteration
+    rc = pattern_solve_async(pat, w.csr.p, w.f.p, -1.0, w.delta.p, cfg->refine_steps);
+    if (rc < 0) return rc;
+    // LU status lives in the pattern's record; the NR status in w.st
+    launch_delta(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
+    launch_update(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
+    VJX_TRY_HIP(hipGetLastError());
+    return VJX_OK;
+  }
+
+  // nr_solve while-loop semantics; returns iterations, converged, max_f.
+  int nr_solve(double c0, double c1, double d1, double c2, double gshunt, const double* Qp,
+               const double* dQ, const double* Q2, const double* floor, int& iters, bool& conv,
+               double& max_f) {
+    iters = 0;
+    conv = false;
+    max_f = std::numeric_limits<double>::infinity();
+    vjx_pattern* pat = c->pat;
+    while (!conv && iters < cfg->tran_itl) {
+      int rc = upload_scalars(c0, c1, d1, c2, gshunt, iters);
+      if (rc < 0) return rc;
+      launch_copy(n_total + m, w.Xw.p, w.Xsave.p, s);
+      launch_status_reset(pat->d_status.p, s);
+      rc = nr_iteration(Qp, dQ, Q2, floor);
+      if (rc < 0) return rc;
+      VJX_TRY_HIP(hipMemcpyAsync(pat->h_status, pat->d_status.p, sizeof(DevStatus),
+                                 hipMemcpyDeviceToHost, s));
+      rc = read_status();
+      if (rc < 0) return rc;
+      // pivot monitor: re-pivot on this matrix and redo the iteration from
+      // the saved iterate
+      const double maxl = bits_to_double(pat->h_status->maxl_bits);
+      if ((pat->h_status->nonfinite > 0 || maxl > 1e12) && repivots < 64) {
+        std::vector<double> h(c->nse);
+        VJX_TRY_HIP(hipMemcpy(h.data(), w.csr.p, sizeof(double) * c->nse, hipMemcpyDeviceToHost));
+        rc = vjx_lu_analyze(pat, h.data());
+        if (rc < 0) return rc;
+        rc = pattern_upload(pat);
+        if (rc < 0) return rc;
+        ++repivots;
+        launch_copy(n_total + m, w.Xsave.p, w.Xw.p, s);
+        --total_evals;
+        continue;
+      }
+      const bool res_ok = w.h_st->res_fail == 0;
+      const bool delta_ok = (iters == 0) || w.h_st->delta_fail == 0;
+      max_f = bits_to_double(w.h_st->maxf_bits);
+      conv = res_ok && delta_ok;
+      ++iters;
+    }
+    // final build at X_final for Q, I_vsource, max_res_contrib (:689-713)
+    int rc = upload_scalars(c0, c1, d1, c2, gshunt, iters);
+    if (rc < 0) return rc;
+    circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, floor, nullptr, w.f.p, w.Qn.p,
+                        w.mrc.p, w.Ivs.p, nullptr, s);
+    ++total_evals;
+    launch_dqdt(n_unk, c0, c1, d1, c2, w.Qn.p, Qp, dQ, Q2, w.dQn.p, s);
+    VJX_TRY_HIP(hipGetLastError());
+    return VJX_OK;
+  }
+};
+
+}  // namespace
+
+}  // namespace vjx
+
+using namespace vjx;
+
+extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx_tran_result* res) {
+  if (!c || !cfg || !res || !res->times || !res->V_out || !res->I_out) {
+    set_error("vjx_transient_run: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (cfg->max_steps < 2 || !(cfg->dt > 0.0) || cfg->tran_itl < 1 ||
+      (cfg->precision != VJX_PREC_F64 && cfg->precision != VJX_PREC_F32)) {
+    set_error("vjx_transient_run: invalid configuration");
+    return VJX_ERR_ARG;
+  }
+  Driver D;
+  D.c = c;
+  D.cfg = cfg;
+  int rc = pattern_stream(c->pat);
+  if (rc < 0) return rc;
+  D.s = c->pat->stream;
+  hipStream_t s = D.s;
+  const int n_total = c->n_total, n_unk = c->n_unk, m = c->m, k = c->k;
+  D.n_total = n_total;
+  D.n_unk = n_unk;
+  D.m = m;
+  D.k = k;
+  D.n_aug = c->n_aug;
+  Work& w = D.w;
+  const int nx = n_total + m;
+  const int max_hist = cfg->max_order + 2;
+  VJX_TRY_HIP(w.Xa.alloc(nx));
+  VJX_TRY_HIP(w.Xw.alloc(nx));
+  VJX_TRY_HIP(w.Xsave.alloc(nx));
+  VJX_TRY_HIP(w.Vp.alloc(n_total));
+  for (auto* b : {&w.Qp, &w.dQ, &w.Q2, &w.Qn, &w.dQn, &w.mrc, &w.hmrc, &w.floor, &w.zeros})
+    VJX_TRY_HIP(b->alloc(std::max(n_unk, 1)));
+  VJX_TRY_HIP(w.Vmax.alloc(n_total));
+  VJX_TRY_HIP(w.f.alloc(std::max(c->n_aug, 1)));
+  VJX_TRY_HIP(w.delta.alloc(std::max(c->n_aug, 1)));
+  VJX_TRY_HIP(w.csr.alloc(std::max(c->nse, 1)));
+  VJX_TRY_HIP(w.vsrc.alloc(std::max(m, 1)));
+  VJX_TRY_HIP(w.isrc.alloc(std::max(k, 1)));
+  VJX_TRY_HIP(w.Ivs.alloc(std::max(m, 1)));
+  VJX_TRY_HIP(w.hist.alloc((size_t)max_hist * n_total));
+  const int mo = std::max(m, 1);
+  VJX_TRY_HIP(w.vout.alloc((size_t)cfg->max_steps * n_total));
+  VJX_TRY_HIP(w.iout.alloc((size_t)cfg->max_steps * mo));
+  VJX_TRY_HIP(w.st.alloc(1));
+  VJX_TRY_HIP(hipHostMalloc((void**)&w.h_st, sizeof(DevStatus), hipHostMallocDefault));
+  launch_fill(n_unk, 0.0, w.zeros.p, s);
+  launch_fill((int)((size_t)cfg->max_steps * mo), 0.0, w.iout.p, s);
+  launch_fill(m, 0.0, w.Ivs.p, s);
+
+  // ---------------- initial state (full_mna.py:871-957) ----------------
+  auto t_dc0 = std::chrono::steady_clock::now();
+  std::vector<double> x0(nx, 0.0);
+  if (!cfg->uic && cfg->v_init_host)
+    for (int i = 0; i < n_total; ++i) x0[i] = cfg->v_init_host[i];
+  x0[0] = 0.0;
+  VJX_TRY_HIP(hipMemcpy(w.Xw.p, x0.data(), sizeof(double) * nx, hipMemcpyHostToDevice));
+  rc = D.set_sources(0.0);
+  if (rc < 0) return rc;
+  launch_fill(n_unk, cfg->abstol, w.floor.p, s);
+  int dc_iters = 0;
+  bool dc_conv = true;
+  double dc_maxf = 0.0;
+  if (cfg->uic) {
+    rc = D.upload_scalars(0.0, 0.0, 0.0, 0.0, 0.0, 1);
+    if (rc < 0) return rc;
+    circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, w.zeros.p, w.zeros.p, w.zeros.p, nullptr,
+                        nullptr, w.f.p, w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, s);
+    ++D.total_evals;
+    launch_fill(n_unk, 0.0, w.mrc.p, s);
+  } else {
+    rc = D.nr_solve(0.0, 0.0, 0.0, 0.0, 0.0, w.zeros.p, w.zeros.p, w.zeros.p, w.floor.p, dc_iters,
+                    dc_conv, dc_maxf);
+    if (rc < 0) return rc;
+  }
+  // state from the initial solution
+  launch_copy(nx, w.Xw.p, w.Xa.p, s);
+  launch_copy(n_unk, w.Qn.p, w.Qp.p, s);
+  launch_fill(n_unk, 0.0, w.dQ.p, s);
+  launch_fill(n_unk, 0.0, w.Q2.p, s);
+  launch_copy(n_unk, w.mrc.p, w.hmrc.p, s);
+  launch_fill((int)((size_t)max_hist * n_total), 0.0, w.hist.p, s);
+  launch_copy(n_total, w.Xa.p, w.hist.p, s);
+  // V_max_historic = |X0|: accept kernel on a zero vector computes max(0,|x|)
+  launch_fill(n_total, 0.0, w.Vmax.p, s);
+  launch_output(n_total, m, w.Xa.p, w.Ivs.p, 0, w.vout.p, w.iout.p, s);
+  {
+    // Vmax = |X0|
+    DBuf<double> tmp;
+    VJX_TRY_HIP(tmp.alloc(nx));
+    launch_accept(n_total, 0, 0, w.Xa.p, tmp.p, tmp.p, w.Vmax.p, nullptr, nullptr, nullptr,
+                  nullptr, nullptr, nullptr, nullptr, s);
+    VJX_TRY_HIP(hipStreamSynchronize(s));
+  }
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(s));
+  res->dc_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_dc0).count();
+  res->dc_converged = dc_conv ? 1 : 0;
+  res->dc_iterations = dc_iters;
+
+  std::vector<double> times(cfg->max_steps, 0.0);
+  int step_idx = 1;
+  int64_t total_nr = 0;
+  int rejected = 0;
+  double t = 0.0;
+  double dt = cfg->dt;
+  auto t0 = std::chrono::steady_clock::now();
+
+  if (cfg->fixed_step) {
+    // ---------------- fixed-step loop (full_mna.py:1198-1441) ----------------
+    if (cfg->tran_fs != 1.0) dt = dt * cfg->tran_fs;
+    int n_steps = (int)std::min<double>(std::floor(cfg->t_stop / dt), (double)(cfg->max_steps - 1));
+    if (n_steps < 1) n_steps = 1;
+    const double inv = 1.0 / dt;
+    double c0 = inv, c1 = -inv, d1 = 0.0, c2 = 0.0;
+    if (cfg->method == VJX_METHOD_TRAP) {
+      c0 = 2.0 * inv;
+      c1 = -2.0 * inv;
+      d1 = -1.0;
+    } else if (cfg->method == VJX_METHOD_GEAR2) {
+      c0 = 1.5 * inv;
+      c1 = -2.0 * inv;
+      c2 = 0.5 * inv;
+    }
+    launch_fill(n_unk, cfg->abstol, w.floor.p, s);
+    for (int i = 0; i < n_steps; ++i) {
+      const double t_next = ((double)i + 1.0) * dt;
+      rc = D.set_sources(t_next);
+      if (rc < 0) return rc;
+      int iters;
+      bool conv;
+      double maxf;
+      rc = D.nr_solve(c0, c1, d1, c2, 0.0, w.Qp.p, w.dQ.p, w.Q2.p, w.floor.p, iters, conv, maxf);
+      if (rc < 0) return rc;
+      total_nr += iters;
+      times[i + 1] = t_next;
+      launch_copy(nx, w.Xw.p, w.Xa.p, s);
+      launch_output(n_total, m, w.Xa.p, w.Ivs.p, 0, w.vout.p + (size_t)(i + 1) * n_total,
+                    w.iout.p + (size_t)(i + 1) * mo, s);
+      launch_copy(n_unk, w.Qp.p, w.Q2.p, s);
+      launch_copy(n_unk, w.Qn.p, w.Qp.p, s);
+      launch_copy(n_unk, w.dQn.p, w.dQ.p, s);
+    }
+    step_idx = n_steps + 1;
+    res->min_dt_used = dt;
+    res->max_dt_used = dt;
+  } else {
+    // ---------------- adaptive loop (full_mna.py:780-1196, 1840-2194) ----------------
+    double max_dt = cfg->max_dt;
+    if (cfg->tran_minpts > 0) {
+      const double hm = cfg->t_stop / cfg->tran_minpts;
+      const double eff = std::min(max_dt, hm);
+      if (eff < max_dt) max_dt = eff;
+    }
+    if (cfg->tran_fs != 1.0) dt = dt * cfg->tran_fs;
+    std::vector<double> dt_hist(max_hist, dt);
+    std::vector<int> slot(max_hist);
+    for (int j = 0; j < max_hist; ++j) slot[j] = j;
+    int hist_count = 1, warmup = 0, consec = 0;
+    double min_dt_used = dt, max_dt_used = dt;
+    auto hist_ptr = [&](int j) { return w.hist.p + (size_t)slot[j] * n_total; };
+    while (t < cfg->t_stop && step_idx < cfg->max_steps) {
+      const double dt_cur = dt;
+      const double inv = 1.0 / dt_cur;
+      double c0, c1, d1, c2, e_int;
+      if (cfg->method == VJX_METHOD_BE) {
+        c0 = inv; c1 = -inv; d1 = 0.0; c2 = 0.0; e_int = -0.5;
+      } else if (cfg->method == VJX_METHOD_TRAP) {
+        c0 = 2.0 * inv; c1 = -2.0 * inv; d1 = -1.0; c2 = 0.0; e_int = 1.0 / 24.0;
+      } else {
+        const double dt_prev = dt_hist[0];
+        const double om = dt_prev > 0 ? dt_cur / dt_prev : 1.0;
+        const double op1 = 1.0 + om;
+        c0 = (1.0 + 2.0 * om) / (op1 * dt_cur);
+        c1 = -op1 / dt_cur;
+        d1 = 0.0;
+        c2 = (om * om) / (op1 * dt_cur);
+        e_int = -2.0 / 9.0;
+      }
+      const double t_next = t + dt_cur;
+      rc = D.set_sources(t_next);
+      if (rc < 0) return rc;
+      const bool can_predict = (warmup >= cfg->warmup_steps) && (hist_count >= 2);
+      // predictor (adaptive.py:94-218)
+      int order = std::max(std::min(hist_count - 1, cfg->max_order), 0);
+      double cs0 = dt_hist[0], cs1 = dt_hist[0] + dt_hist[1];
+      double tau1 = -(dt_cur + cs0) / dt_cur;
+      double tau2 = -(dt_cur + cs1) / dt_cur;
+      double ratio1 = dt_cur / std::max(dt_hist[0], 1e-30);
+      double err1 = -ratio1 * (1 + ratio1) / 2;
+      const double tau0 = -1.0;
+      double den0 = (tau0 - tau1) * (tau0 - tau2);
+      double den1 = (tau1 - tau0) * (tau1 - tau2);
+      double den2 = (tau2 - tau0) * (tau2 - tau1);
+      if (std::fabs(den0) < 1e-30) den0 = 1e-30;
+      if (std::fabs(den1) < 1e-30) den1 = 1e-30;
+      if (std::fabs(den2) < 1e-30) den2 = 1e-30;
+      double l0 = (0 - tau1) * (0 - tau2) / den0;
+      double l1 = (0 - tau0) * (0 - tau2) / den1;
+      double l2 = (0 - tau0) * (0 - tau1) / den2;
+      double err2 = -tau0 * tau1 * tau2 / 6;
+      double a[3], pred_err;
+      if (order == 0) {
+        a[0] = 1.0; a[1] = 0.0; a[2] = 0.0; pred_err = 1.0;
+      } else if (order == 1) {
+        a[0] = 1.0 + ratio1; a[1] = -ratio1; a[2] = 0.0; pred_err = err1;
+      } else {
+        a[0] = l0; a[1] = l1; a[2] = l2; pred_err = err2;
+      }
+      for (int j = order + 1; j < 3; ++j) a[j] = 0.0;
+      launch_predict(n_total, m, w.Xa.p, hist_ptr(0), hist_ptr(1), hist_ptr(2), a[0], a[1], a[2],
+                     can_predict ? 1 : 0, w.Vp.p, w.Xw.p, s);
+      double ramp = cfg->gshunt_steps > 0
+                        ? std::min(std::max((double)step_idx / cfg->gshunt_steps, 0.0), 1.0)
+                        : 1.0;
+      const double gshunt = cfg->gshunt_init + ramp * (cfg->gshunt_target - cfg->gshunt_init);
+      launch_floor(n_unk, w.hmrc.p, cfg->reltol, cfg->abstol, w.floor.p, s);
+      int iters;
+      bool conv;
+      double maxf;
+      rc = D.nr_solve(c0, c1, d1, c2, gshunt, w.Qp.p, w.dQ.p, w.Q2.p, w.floor.p, iters, conv, maxf);
+      if (rc < 0) return rc;
+      total_nr += iters;
+      const bool nr_failed = !conv;
+      const bool at_min = dt_cur <= cfg->min_dt;
+      const bool fail_min = nr_failed && at_min;
+      // LTE (adaptive.py:221-339)
+      launch_status_reset(w.st.p, s);
+      launch_lte(n_total, w.Xw.p, w.Vp.p, w.Vmax.p, e_int / (e_int - pred_err), cfg->reltol,
+                 cfg->abstol, w.st.p, s);
+      rc = D.read_status();
+      if (rc < 0) return rc;
+      const double lte_norm = bits_to_double(w.h_st->maxf_bits);
+      const int lorder = std::min(hist_count - 1, cfg->max_order);
+      double factor = std::pow(cfg->lte_ratio / std::max(lte_norm, 1e-10), 1.0 / (lorder + 1));
+      factor = std::min(std::max(factor, 0.1), cfg->grow_factor);
+      const double dt_lte = dt_cur * factor;
+      const bool lte_reject = (dt_cur / dt_lte > cfg->redo_factor) && can_predict && conv;
+      const bool nr_reject = nr_failed && !at_min;
+      const bool force = consec >= cfg->max_consecutive_rejects;
+      const bool reject = (lte_reject || nr_reject) && !force;
+      const bool accept = !reject;
+      if (accept)
+        consec = 0;
+      else if (lte_reject)
+        ++consec;
+      const double dt_from_lte = can_predict ? dt_lte : dt_cur;
+      const double dt_halved = std::max(dt_cur * cfg->tran_ft, cfg->min_dt);
+      const double last_good = hist_count > 0 ? dt_hist[0] : cfg->min_dt * 16.0;
+      const double dt_rec = std::max(last_good / 8.0, cfg->min_dt * 2.0);
+      double new_dt = nr_failed ? (at_min ? dt_rec : dt_halved) : dt_from_lte;
+      new_dt = std::min(std::max(new_dt, cfg->min_dt), max_dt);
+      new_dt = std::min(new_dt, cfg->t_stop - t_next);
+      const bool use_new = accept && !fail_min;
+      if (accept) t = t_next;
+      if (use_new) {
+        // roll history: the oldest slot receives V_new and becomes slot 0
+        int last = slot[max_hist - 1];
+        for (int j = max_hist - 1; j > 0; --j) slot[j] = slot[j - 1];
+        slot[0] = last;
+        for (int j = max_hist - 1; j > 0; --j) dt_hist[j] = dt_hist[j - 1];
+        dt_hist[0] = dt_cur;
+        launch_accept(n_total, n_unk, m, w.Xw.p, w.Xa.p, hist_ptr(0), w.Vmax.p, w.Qn.p, w.dQn.p,
+                      w.Qp.p, w.dQ.p, w.Q2.p, w.mrc.p, w.hmrc.p, s);
+        hist_count = std::min(hist_count + 1, max_hist);
+        ++warmup;
+      }
+      if (accept) {
+        times[step_idx] = t_next;
+        launch_output(n_total, m, w.Xa.p, w.Ivs.p, fail_min ? 1 : 0,
+                      w.vout.p + (size_t)step_idx * n_total, w.iout.p + (size_t)step_idx * mo, s);
+        ++step_idx;
+        min_dt_used = std::min(min_dt_used, dt_cur);
+        max_dt_used = std::max(max_dt_used, dt_cur);
+      } else {
+        ++rejected;
+      }
+      dt = new_dt;
+    }
+    res->min_dt_used = min_dt_used;
+    res->max_dt_used = max_dt_used;
+  }
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(s));
+  res->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  res->n_steps = step_idx;
+  res->rejected_steps = rejected;
+  res->total_nr_iterations = total_nr;
+  res->total_evals = D.total_evals;
+  res->repivots = D.repivots;
+  std::memcpy(res->times, times.data(), sizeof(double) * step_idx);
+  VJX_TRY_HIP(hipMemcpy(res->V_out, w.vout.p, sizeof(double) * (size_t)step_idx * n_total,
+                        hipMemcpyDeviceToHost));
+  VJX_TRY_HIP(hipMemcpy(res->I_out, w.iout.p, sizeof(double) * (size_t)step_idx * mo,
+                        hipMemcpyDeviceToHost));
+  return dc_conv ? VJX_OK : VJX_WARN_NOCONV;
+}

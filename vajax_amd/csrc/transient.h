@@ -1,0 +1,30 @@
+// Launchers of the Newton / time-step element-wise kernels.
+#pragma once
+
+#include "internal.h"
+
+namespace vjx {
+
+void launch_delta(int n_unk, int m, int n_total, const double* delta, const double* X,
+                  const DevScalars* sc, DevStatus* st, hipStream_t s);
+void launch_update(int n_unk, int m, int n_total, const double* delta, double* X,
+                   const DevScalars* sc, const DevStatus* st, hipStream_t s);
+void launch_predict(int n_total, int m, const double* Xa, const double* h0, const double* h1,
+                    const double* h2, double a0, double a1, double a2, int can_predict, double* Vp,
+                    double* Xw, hipStream_t s);
+void launch_floor(int n, const double* hmrc, double reltol, double abstol, double* floor,
+                  hipStream_t s);
+void launch_fill(int n, double v, double* x, hipStream_t s);
+void launch_copy(int n, const double* a, double* b, hipStream_t s);
+void launch_dqdt(int n, double c0, double c1, double d1, double c2, const double* Q,
+                 const double* Qp, const double* dQ, const double* Q2, double* out,
+                 hipStream_t s);
+void launch_lte(int n, const double* Vn, const double* Vp, const double* Vmax, double scale,
+                double reltol, double abstol, DevStatus* st, hipStream_t s);
+void launch_accept(int n_total, int n_unk, int m, const double* Xw, double* Xa, double* hist_new,
+                   double* Vmax, const double* Qn, const double* dQn, double* Qp, double* dQ,
+                   double* Q2, const double* mrc, double* hmrc, hipStream_t s);
+void launch_output(int n_total, int m, const double* Xa, const double* Ivs, int zero_current,
+                   double* vrow, double* irow, hipStream_t s);
+
+}  // namespace vjx

@@ -1,0 +1,415 @@
+"""Transient analysis API mirroring vajax's ``CircuitEngine``.
+
+    engine = CircuitEngine(netlist)        # a vajax_amd.netlist.Netlist
+    engine.parse()                         # numbering, pattern, reduction maps
+    engine.prepare(t_stop=..., dt=..., precision="f64")
+    result = engine.run_transient()        # TransientResult(times, voltages, currents, stats)
+
+Contract followed: vajax/analysis/engine.py:115-189 (TransientResult),
+:421-485 (parse), :824-982 (prepare) and :985-1023 (run_transient);
+AdaptiveConfig / SimulationOptions defaults from
+vajax/analysis/transient/adaptive.py:37-91 and vajax/analysis/options.py:35-192.
+The time loop itself runs on the GPU through ``vjx_transient_run``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field, fields, replace
+from typing import Any, Dict, Optional
+
+import numpy as np
+
+from . import _lib as L
+from .models import param_block
+from .netlist import Netlist
+from .sparse import SparseLU
+from .topology import Topology, build_reduction_map, build_topology
+
+_METHOD_ALIASES = {"be": "be", "euler": "be", "backward_euler": "be", "trap": "trap",
+                   "trapezoidal": "trap", "am2": "trap", "gear2": "gear2", "gear": "gear2",
+                   "bdf2": "gear2", "bdf": "gear2"}
+
+
+@dataclass
+class AdaptiveConfig:
+    """Step-control parameters (vajax/analysis/transient/adaptive.py:37-91)."""
+
+    lte_ratio: float = 3.5
+    redo_factor: float = 2.5
+    reltol: float = 1e-3
+    abstol: float = 1e-12
+    min_dt: float = 1e-15
+    max_dt: float = float("inf")
+    tran_minpts: int = 50
+    warmup_steps: int = 2
+    max_order: int = 2
+    grow_factor: float = 2.0
+    nr_convtol: float = 1.0
+    gshunt_init: float = 0.0
+    gshunt_steps: int = 5
+    gshunt_target: float = 0.0
+    tran_fs: float = 0.25
+    tran_ft: float = 0.25
+    integration_method: str = "trap"
+    max_consecutive_rejects: int = 5
+
+
+@dataclass
+class SimulationOptions:
+    """Options used on the transient path (vajax/analysis/options.py:35-192)."""
+
+    nr_damping: float = 1.0
+    tran_itl: int = 20
+    reltol: float = 1e-3
+    abstol: float = 1e-12
+    vntol: float = 1e-6
+    gmin: float = 1e-12
+    icmode: str = "op"
+    fixed_step_transient: bool = False
+
+
+@dataclass
+class TransientResult:
+    """vajax/analysis/engine.py:115-189."""
+
+    times: np.ndarray
+    voltages: Dict[str, np.ndarray]
+    currents: Dict[str, np.ndarray]
+    stats: Dict[str, Any]
+
+    @property
+    def num_steps(self) -> int:
+        return len(self.times)
+
+    def voltage(self, node: str) -> np.ndarray:
+        if node in self.voltages:
+            return self.voltages[node]
+        low = node.lower()
+        for k, v in self.voltages.items():
+            if k.lower() == low:
+                return v
+        raise KeyError(f"Node '{node}' not found. Available: {self.node_names}")
+
+    @property
+    def node_names(self):
+        return list(self.voltages.keys())
+
+    def current(self, source: str) -> np.ndarray:
+        if source in self.currents:
+            return self.currents[source]
+        low = source.lower()
+        for k, v in self.currents.items():
+            if k.lower() == low:
+                return v
+        raise KeyError(f"Source '{source}' not found. Available: {self.source_names}")
+
+    @property
+    def source_names(self):
+        return list(self.currents.keys())
+
+
+def method_name(m: str) -> str:
+    try:
+        return _METHOD_ALIASES[str(m).lower()]
+    except KeyError:
+        raise ValueError(f"Unknown integration method: {m}. Supported: be, trap, gear2") from None
+
+
+def _soa(a: np.ndarray) -> np.ndarray:
+    """[N, k] -> contiguous int32 [k, N]."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64).T, dtype=np.int32)
+
+
+class DeviceCircuit:
+    """A circuit resident on the GPU (vjx_circuit handle)."""
+
+    def __init__(self, topo: Topology, net: Netlist):
+        self.topo = topo
+        self.net = net
+        red = build_reduction_map(topo)
+        self.red = red
+        n_unk, m, k = topo.n_unknowns, topo.n_vsources, net.isources.count
+        nse = topo.nse
+        keep = []  # keep numpy buffers alive while the C side copies them
+
+        def hold(a):
+            keep.append(a)
+            return a
+
+        slot_diag = np.zeros(nse, np.int32)
+        slot_diag[topo.diag_pos] = 1
+        slot_b = np.zeros(nse, np.float64)
+        for key, val in (("b_p", 1.0), ("b_n", -1.0), ("bt_p", 1.0), ("bt_n", -1.0)):
+            pos = topo.vs_pos.get(key, np.zeros(0, np.int32))
+            pos = pos[pos >= 0]
+            if np.any(slot_b[pos] != 0.0):
+                raise ValueError("voltage source with both terminals on one node")
+            slot_b[pos] += val
+        br_ptr = np.zeros(n_unk + 1, np.int32)
+        ents = [[] for _ in range(n_unk)]
+        for kk in range(m):
+            p = int(topo.vs_node_p[kk])
+            if p > 0:
+                ents[p - 1].append(kk)
+        for kk in range(m):
+            nn = int(topo.vs_node_n[kk])
+            if nn > 0:
+                ents[nn - 1].append(~kk)
+        br_ptr[1:] = np.cumsum([len(e) for e in ents])
+        br_ent = np.array([x for e in ents for x in e], dtype=np.int32)
+
+        def src_params(src):
+            cols = ("dc", "val0", "val1", "rise", "fall", "width", "period", "delay")
+            if src.count == 0:
+                return np.zeros((8, 0))
+            return np.ascontiguousarray(np.stack([src.params[c] for c in cols]), dtype=np.float64)
+
+        d = L.CircuitDesc()
+        d.n_total = topo.n_total
+        d.n_vsources = m
+        d.n_isources = k
+        d.nse = nse
+        d.indptr = L.ptr(hold(np.ascontiguousarray(topo.indptr, np.int32)))
+        d.indices = L.ptr(hold(np.ascontiguousarray(topo.indices, np.int32)))
+        d.slot_diag = L.ptr(hold(slot_diag))
+        d.slot_b = L.ptr(hold(slot_b), C.c_double)
+        d.vs_node_p = L.ptr(hold(np.ascontiguousarray(topo.vs_node_p, np.int32)))
+        d.vs_node_n = L.ptr(hold(np.ascontiguousarray(topo.vs_node_n, np.int32)))
+        d.vs_wave = L.ptr(hold(np.ascontiguousarray(net.vsources.wave, np.int32)))
+        d.vs_params = L.ptr(hold(src_params(net.vsources)), C.c_double)
+        d.is_wave = L.ptr(hold(np.ascontiguousarray(net.isources.wave, np.int32)))
+        d.is_params = L.ptr(hold(src_params(net.isources)), C.c_double)
+        d.is_dst = L.ptr(hold(np.ascontiguousarray(red.is_dst, np.int32)))
+        d.n_jac_contrib = red.n_jac_contrib
+        d.n_res_contrib = red.n_res_contrib
+        d.jac_seg = L.ptr(hold(np.ascontiguousarray(red.jac_seg, np.int32)))
+        d.res_seg = L.ptr(hold(np.ascontiguousarray(red.res_seg, np.int32)))
+        d.br_ptr = L.ptr(hold(br_ptr))
+        d.br_ent = L.ptr(hold(br_ent))
+        d.conv_mask = None
+        models = list(topo.models.items())
+        if len(models) > L.MAX_MODELS:
+            raise ValueError("too many model groups")
+        d.n_models = len(models)
+        for i, (name, ms) in enumerate(models):
+            md = d.models[i]
+            md.model_id = L.MODEL_IDS[name]
+            md.count = ms.count
+            n_jac = ms.csr_pos.shape[1]
+            n_res = ms.res_idx.shape[1]
+            md.v_node1 = L.ptr(hold(_soa(ms.v_node1)))
+            md.v_node2 = L.ptr(hold(_soa(ms.v_node2)))
+            md.jac_dst = L.ptr(hold(_soa(red.jac_dst[name].reshape(ms.count, n_jac))))
+            md.res_dst = L.ptr(hold(_soa(red.res_dst[name].reshape(ms.count, n_res))))
+            md.params = L.ptr(hold(param_block(name, ms.params)), C.c_double)
+        h = C.c_void_p()
+        L.check(L.lib.vjx_circuit_create(C.byref(d), C.byref(h)), "vjx_circuit_create")
+        self.h = h
+        del keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            L.lib.vjx_circuit_destroy(self.h)
+            self.h = None
+
+    @property
+    def lu(self) -> SparseLU:
+        ph = L.lib.vjx_circuit_pattern(self.h)
+        return SparseLU(self.topo.indptr, self.topo.indices, _handle=ph, _owner=self)
+
+    def build_system(self, X, vsrc, isrc, Q_prev=None, dQdt_prev=None, Q_prev2=None, c0=0.0,
+                     c1=0.0, d1=0.0, c2=0.0, gmin=1e-12, gshunt=0.0, want_jacobian=True):
+        """Device build_system on torch CUDA tensors; returns a dict of tensors."""
+        import torch
+
+        t = self.topo
+        dev = X.device
+        out = {
+            "csr_data": torch.empty(t.nse, dtype=torch.float64, device=dev) if want_jacobian else None,
+            "f": torch.empty(t.n_augmented, dtype=torch.float64, device=dev),
+            "Q": torch.empty(t.n_unknowns, dtype=torch.float64, device=dev),
+            "I_vsource": torch.empty(max(t.n_vsources, 1), dtype=torch.float64, device=dev),
+            "max_res_contrib": torch.empty(t.n_unknowns, dtype=torch.float64, device=dev),
+        }
+
+        def p(x):
+            return C.c_void_p(x.data_ptr()) if x is not None else None
+
+        torch.cuda.synchronize()
+        L.check(L.lib.vjx_build_system(self.h, p(X), p(vsrc), p(isrc), p(Q_prev), p(dQdt_prev),
+                                       p(Q_prev2), c0, c1, d1, c2, gmin, gshunt,
+                                       p(out["csr_data"]), p(out["f"]), p(out["Q"]),
+                                       p(out["I_vsource"]), p(out["max_res_contrib"])),
+                "vjx_build_system")
+        return out
+
+
+def init_mid_rail(topo: Topology, net: Netlist) -> np.ndarray:
+    """Initial guess for the DC solve (full_mna.py:694-745, sources.py:553-568)."""
+    vs = net.vsources
+    vdd = 0.0
+    for i in range(vs.count):
+        if vs.params["dc"][i] > vdd:
+            vdd = float(vs.params["dc"][i])
+    V = np.full(topo.n_total, vdd / 2.0)
+    V[0] = 0.0
+    for name, idx in topo.node_names.items():
+        low = name.lower()
+        if "vdd" in low or "vcc" in low:
+            V[idx] = vdd
+        elif low in ("gnd", "vss", "0"):
+            V[idx] = 0.0
+    for i in range(vs.count):
+        p, n = int(topo.vs_node_p[i]), int(topo.vs_node_n[i])
+        if n == 0 and p > 0:
+            V[p] = float(vs.params["dc"][i])
+    return V
+
+
+class CircuitEngine:
+    """Parse / prepare / run_transient on a flattened netlist."""
+
+    def __init__(self, netlist: Netlist):
+        if not isinstance(netlist, Netlist):
+            raise TypeError("CircuitEngine expects a vajax_amd.netlist.Netlist")
+        self.netlist = netlist
+        self.options = SimulationOptions()
+        self.analysis_params: Dict[str, Any] = dict(netlist.analysis)
+        for k in ("icmode",):
+            if k in netlist.options:
+                setattr(self.options, k, netlist.options[k])
+        self.topology: Optional[Topology] = None
+        self.device: Optional[DeviceCircuit] = None
+        self.node_names: Dict[str, int] = {}
+        self._prepared = False
+
+    def parse(self):
+        self.topology = build_topology(self.netlist)
+        self.node_names = dict(self.topology.node_names)
+        self.num_nodes = self.topology.n_total
+        self.device = DeviceCircuit(self.topology, self.netlist)
+        self._prepared = False
+        return self
+
+    def prepare(self, *, t_stop: Optional[float] = None, dt: Optional[float] = None,
+                adaptive_config: Optional[AdaptiveConfig] = None, precision: str = "f64",
+                refine_steps: Optional[int] = None, max_steps: Optional[int] = None,
+                use_graph: bool = False, **_ignored) -> None:
+        if self.topology is None:
+            self.parse()
+        if t_stop is None:
+            t_stop = float(self.analysis_params.get("stop", 1e-3))
+        if dt is None:
+            dt = float(self.analysis_params.get("step", 1e-6))
+        if max_steps is None:
+            max_steps = int(t_stop / dt * 1.1) + 10
+        kw = {}
+        opts = self.netlist.options
+        mapping = {"tran_lteratio": "lte_ratio", "tran_redofactor": "redo_factor",
+                   "tran_gshunt": "gshunt_init", "reltol": "reltol", "abstol": "abstol",
+                   "tran_fs": "tran_fs", "tran_ft": "tran_ft", "tran_minpts": "tran_minpts",
+                   "maxstep": "max_dt", "tran_method": "integration_method"}
+        for src, dst in mapping.items():
+            if src in opts:
+                kw[dst] = opts[src]
+        cfg = AdaptiveConfig(**kw)
+        if adaptive_config is not None:
+            defaults = AdaptiveConfig()
+            over = {f.name: getattr(adaptive_config, f.name) for f in fields(AdaptiveConfig)
+                    if getattr(adaptive_config, f.name) != getattr(defaults, f.name)}
+            if over:
+                cfg = replace(cfg, **over)
+        if precision not in ("f64", "f32"):
+            raise ValueError("precision must be 'f64' or 'f32'")
+        self._prep = dict(t_stop=float(t_stop), dt=float(dt), cfg=cfg, precision=precision,
+                          refine_steps=(1 if precision == "f32" else 0) if refine_steps is None
+                          else int(refine_steps), max_steps=int(max_steps), use_graph=use_graph)
+        self._prepared = True
+
+    def _tran_config(self) -> L.TranConfig:
+        p = self._prep
+        cfg: AdaptiveConfig = p["cfg"]
+        o = self.options
+        tc = L.TranConfig()
+        tc.t_stop = p["t_stop"]
+        tc.dt = p["dt"]
+        tc.max_steps = p["max_steps"]
+        tc.method = L.VJX_METHOD[method_name(cfg.integration_method)]
+        tc.fixed_step = 1 if o.fixed_step_transient else 0
+        tc.uic = 1 if str(o.icmode).lower() == "uic" else 0
+        tc.precision = L.VJX_PREC_F32 if p["precision"] == "f32" else L.VJX_PREC_F64
+        tc.refine_steps = p["refine_steps"]
+        tc.tran_itl = o.tran_itl
+        tc.use_graph = 1 if p["use_graph"] else 0
+        tc.reltol = o.reltol
+        tc.abstol = o.abstol
+        tc.vntol = o.vntol
+        tc.gmin = o.gmin
+        tc.nr_damping = o.nr_damping
+        tc.lte_ratio = cfg.lte_ratio
+        tc.redo_factor = cfg.redo_factor
+        tc.min_dt = cfg.min_dt
+        tc.max_dt = cfg.max_dt
+        tc.tran_minpts = cfg.tran_minpts
+        tc.warmup_steps = cfg.warmup_steps
+        tc.max_order = cfg.max_order
+        tc.grow_factor = cfg.grow_factor
+        tc.gshunt_init = cfg.gshunt_init
+        tc.gshunt_target = cfg.gshunt_target
+        tc.gshunt_steps = cfg.gshunt_steps
+        tc.tran_fs = cfg.tran_fs
+        tc.tran_ft = cfg.tran_ft
+        tc.max_consecutive_rejects = cfg.max_consecutive_rejects
+        if cfg.reltol != o.reltol or cfg.abstol != o.abstol:
+            # the LTE uses the adaptive config's tolerances, NR the options';
+            # one pair is carried, so they must agree
+            raise ValueError("AdaptiveConfig and SimulationOptions tolerances differ")
+        return tc
+
+    def run_transient(self) -> TransientResult:
+        if not self._prepared:
+            self.prepare()
+        topo = self.topology
+        tc = self._tran_config()
+        ms = tc.max_steps
+        n_total, m = topo.n_total, topo.n_vsources
+        times = np.zeros(ms)
+        V = np.zeros((ms, n_total))
+        I = np.zeros((ms, max(m, 1)))
+        v0 = np.ascontiguousarray(init_mid_rail(topo, self.netlist))
+        tc.v_init_host = L.ptr(v0, C.c_double)
+        res = L.TranResult()
+        res.times = L.ptr(times, C.c_double)
+        res.V_out = L.ptr(V, C.c_double)
+        res.I_out = L.ptr(I, C.c_double)
+        rc = L.check(L.lib.vjx_transient_run(self.device.h, C.byref(tc), C.byref(res)),
+                     "vjx_transient_run")
+        n = res.n_steps
+        stats = {
+            "n_steps": n,
+            "total_timesteps": n,
+            "accepted_steps": n if not tc.fixed_step else n - 1,
+            "rejected_steps": res.rejected_steps,
+            "total_nr_iterations": int(res.total_nr_iterations),
+            "total_evals": int(res.total_evals),
+            "avg_nr_iterations": res.total_nr_iterations / max(n, 1),
+            "wall_time": res.wall_time,
+            "time_per_step_ms": res.wall_time / n * 1000 if n else 0.0,
+            "min_dt_used": res.min_dt_used,
+            "max_dt_used": res.max_dt_used,
+            "convergence_rate": n / max(n + res.rejected_steps, 1),
+            "dc_converged": bool(res.dc_converged),
+            "dc_iterations": res.dc_iterations,
+            "dc_time": res.dc_time,
+            "repivots": res.repivots,
+            "strategy": "fixed_step_full_mna" if tc.fixed_step else "adaptive_full_mna",
+            "precision": self._prep["precision"],
+            "return_code": rc,
+            "V_out": V[:n],
+            "I_out": I[:n],
+        }
+        voltages = {name: V[:n, idx] for name, idx in topo.node_names.items() if idx > 0}
+        currents = {name: I[:n, i] for i, name in enumerate(self.netlist.vsources.names)}
+        return TransientResult(times=times[:n], voltages=voltages, currents=currents, stats=stats)

@@ -1,0 +1,275 @@
+"""Seeded synthetic circuits used for parity tests and benchmarks.
+
+All generators draw from ``numpy.random.default_rng(seed)`` so a (generator,
+size, seed) triple names one circuit exactly.  Node names are zero-padded so
+that Python's sorted order (the reference's node numbering,
+vajax/analysis/engine.py:459-468) equals construction order.
+
+* :func:`rc_ladder` -- series R / shunt C low-pass ladder driven by a pulse.
+* :func:`ring_oscillator` -- odd-stage CMOS inverter ring with load caps and a
+  one-shot current kick on the first stage output.
+* :func:`gate_dag` -- random DAG of 2-input NAND/NOR gates with bounded fanout
+  and 64 pulse-train primary inputs.  Gate inputs are drawn from a window of
+  recently created signals (geometric distance), which gives the netlist the
+  locality of a placed design instead of the expander structure of a uniform
+  random graph.
+* :func:`monte_carlo` -- per-device threshold/mobility variation of a netlist.
+
+MOSFETs use the EKV long-channel substitute model (``mosfet`` model type) at
+the technology values in :data:`NMOS_DEFAULTS` / :data:`PMOS_DEFAULTS`.
+"""
+
+from __future__ import annotations
+
+import copy
+from typing import Dict
+
+import numpy as np
+
+from .netlist import (
+    SOURCE_DEFAULTS,
+    WAVE_DC,
+    WAVE_PULSE,
+    DeviceGroup,
+    Netlist,
+    NetlistBuilder,
+    SourceGroup,
+    empty_sources,
+)
+
+NMOS_DEFAULTS: Dict[str, float] = {
+    "type": 1.0,
+    "vto": 0.35,
+    "gamma": 0.4,
+    "phi": 0.8,
+    "kp": 200e-6,
+    "theta": 0.2,
+    "cox": 0.02,
+    "cov": 2e-10,
+    "cj": 5e-10,
+}
+PMOS_DEFAULTS: Dict[str, float] = {
+    "type": -1.0,
+    "vto": 0.35,
+    "gamma": 0.4,
+    "phi": 0.8,
+    "kp": 80e-6,
+    "theta": 0.2,
+    "cox": 0.02,
+    "cov": 2e-10,
+    "cj": 5e-10,
+}
+
+VDD = 1.0
+
+
+def rc_ladder(sections: int = 200, seed: int = 0, t_stop: float = 10e-9, dt: float = 1e-12) -> Netlist:
+    """RC low-pass ladder: in -> R1 -> n001 -> R2 -> ... -> n{sections}.
+
+    R_i ~ U[1k, 10k] ohm in series, C_i ~ U[0.1, 1] pF from n_i to ground,
+    driven by a 0->1 V pulse with 10 ps rise/fall and 1 ns width.
+    """
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(1e3, 1e4, sections)
+    c = rng.uniform(1e-13, 1e-12, sections)
+    width = len(str(sections))
+    b = NetlistBuilder(f"rc_ladder_{sections}_s{seed}")
+    b.source("vsource", "v1", "in", "0", type="pulse", val0=0.0, val1=1.0, rise=10e-12,
+             fall=10e-12, width=1e-9, delay=0.0, period=1.0)
+    prev = "in"
+    for i in range(sections):
+        node = f"n{i + 1:0{max(3, width)}d}"
+        b.device("resistor", f"r{i + 1}", [prev, node], r=r[i])
+        b.device("capacitor", f"c{i + 1}", [node, "0"], c=c[i])
+        prev = node
+    return b.build(analysis={"stop": t_stop, "step": dt},
+                   options={"tran_method": "gear2", "icmode": "op"})
+
+
+def ring_oscillator(stages: int = 101, seed: int = 0, t_stop: float = 50e-9, dt: float = 1e-12) -> Netlist:
+    """CMOS inverter ring with per-stage random W/L and load caps.
+
+    Each stage: NMOS(out_i, in_i, 0, 0) and PMOS(out_i, in_i, vdd, vdd) with
+    in_i = out_{i-1}; W ~ U[0.2, 2] um, L ~ U[45, 90] nm, C_load ~ U[1, 5] fF.
+    Started from all-zero initial conditions (icmode uic) with a 100 uA,
+    50 ps current pulse drawn out of stage 0.
+    """
+    if stages % 2 == 0:
+        raise ValueError("ring needs an odd number of stages")
+    rng = np.random.default_rng(seed)
+    wn = rng.uniform(0.2e-6, 2e-6, stages)
+    ln = rng.uniform(45e-9, 90e-9, stages)
+    wp = rng.uniform(0.2e-6, 2e-6, stages)
+    lp = rng.uniform(45e-9, 90e-9, stages)
+    cl = rng.uniform(1e-15, 5e-15, stages)
+    b = NetlistBuilder(f"ring_{stages}_s{seed}")
+    b.source("vsource", "vdd", "vdd", "0", type="dc", dc=VDD)
+    out = [f"out{i:03d}" for i in range(stages)]
+    for i in range(stages):
+        src = out[i - 1]
+        b.device("mosfet", f"mn{i:03d}", [out[i], src, "0", "0"], w=wn[i], l=ln[i], **NMOS_DEFAULTS)
+        b.device("mosfet", f"mp{i:03d}", [out[i], src, "vdd", "vdd"], w=wp[i], l=lp[i], **PMOS_DEFAULTS)
+        b.device("capacitor", f"cl{i:03d}", [out[i], "0"], c=cl[i])
+    b.source("isource", "ikick", out[0], "0", type="pulse", val0=0.0, val1=100e-6, rise=10e-12,
+             fall=10e-12, width=50e-12, delay=0.0, period=1.0)
+    return b.build(analysis={"stop": t_stop, "step": dt},
+                   options={"tran_method": "gear2", "icmode": "uic"})
+
+
+def _dag_inputs(rng: np.random.Generator, n_gates: int, n_pi: int, fanout_max: int,
+                window: int, mean_dist: float) -> np.ndarray:
+    """Pick two distinct driver signals per gate from the recent-signal window."""
+    n_sig = n_pi + n_gates
+    fanout = np.zeros(n_sig, dtype=np.int64)
+    inputs = np.empty((n_gates, 2), dtype=np.int64)
+    p = 1.0 / mean_dist
+    # pre-draw candidate distances in bulk; fall back to a scan when exhausted
+    draws = rng.geometric(p, size=(n_gates, 24))
+    for g in range(n_gates):
+        avail = n_pi + g  # signals 0..avail-1 exist
+        lo = max(0, avail - window)
+        chosen = []
+        for d in draws[g]:
+            if len(chosen) == 2:
+                break
+            if d > avail - lo:
+                continue
+            s = avail - int(d)
+            if fanout[s] >= fanout_max or s in chosen:
+                continue
+            chosen.append(s)
+        if len(chosen) < 2:
+            # deterministic fallback: nearest signals with spare fanout
+            s = avail - 1
+            while len(chosen) < 2 and s >= 0:
+                if fanout[s] < fanout_max and s not in chosen:
+                    chosen.append(s)
+                s -= 1
+            if len(chosen) < 2:
+                raise RuntimeError("gate DAG generator ran out of fanout capacity")
+        for s in chosen:
+            fanout[s] += 1
+        inputs[g] = chosen
+    return inputs
+
+
+def gate_dag(n_gates: int = 20000, seed: int = 0, n_pi: int = 64, fanout_max: int = 4,
+             window: int = 96, mean_dist: float = 12.0, t_stop: float = 1e-9,
+             dt: float = 1e-12, vdd_rise: float = 100e-12) -> Netlist:
+    """Random DAG of 2-input NAND/NOR gates (4 MOSFETs each).
+
+    Gate g picks NAND or NOR with probability 1/2 and two distinct inputs
+    among the last ``window`` signals (primary inputs and earlier gate
+    outputs) at geometric distance with mean ``mean_dist``; a signal drives
+    at most ``fanout_max`` gate inputs.  Each gate adds an output node
+    ``g<k>y`` and a series-stack node ``g<k>m``.  Primary inputs are pulse
+    trains: delay ~ U[0, 1 ns], width ~ U[0.2, 2] ns, rise = fall = 20 ps,
+    period = 2 width + rise + fall.  W ~ U[0.2, 2] um, L ~ U[45, 90] nm.
+    The transient starts from zero initial conditions (icmode uic) with the
+    supply ramped 0 -> 1 V over ``vdd_rise`` (a power-up): the DC point of a CMOS netlist
+    whose cut-off stack nodes are held only by gmin is numerically singular
+    for plain Newton from the mid-rail guess, while every transient Newton
+    matrix is regularised by c0*C.
+    """
+    rng = np.random.default_rng(seed)
+    is_nand = rng.random(n_gates) < 0.5
+    inputs = _dag_inputs(rng, n_gates, n_pi, fanout_max, window, mean_dist)
+    n_mos = 4 * n_gates
+    w = rng.uniform(0.2e-6, 2e-6, n_mos)
+    l = rng.uniform(45e-9, 90e-9, n_mos)
+    pi_delay = rng.uniform(0.0, 1e-9, n_pi)
+    pi_width = rng.uniform(0.2e-9, 2e-9, n_pi)
+
+    gw = len(str(n_gates))
+    gw = max(6, gw)
+    pw = max(2, len(str(n_pi)))
+    # node table: ground, vdd, PIs, then per gate (m, y)
+    nodes = ["0", "vdd"] + [f"pi{i:0{pw}d}" for i in range(n_pi)]
+    base = len(nodes)
+    gate_names_m = [f"g{g:0{gw}d}m" for g in range(n_gates)]
+    gate_names_y = [f"g{g:0{gw}d}y" for g in range(n_gates)]
+    inter = [None] * (2 * n_gates)
+    inter[0::2] = gate_names_m
+    inter[1::2] = gate_names_y
+    nodes.extend(inter)
+    GND, VDDN = 0, 1
+    sig_node = np.empty(n_pi + n_gates, dtype=np.int64)
+    sig_node[:n_pi] = 2 + np.arange(n_pi)
+    m_node = base + 2 * np.arange(n_gates)
+    y_node = m_node + 1
+    sig_node[n_pi:] = y_node
+    a = sig_node[inputs[:, 0]]
+    bb = sig_node[inputs[:, 1]]
+
+    # four devices per gate in order: P1, P2, N1, N2
+    term = np.empty((n_gates, 4, 4), dtype=np.int64)
+    typ = np.empty((n_gates, 4), dtype=np.float64)
+    typ[:, 0:2] = -1.0
+    typ[:, 2:4] = 1.0
+    vdd_col = np.full(n_gates, VDDN)
+    gnd_col = np.full(n_gates, GND)
+    # NAND: P(y,a,vdd,vdd) P(y,b,vdd,vdd) N(y,a,m,0) N(m,b,0,0)
+    # NOR:  P(m,a,vdd,vdd) P(y,b,m,vdd)   N(y,a,0,0) N(y,b,0,0)
+    nd = is_nand
+    term[:, 0] = np.stack([np.where(nd, y_node, m_node), a, vdd_col, vdd_col], axis=1)
+    term[:, 1] = np.stack([y_node, bb, np.where(nd, vdd_col, m_node), vdd_col], axis=1)
+    term[:, 2] = np.stack([y_node, a, np.where(nd, m_node, gnd_col), gnd_col], axis=1)
+    term[:, 3] = np.stack([np.where(nd, m_node, y_node), bb, gnd_col, gnd_col], axis=1)
+    terminals = term.reshape(n_mos, 4)
+    typ = typ.reshape(n_mos)
+
+    params = {"type": typ, "w": w, "l": l}
+    for k in ("vto", "gamma", "phi", "kp", "theta", "cox", "cov", "cj"):
+        params[k] = np.where(typ > 0, NMOS_DEFAULTS[k], PMOS_DEFAULTS[k]).astype(np.float64)
+    mos = DeviceGroup(model="mosfet", terminals=terminals, params=params, names=None)
+
+    m_src = 1 + n_pi
+    vs_terms = np.zeros((m_src, 2), dtype=np.int64)
+    vs_terms[0] = [VDDN, GND]
+    vs_terms[1:, 0] = 2 + np.arange(n_pi)
+    vs_wave = np.full(m_src, WAVE_PULSE, dtype=np.int32)
+    vsp = {k: np.full(m_src, v, dtype=np.float64) for k, v in SOURCE_DEFAULTS.items()}
+    # supply: power-up ramp 0 -> VDD over vdd_rise, then held (one pulse with
+    # a period far beyond any run)
+    vsp["dc"][0] = VDD
+    vsp["val0"][0] = 0.0
+    vsp["val1"][0] = VDD
+    vsp["rise"][0] = vdd_rise
+    vsp["width"][0] = 1.0
+    vsp["period"][0] = 10.0
+    vsp["delay"][0] = 0.0
+    vsp["val0"][1:] = 0.0
+    vsp["val1"][1:] = VDD
+    vsp["rise"][1:] = 20e-12
+    vsp["fall"][1:] = 20e-12
+    vsp["width"][1:] = pi_width
+    vsp["period"][1:] = 2 * pi_width + 40e-12
+    vsp["delay"][1:] = pi_delay
+    vs = SourceGroup(kind="vsource", names=["vdd"] + [f"vpi{i:0{pw}d}" for i in range(n_pi)],
+                     terminals=vs_terms, wave=vs_wave, params=vsp)
+    return Netlist(
+        title=f"gate_dag_{n_gates}_s{seed}",
+        nodes=nodes,
+        ground="0",
+        groups={"mosfet": mos},
+        vsources=vs,
+        isources=empty_sources("isource"),
+        analysis={"stop": t_stop, "step": dt},
+        options={"tran_method": "gear2", "icmode": "uic"},
+    )
+
+
+def monte_carlo(net: Netlist, seed: int, sigma: float = 0.03) -> Netlist:
+    """Copy of ``net`` with per-MOSFET Vth and mobility variation.
+
+    dVth ~ N(0, sigma * vto) added to vto, kp scaled by N(1, sigma).
+    """
+    rng = np.random.default_rng(seed)
+    out = copy.deepcopy(net)
+    out.title = f"{net.title}_mc{seed}"
+    g = out.groups.get("mosfet")
+    if g is not None:
+        n = g.count
+        g.params["vto"] = g.params["vto"] + rng.normal(0.0, 1.0, n) * sigma * g.params["vto"]
+        g.params["kp"] = g.params["kp"] * rng.normal(1.0, sigma, n)
+    return out

@@ -1,0 +1,217 @@
+"""Flat circuit description consumed by the transient core.
+
+A :class:`Netlist` is the flattened leaf-device view that the reference builds in
+``CircuitEngine.parse()`` (vajax/analysis/engine.py:421-485): a set of named
+nodes, one ground node, and device instances grouped by model type.  Instances
+of one model are stored as structure-of-arrays (terminal index columns and one
+float64 column per parameter), which is the layout the HIP eval kernels read.
+
+Node names are kept as given; numbering (ground = 0, the other names in Python
+sorted order from 1) is applied by :mod:`vajax_amd.topology`, following
+vajax/analysis/engine.py:459-468.
+
+Voltage and current sources carry a waveform record.  Supported waveform
+types are ``dc`` and ``pulse`` with the reference's parameter names and
+defaults (vajax/analysis/sources.py:204-227).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+# Model types known to the transient core, with their terminal names.  The
+# MOSFET entry is the EKV 2.6 long-channel core
+# (vajax/devices/models/ekv_longchannel/ekv_longchannel.va) extended with a
+# polarity flag and linear terminal capacitances; it stands in for PSP103,
+# whose Verilog-A source is not available.
+MODEL_TERMINALS: Dict[str, List[str]] = {
+    "resistor": ["A", "B"],
+    "capacitor": ["A", "B"],
+    "mosfet": ["d", "g", "s", "b"],
+}
+
+# Per-model instance parameters and defaults.
+MODEL_PARAMS: Dict[str, Dict[str, float]] = {
+    # vajax/devices/models/resistor.va:19
+    "resistor": {"r": 1.0},
+    # vajax/devices/models/capacitor.va:17
+    "capacitor": {"c": 1e-12},
+    # EKV long-channel defaults (ekv_longchannel.va:44-56) plus polarity and
+    # capacitance coefficients of this build's substitute model.
+    "mosfet": {
+        "type": 1.0,  # +1 NMOS, -1 PMOS
+        "w": 10e-6,
+        "l": 10e-6,
+        "vto": 0.5,
+        "gamma": 0.7,
+        "phi": 0.5,
+        "kp": 20e-6,
+        "theta": 50e-3,
+        "cox": 0.0,  # F/m^2, intrinsic gate capacitance per area
+        "cov": 0.0,  # F/m, gate overlap capacitance per width
+        "cj": 0.0,  # F/m, drain/source junction capacitance per width
+    },
+}
+
+SOURCE_DEFAULTS: Dict[str, float] = {
+    # vajax/analysis/sources.py:219-227 (pulse defaults)
+    "dc": 0.0,
+    "val0": 0.0,
+    "val1": 1.0,
+    "rise": 1e-9,
+    "fall": 1e-9,
+    "width": 1e-6,
+    "period": 2e-6,
+    "delay": 0.0,
+}
+
+WAVE_DC = 0
+WAVE_PULSE = 1
+WAVE_CODES = {"dc": WAVE_DC, "pulse": WAVE_PULSE}
+
+
+@dataclass
+class DeviceGroup:
+    """All instances of one model type, as structure-of-arrays."""
+
+    model: str
+    terminals: np.ndarray  # int64 [N, n_terminals], indices into Netlist.nodes
+    params: Dict[str, np.ndarray]  # name -> float64 [N]
+    names: Optional[List[str]] = None
+
+    @property
+    def count(self) -> int:
+        return int(self.terminals.shape[0])
+
+
+@dataclass
+class SourceGroup:
+    """Independent sources (vsource or isource) in device order."""
+
+    kind: str  # "vsource" | "isource"
+    names: List[str]
+    terminals: np.ndarray  # int64 [m, 2] (p, n)
+    wave: np.ndarray  # int32 [m] waveform code
+    params: Dict[str, np.ndarray]  # SOURCE_DEFAULTS keys -> float64 [m]
+
+    @property
+    def count(self) -> int:
+        return len(self.names)
+
+
+def empty_sources(kind: str) -> SourceGroup:
+    return SourceGroup(
+        kind=kind,
+        names=[],
+        terminals=np.zeros((0, 2), dtype=np.int64),
+        wave=np.zeros(0, dtype=np.int32),
+        params={k: np.zeros(0) for k in SOURCE_DEFAULTS},
+    )
+
+
+@dataclass
+class Netlist:
+    """Flattened circuit: nodes, device groups (model order = first appearance)."""
+
+    title: str
+    nodes: List[str]
+    ground: str
+    groups: Dict[str, DeviceGroup]
+    vsources: SourceGroup
+    isources: SourceGroup
+    analysis: Dict[str, float] = field(default_factory=dict)
+    options: Dict[str, object] = field(default_factory=dict)
+
+    def device_count(self) -> int:
+        return sum(g.count for g in self.groups.values())
+
+
+class NetlistBuilder:
+    """Incremental construction of a :class:`Netlist` from named instances."""
+
+    def __init__(self, title: str, ground: str = "0"):
+        self.title = title
+        self.ground = ground
+        self._node_index: Dict[str, int] = {}
+        self._nodes: List[str] = []
+        self._dev_terms: Dict[str, List[List[int]]] = {}
+        self._dev_params: Dict[str, Dict[str, List[float]]] = {}
+        self._dev_names: Dict[str, List[str]] = {}
+        self._src: Dict[str, Dict[str, list]] = {
+            k: {"names": [], "terms": [], "wave": [], **{p: [] for p in SOURCE_DEFAULTS}}
+            for k in ("vsource", "isource")
+        }
+        self.node(ground)
+
+    def node(self, name: str) -> int:
+        idx = self._node_index.get(name)
+        if idx is None:
+            idx = len(self._nodes)
+            self._node_index[name] = idx
+            self._nodes.append(name)
+        return idx
+
+    def device(self, model: str, name: str, terminals, **params) -> None:
+        if model not in MODEL_TERMINALS:
+            raise ValueError(f"unknown model type {model!r}")
+        if len(terminals) != len(MODEL_TERMINALS[model]):
+            raise ValueError(f"{model} needs {len(MODEL_TERMINALS[model])} terminals")
+        if model not in self._dev_terms:
+            self._dev_terms[model] = []
+            self._dev_params[model] = {p: [] for p in MODEL_PARAMS[model]}
+            self._dev_names[model] = []
+        self._dev_terms[model].append([self.node(t) for t in terminals])
+        defaults = MODEL_PARAMS[model]
+        unknown = set(params) - set(defaults)
+        if unknown:
+            raise ValueError(f"unknown {model} parameters {sorted(unknown)}")
+        for p, d in defaults.items():
+            self._dev_params[model][p].append(float(params.get(p, d)))
+        self._dev_names[model].append(name)
+
+    def source(self, kind: str, name: str, p: str, n: str, type: str = "dc", **params) -> None:
+        if kind not in self._src:
+            raise ValueError(f"unknown source kind {kind!r}")
+        if type not in WAVE_CODES:
+            raise ValueError(f"unsupported waveform {type!r}")
+        unknown = set(params) - set(SOURCE_DEFAULTS)
+        if unknown:
+            raise ValueError(f"unknown source parameters {sorted(unknown)}")
+        s = self._src[kind]
+        s["names"].append(name)
+        s["terms"].append([self.node(p), self.node(n)])
+        s["wave"].append(WAVE_CODES[type])
+        for k, d in SOURCE_DEFAULTS.items():
+            s[k].append(float(params.get(k, d)))
+
+    def build(self, analysis=None, options=None) -> Netlist:
+        groups = {}
+        for model, terms in self._dev_terms.items():
+            groups[model] = DeviceGroup(
+                model=model,
+                terminals=np.asarray(terms, dtype=np.int64).reshape(-1, len(MODEL_TERMINALS[model])),
+                params={k: np.asarray(v, dtype=np.float64) for k, v in self._dev_params[model].items()},
+                names=self._dev_names[model],
+            )
+        srcs = {}
+        for kind, s in self._src.items():
+            srcs[kind] = SourceGroup(
+                kind=kind,
+                names=list(s["names"]),
+                terminals=np.asarray(s["terms"], dtype=np.int64).reshape(-1, 2),
+                wave=np.asarray(s["wave"], dtype=np.int32),
+                params={k: np.asarray(s[k], dtype=np.float64) for k in SOURCE_DEFAULTS},
+            )
+        return Netlist(
+            title=self.title,
+            nodes=list(self._nodes),
+            ground=self.ground,
+            groups=groups,
+            vsources=srcs["vsource"],
+            isources=srcs["isource"],
+            analysis=dict(analysis or {}),
+            options=dict(options or {}),
+        )
