@@ -1,0 +1,220 @@
+"""Benchmark: transient time steps of the 65k-gate CMOS DAG on one MI355X.
+
+Workload (one "step" = one accepted transient time step, each running the full
+Newton loop: device eval, stamping, sparse LU refactorisation, triangular
+solves): the seeded gate DAG with 65 000 two-input NAND/NOR gates (260 000
+MOSFETs, ~130 000 unknowns), adaptive BDF2 from the power-up initial state.
+``--steps K`` steps are timed after ``--warmup W`` untimed steps (the warm-up
+run also performs the one-time symbolic analysis, which is excluded like the
+reference excludes JIT and setup).
+
+Prints one JSON line with ms/step and Newton iterations/s (``value``), a
+roofline estimate for the dominant phase from HIP events recorded on the
+library's own stream in a second, eager (non-graph) pass over the same steps,
+and a CPU baseline: the numpy/SciPy oracle (a restatement of the reference
+algorithm) timed on a bounded sample of the same workload on one host thread.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("ms/timestep + Newton iters/s at ~150k unknowns; batched instance·steps/s")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_circuit(gates: int, seed: int):
+    from vajax_amd import generators as G
+
+    return G.gate_dag(gates, seed=seed)
+
+
+def run_gpu(eng, steps: int, precision: str, use_graph: bool):
+    from vajax_amd.engine import AdaptiveConfig
+
+    eng.prepare(t_stop=1e-9, dt=1e-12, adaptive_config=AdaptiveConfig(integration_method="gear2"),
+                precision=precision, max_steps=steps + 1, use_graph=use_graph)
+    return eng.run_transient()
+
+
+def algorithmic_bytes(topo, lu_info, precision: str):
+    """Per-Newton-iteration algorithmic bytes of each phase (formulas of the
+    build's measurement plan): eval+stamp, LU refactor, solves."""
+    n_aug = topo.n_augmented
+    nse = topo.nse
+    b_eval = 0
+    b_stamp = 8 * nse + 8 * n_aug * 4
+    for name, ms in topo.models.items():
+        n = ms.count
+        n_v = ms.v_node1.shape[1]
+        n_par = {"resistor": 1, "capacitor": 1, "mosfet": 11}[name]
+        n_jac = ms.csr_pos.shape[1]
+        n_res = ms.res_idx.shape[1]
+        b_eval += n * (8 * n_par + n_v * (2 * 4 + 2 * 8))
+        b_stamp += n * (4 * n_jac + 4 * n_res)
+    nnz_lu = lu_info["nnz_l"] + lu_info["nnz_u"]
+    vb = 4 if precision == "f32" else 8
+    b_lu = (vb + 4) * nnz_lu
+    b_solve = (vb + 4) * nnz_lu + 16 * n_aug
+    if precision == "f32":
+        b_solve = 2 * b_solve + 12 * nse + 16 * n_aug
+    return {"build": b_eval + b_stamp, "lu": b_lu, "solve": b_solve}
+
+
+def cpu_baseline(net, steps: int):
+    from threadpoolctl import threadpool_limits
+
+    from oracle import mna
+    from oracle import transient as T
+
+    with threadpool_limits(limits=1):
+        cir = mna.setup(net)
+        cfg = T.AdaptiveConfig(integration_method="gear2")
+        out = T.run_adaptive(cir, 1e-9, 1e-12, cfg, T.Options(icmode="uic"), max_steps=steps + 1)
+    st = out.stats
+    acc = max(st["n_steps"] - 1, 1)
+    return {
+        "value": st["total_nr_iterations"] / st["wall_time"],
+        "unit": "Newton iters/s",
+        "ms_per_step": st["wall_time"] / acc * 1e3,
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"numpy/SciPy-SuperLU oracle, same {net.title} circuit, first {acc} accepted "
+                   f"steps ({st['total_nr_iterations']} Newton iterations) in "
+                   f"{st['wall_time']:.1f} s, one host thread"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--gates", type=int, default=65000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precision", choices=("f64", "f32"), default="f64")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
+                                   "note only (see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from vajax_amd import _lib as L
+    from vajax_amd.engine import CircuitEngine
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    t0 = time.time()
+    net = build_circuit(args.gates, args.seed)
+    eng = CircuitEngine(net)
+    eng.parse()
+    topo = eng.topology
+    log(f"setup {time.time() - t0:.1f}s: n_unknowns={topo.n_augmented} nse={topo.nse}")
+
+    use_graph = not args.no_graph
+    t0 = time.time()
+    run_gpu(eng, args.warmup, args.precision, use_graph)
+    log(f"warm-up ({args.warmup} steps incl. symbolic analysis) {time.time() - t0:.1f}s")
+    lu_info = eng.device.lu.info()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = run_gpu(eng, args.steps, args.precision, use_graph)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = res.stats
+    steps_done = st["n_steps"] - 1
+    nr = st["total_nr_iterations"]
+    ms_per_step = elapsed / max(steps_done, 1) * 1e3
+    log(f"timed: {steps_done} steps, {nr} Newton iterations, {elapsed:.3f}s")
+
+    # eager pass with phase events on the library stream
+    L.lib.vjx_profile_enable(1)
+    L.lib.vjx_profile_reset()
+    run_gpu(eng, args.steps, args.precision, False)
+    pt = L.PhaseTimes()
+    L.lib.vjx_profile_read(C.byref(pt))
+    L.lib.vjx_profile_enable(0)
+    ph = pt.as_dict()
+    iters = max(ph["n_iterations"], 1)
+    per_iter = {"build": ph["build_ms"] / iters, "lu": ph["lu_ms"] / iters,
+                "solve": ph["solve_ms"] / iters}
+    dom = max(per_iter, key=per_iter.get)
+    bytes_ = algorithmic_bytes(topo, lu_info, args.precision)
+    achieved = bytes_[dom] / (per_iter[dom] * 1e-3) / 1e9
+    kernels = {"build": "k_eval<mosfet> + k_reduce_jac + k_residual",
+               "lu": "k_lu_gather + k_lu_wide/k_lu_narrow levels + k_lu_scale",
+               "solve": "k_solve_init + k_tri_wide/k_tri_narrow levels + k_solve_finish"}
+    roofline = {
+        "bound": "hbm",
+        "kernel": kernels[dom],
+        "achieved": round(achieved, 3),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6),
+        "traffic": None,
+        "phase_ms_per_newton_iter": {k: round(v, 4) for k, v in per_iter.items()},
+        "algorithmic_bytes_per_newton_iter": bytes_,
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        cpu = cpu_baseline(net, args.cpu_steps)
+        log(f"cpu baseline {time.time() - t0:.1f}s: {cpu['ms_per_step']:.1f} ms/step")
+
+    line = {
+        "metric": METRIC,
+        "value": round(nr / elapsed, 3),
+        "unit": "Newton iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "f64" else "f32-lu+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"gate_dag_{args.gates}_seed{args.seed}",
+            "gates": args.gates,
+            "mosfets": 4 * args.gates,
+            "unknowns": topo.n_augmented,
+            "csr_nnz": topo.nse,
+            "lu_nnz": lu_info["nnz_l"] + lu_info["nnz_u"],
+            "lu_levels": lu_info["fact_levels"],
+            "integration": "gear2 adaptive",
+            "precision": args.precision,
+            "hip_graph": use_graph,
+            "accepted_steps": steps_done,
+            "newton_iterations": nr,
+            "parallelism": "single-gpu",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu_ms_per_step"] = round(cpu["ms_per_step"] / ms_per_step, 2)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

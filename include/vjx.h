@@ -253,6 +253,20 @@ typedef struct {
 
 int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx_tran_result* res);
 
+/* Phase timing of the Newton iterations of vjx_transient_run, from HIP events
+ * recorded on the library's stream (only when use_graph = 0).  Times are
+ * accumulated until the next vjx_profile_reset. */
+typedef struct {
+  double build_ms;  /* device eval + stamping + residual */
+  double lu_ms;     /* numeric refactorisation */
+  double solve_ms;  /* triangular solves (+ refinement) */
+  int64_t n_iterations;
+} vjx_phase_times;
+
+int vjx_profile_enable(int32_t enable);
+int vjx_profile_reset(void);
+int vjx_profile_read(vjx_phase_times* out);
+
 #ifdef __cplusplus
 }
 #endif

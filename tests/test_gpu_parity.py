@@ -213,6 +213,20 @@ def test_transient_gate_dag_fixed_step_fp64_and_mixed():
     assert e32 <= 1e-6, e32
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_hip_graph_replays_match_eager(precision):
+    """The captured Newton-iteration graph is replayed once per iteration with
+    host work (scalar upload, status read-back, step control) in between; every
+    replay must give exactly the eager result."""
+    net = G.gate_dag(300, seed=9)
+    _, a = run_gpu(net, 0.05e-9, 1e-12, precision=precision, use_graph=True)
+    _, b = run_gpu(net, 0.05e-9, 1e-12, precision=precision, use_graph=False)
+    assert a.stats["total_nr_iterations"] > 20
+    assert np.array_equal(a.times, b.times)
+    assert np.array_equal(a.stats["V_out"], b.stats["V_out"])
+    assert np.array_equal(a.stats["I_out"], b.stats["I_out"])
+
+
 def test_transient_gate_dag_adaptive():
     """Adaptive BDF2 with LTE rejections.
 
@@ -221,8 +235,10 @@ def test_transient_gate_dag_adaptive():
     follow the same step sequence only until the first decision that lies
     within rounding of its threshold.  Bars: the common prefix of accepted
     time points covers at least 30 % of the run with waveforms within
-    1e-9 V on it, and the whole waveforms agree to the LTE tolerance scale
-    (reltol * lte_ratio * 1 V = 3.5e-3 V) after interpolation."""
+    1e-7 V on it (ten times below the Newton voltage tolerance vntol = 1e-6,
+    where each run's Newton loop is free to stop), and the whole waveforms
+    agree to the LTE tolerance scale (reltol * lte_ratio * 1 V = 3.5e-3 V)
+    after interpolation."""
     net = G.gate_dag(200, seed=7)
     _, ref = run_oracle(net, 0.2e-9, 1e-12)
     _, res = run_gpu(net, 0.2e-9, 1e-12)
@@ -231,7 +247,7 @@ def test_transient_gate_dag_adaptive():
     k = n if same.all() else int(np.argmin(same))
     assert k >= 0.3 * len(ref.times), (k, len(ref.times), len(res.times))
     err = float(np.max(np.abs(res.stats["V_out"][:k] - ref.V[:k])))
-    assert err <= 1e-9, err
+    assert err <= 1e-7, err
     e_all = waveform_error(ref.times, ref.V, res.times, res.stats["V_out"])
     assert e_all <= 3.5e-3, e_all
     assert res.times[-1] == pytest.approx(0.2e-9, rel=1e-12)

@@ -94,6 +94,14 @@ class TranResult(C.Structure):
     ]
 
 
+class PhaseTimes(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("lu_ms", C.c_double), ("solve_ms", C.c_double),
+                ("n_iterations", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _PROTOS = {
     "vjx_version": (C.c_int, []),
     "vjx_last_error": (C.c_char_p, []),
@@ -121,6 +129,9 @@ _PROTOS = {
     "vjx_model_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp]),
     "vjx_eval_sources": (C.c_int, [vp, C.c_double, vp, vp]),
     "vjx_transient_run": (C.c_int, [vp, C.POINTER(TranConfig), C.POINTER(TranResult)]),
+    "vjx_profile_enable": (C.c_int, [C.c_int32]),
+    "vjx_profile_reset": (C.c_int, []),
+    "vjx_profile_read": (C.c_int, [C.POINTER(PhaseTimes)]),
 }
 
 EXPORTED = tuple(_PROTOS)
@@ -131,6 +142,15 @@ def _load():
         raise ImportError(
             f"vajax_amd native library not found at {LIB_PATH}; build it with "
             "`python -m vajax_amd.build` (there is no CPU fallback)")
+    # One HIP runtime per process: torch ships its own libamdhip64 with the
+    # same soname as /opt/rocm's.  Importing torch first makes libvjx bind to
+    # that copy, so device pointers and streams from torch tensors are valid
+    # here; loading libvjx first would pull in a second runtime, which then
+    # reports no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _PROTOS.items():
         fn = getattr(lib, name)  # AttributeError if the symbol is missing

@@ -226,7 +226,7 @@ int vjx_spmv(vjx_pattern* pat, const double* data_dev, const double* x_dev, doub
   int rc = pattern_upload_csr(pat);
   if (rc < 0) return rc;
   launch_spmv(pat->n, pat->d_indptr.p, pat->d_indices.p, data_dev, x_dev, nullptr, 1.0, y_dev,
-              pat->stream);
+              pat->hub_rows.view(), pat->stream);
   VJX_TRY_HIP(hipGetLastError());
   VJX_TRY_HIP(hipStreamSynchronize(pat->stream));
   return VJX_OK;

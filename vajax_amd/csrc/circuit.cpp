@@ -1,6 +1,7 @@
 // Circuit handle: device copies of the topology, and the system build
 // (device eval + stamping + residual) behind vjx_build_system.
 #include <cstring>
+#include <vector>
 
 #include "circuit.h"
 
@@ -20,7 +21,7 @@ void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, co
   launch_isrc_contrib(c->k, isrc, c->is_dst.p, c->rf.p, c->rq.p, s);
   if (csr)
     launch_reduce_jac(c->nse, c->jac_seg.p, c->jac_contrib.p, c->slot_diag.p, c->slot_b.p,
-                      c->d_sc.p, csr, s);
+                      c->d_sc.p, csr, c->hub_slots.view(), s);
   ResidualArgs a;
   a.n_unk = c->n_unk;
   a.m = c->m;
@@ -45,6 +46,7 @@ void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, co
   a.mrc = mrc;
   a.I_vs = I_vs;
   a.st = st;
+  a.hub = c->hub_rows.view();
   launch_residual(a, s);
 }
 
@@ -85,6 +87,8 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
   VJX_TRY_HIP(c->slot_b.upload(d->slot_b, c->nse));
   VJX_TRY_HIP(c->jac_seg.upload(d->jac_seg, c->nse + 1));
   VJX_TRY_HIP(c->res_seg.upload(d->res_seg, nu + 1));
+  VJX_TRY_HIP(c->hub_slots.build(d->jac_seg, c->nse, 1));
+  VJX_TRY_HIP(c->hub_rows.build(d->res_seg, nu, 3));
   VJX_TRY_HIP(c->br_ptr.upload(d->br_ptr, nu + 1));
   VJX_TRY_HIP(c->br_ent.upload(d->br_ent, d->br_ptr[nu]));
   VJX_TRY_HIP(c->vs_p.upload(d->vs_node_p, m));

@@ -15,6 +15,7 @@ struct vjx_pattern {
   vjx::DevLU lu;
   bool uploaded = false;
   vjx::DBuf<int32_t> d_indptr, d_indices;
+  vjx::HubPlan hub_rows;  // rows of A with more than kHeavySegment entries
   hipStream_t stream = nullptr;
   vjx::DBuf<vjx::DevStatus> d_status;
   vjx::DevStatus* h_status = nullptr;  // pinned
@@ -30,6 +31,7 @@ struct vjx_circuit {
   std::vector<std::unique_ptr<vjx::DevModel>> models;
   vjx::DBuf<int32_t> slot_diag, jac_seg, res_seg, br_ptr, br_ent, vs_p, vs_n, vs_wave, is_wave,
       is_dst, conv_mask;
+  vjx::HubPlan hub_slots, hub_rows;  // long stamping / residual segments
   vjx::DBuf<double> slot_b, vs_par, is_par;
   std::vector<int32_t> h_vs_wave, h_is_wave;
   std::vector<double> h_vs_par, h_is_par;
@@ -56,7 +58,8 @@ int pattern_solve_async(vjx_pattern* pat, const double* data_dev, const double* 
                         double* x, int refine_steps);
 
 void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const double* data,
-                 const double* x, const double* base, double sgn, double* out, hipStream_t s);
+                 const double* x, const double* base, double sgn, double* out,
+                 const HubChunks& hub, hipStream_t s);
 void launch_status_reset(vjx::DevStatus* st, hipStream_t s);
 
 // Enqueue one system build: eval, stamping reductions, residual.

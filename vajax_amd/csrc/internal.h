@@ -84,8 +84,12 @@ struct DevModel {
 // Level program on the device: item ids per level plus host-side launch plan.
 struct DevSchedule {
   DBuf<int32_t> items;
-  DBuf<int32_t> lvl_dev;         // device copy of lvl_ptr
-  std::vector<int32_t> lvl_ptr;  // host copy
+  DBuf<int32_t> lvl_dev;           // device copy of lvl_ptr
+  DBuf<int32_t> heavy_dev;         // device copy of lvl_heavy
+  DBuf<int32_t> mid_dev;           // device copy of lvl_mid
+  std::vector<int32_t> lvl_ptr;    // host copies
+  std::vector<int32_t> lvl_heavy;
+  std::vector<int32_t> lvl_mid;
   std::vector<LaunchGroup> groups;
 };
 
@@ -103,6 +107,45 @@ struct DevLU {
   int wide_block = 256;
 };
 
+// Segments longer than kHeavySegment ("hubs": the ground slot, supply rails)
+// are summed in kHubChunk-element chunks by one workgroup each; the consuming
+// kernel then adds a hub's chunk partials in chunk order, so the result is
+// fixed by the plan and not by scheduling.
+constexpr int kHubChunk = 4096;
+
+struct HubChunks {
+  const int32_t* item = nullptr;  // hub item (slot or row) ids
+  const int32_t* ptr = nullptr;   // hub h owns chunks [ptr[h], ptr[h + 1])
+  const int32_t* cs = nullptr;    // chunk c covers [cs[c], ce[c])
+  const int32_t* ce = nullptr;
+  double* part = nullptr;  // per-chunk partials (sum)
+  double* part2 = nullptr;  // second sum (residual rows: charge)
+  double* part3 = nullptr;  // max |.| (residual rows)
+  int n_items = 0;
+  int n_chunks = 0;
+};
+
+struct HubPlan {
+  DBuf<int32_t> item, ptr, cs, ce;
+  DBuf<double> part, part2, part3;
+  int n_items = 0, n_chunks = 0;
+  // seg: host segment pointers of n segments; nparts: 1 or 3 partial arrays.
+  hipError_t build(const int32_t* seg, int n, int nparts);
+  HubChunks view() const {
+    HubChunks h;
+    h.item = item.p;
+    h.ptr = ptr.p;
+    h.cs = cs.p;
+    h.ce = ce.p;
+    h.part = part.p;
+    h.part2 = part2.p;
+    h.part3 = part3.p;
+    h.n_items = n_items;
+    h.n_chunks = n_chunks;
+    return h;
+  }
+};
+
 // ----- kernel launchers (kernels.hip) -----
 struct EvalOut {
   double* jac;  // contribution arrays (sorted slots), may be null
@@ -115,7 +158,8 @@ void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalO
 void launch_isrc_contrib(int k, const double* isrc, const int32_t* is_dst, double* rf, double* rq,
                          hipStream_t s);
 void launch_reduce_jac(int nse, const int32_t* seg, const double* contrib, const int32_t* slot_diag,
-                       const double* slot_b, const DevScalars* sc, double* csr, hipStream_t s);
+                       const double* slot_b, const DevScalars* sc, double* csr,
+                       const HubChunks& hub, hipStream_t s);
 struct ResidualArgs {
   int n_unk, m, n_total;
   const int32_t* res_seg;
@@ -138,6 +182,7 @@ struct ResidualArgs {
   double* mrc;
   double* I_vs;
   DevStatus* st;  // may be null
+  HubChunks hub;  // node rows with long residual segments
 };
 void launch_residual(const ResidualArgs& a, hipStream_t s);
 

@@ -15,6 +15,7 @@
 //
 // Compiled with -ffp-contract=off so every FP64 expression rounds exactly as
 // written (and as the numpy oracle evaluates it).
+#include "block_flags.h"
 #include "circuit.h"
 #include "internal.h"
 
@@ -280,81 +281,217 @@ __global__ void k_isrc(int k, const double* __restrict__ isrc, const int32_t* __
   }
 }
 
+// ---- fixed-order workgroup reductions ---------------------------------------
+//
+// Long segments (a supply rail's CSR slot or KCL row, a dense LU row) are
+// summed by one workgroup: kReduceLanes strided partial sums, then a halving
+// tree in LDS.  The order is fixed, so results are deterministic; the CPU
+// reference program emulates the same order.
+
+constexpr int RB = kReduceLanes;
+
+// All threads of the block call; threads >= RB contribute nothing.
+template <typename T, typename F>
+__device__ __forceinline__ T block_sum(int start, int end, F term, T* red) {
+  const int t = threadIdx.x;
+  if (t < RB) {
+    T acc = (T)0;
+    for (int u = start + t; u < end; u += RB) acc += term(u);
+    red[t] = acc;
+  }
+  __syncthreads();
+  for (int w = RB / 2; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  const T total = red[0];
+  __syncthreads();
+  return total;
+}
+
+// G consecutive lanes (G divides the wavefront): strided partials, then the
+// halving tree through shuffles; lane 0 of the group returns the sum.  All G
+// lanes must call.
+template <int G, typename T, typename F>
+__device__ __forceinline__ T group_sum(int start, int end, int lane, F term) {
+  T acc = (T)0;
+  for (int u = start + lane; u < end; u += G) acc += term(u);
+#pragma unroll
+  for (int w = G / 2; w > 0; w >>= 1) acc += __shfl_down(acc, w, G);
+  return acc;
+}
+
+template <typename F>
+__device__ __forceinline__ double block_max(int start, int end, F term, double* red) {
+  const int t = threadIdx.x;
+  if (t < RB) {
+    double m = 0.0;
+    for (int u = start + t; u < end; u += RB) m = fmax(m, term(u));
+    red[t] = m;
+  }
+  __syncthreads();
+  for (int w = RB / 2; w > 0; w >>= 1) {
+    if (t < w) red[t] = fmax(red[t], red[t + w]);
+    __syncthreads();
+  }
+  const double total = red[0];
+  __syncthreads();
+  return total;
+}
+
+// ---- stamping reductions ----------------------------------------------------
+
+__device__ __forceinline__ double finish_slot(double acc, int s, const int32_t* slot_diag,
+                                              const double* slot_b, const DevScalars* sc) {
+  if (slot_diag[s]) acc += sc->gmin + sc->gshunt;
+  const double b = slot_b[s];
+  if (b != 0.0) acc += b;
+  return acc;
+}
+
+// Chunk pre-pass for hub segments: chunk c sums [start[c], end[c]) with one
+// workgroup; the reducing kernels then add a segment's chunk partials in order.
+__global__ void __launch_bounds__(256) k_chunk_sum(const int32_t* __restrict__ cs,
+                                                   const int32_t* __restrict__ ce,
+                                                   const double* __restrict__ v, double* part) {
+  __shared__ double red[RB];
+  const int c = blockIdx.x;
+  const double s = block_sum<double>(cs[c], ce[c], [&](int u) { return v[u]; }, red);
+  if (threadIdx.x == 0) part[c] = s;
+}
+
+// residual rows: sums of rf and rq and max |rf| per chunk
+__global__ void __launch_bounds__(256) k_chunk_res(const int32_t* __restrict__ cs,
+                                                   const int32_t* __restrict__ ce,
+                                                   const double* __restrict__ rf,
+                                                   const double* __restrict__ rq, double* pf,
+                                                   double* pq, double* pm) {
+  __shared__ double red[RB];
+  const int c = blockIdx.x;
+  const double f = block_sum<double>(cs[c], ce[c], [&](int u) { return rf[u]; }, red);
+  const double q = block_sum<double>(cs[c], ce[c], [&](int u) { return rq[u]; }, red);
+  const double m = block_max(cs[c], ce[c], [&](int u) { return fabs(rf[u]); }, red);
+  if (threadIdx.x == 0) {
+    pf[c] = f;
+    pq[c] = q;
+    pm[c] = m;
+  }
+}
+
+// SpMV rows: sums of a[e] * x[ci[e]] per chunk
+__global__ void __launch_bounds__(256) k_chunk_spmv(const int32_t* __restrict__ cs,
+                                                    const int32_t* __restrict__ ce,
+                                                    const double* __restrict__ a,
+                                                    const int32_t* __restrict__ ci,
+                                                    const double* __restrict__ x, double* part) {
+  __shared__ double red[RB];
+  const int c = blockIdx.x;
+  const double s = block_sum<double>(cs[c], ce[c], [&](int e) { return a[e] * x[ci[e]]; }, red);
+  if (threadIdx.x == 0) part[c] = s;
+}
+
+// Blocks [0, n_heavy) finish one hub slot each from its chunk partials; the
+// other blocks reduce one slot per lane.
 __global__ void __launch_bounds__(256) k_reduce_jac(int nse, const int32_t* __restrict__ seg,
                                                     const double* __restrict__ contrib,
                                                     const int32_t* __restrict__ slot_diag,
                                                     const double* __restrict__ slot_b,
                                                     const DevScalars* __restrict__ sc,
-                                                    double* __restrict__ csr) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nse) return;
-  double acc = 0.0;
-  for (int u = seg[s], e = seg[s + 1]; u < e; ++u) acc += contrib[u];
-  if (slot_diag[s]) acc += sc->gmin + sc->gshunt;
-  const double b = slot_b[s];
-  if (b != 0.0) acc += b;
-  csr[s] = acc;
-}
-
-__device__ __forceinline__ void block_max_u64(unsigned long long v, unsigned long long* out) {
-  for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long w = __shfl_xor(v, o);
-    v = w > v ? w : v;
+                                                    double* __restrict__ csr, HubChunks hub) {
+  if ((int)blockIdx.x < hub.n_items) {
+    if (threadIdx.x == 0) {
+      const int s = hub.item[blockIdx.x];
+      double tot = 0.0;
+      for (int c = hub.ptr[blockIdx.x]; c < hub.ptr[blockIdx.x + 1]; ++c) tot += hub.part[c];
+      csr[s] = finish_slot(tot, s, slot_diag, slot_b, sc);
+    }
+    return;
   }
-  if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
+  const int n_heavy = hub.n_items;
+  const int s = (blockIdx.x - n_heavy) * blockDim.x + threadIdx.x;
+  if (s >= nse) return;
+  const int u0 = seg[s], u1 = seg[s + 1];
+  if (u1 - u0 > kHeavySegment) return;
+  double acc = 0.0;
+  for (int u = u0; u < u1; ++u) acc += contrib[u];
+  csr[s] = finish_slot(acc, s, slot_diag, slot_b, sc);
 }
 
-__device__ __forceinline__ void block_count(bool flag, unsigned int* out) {
-  unsigned long long m = __ballot(flag);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (unsigned int)__popcll(m));
+// Residual of one node row from its reduced sums (mna_builder.py:1012-1076);
+// returns the value entering the convergence test and whether it fails.
+__device__ __forceinline__ bool finish_row(const ResidualArgs& a, int i, double f, double q,
+                                           double mx, double& fchk) {
+  const DevScalars* sc = a.sc;
+  double fr = f - 0.0;
+  if (a.m > 0) {
+    double br = 0.0;
+    for (int e = a.br_ptr[i]; e < a.br_ptr[i + 1]; ++e) {
+      const int k = a.br_ent[e];
+      if (k >= 0) {
+        a.I_vs[k] = -fr;
+        br += a.X[a.n_total + k];
+      } else {
+        br += -a.X[a.n_total + ~k];
+      }
+    }
+    fr = fr + br;
+  }
+  const double qp = a.Q_prev ? a.Q_prev[i] : 0.0;
+  const double dq = a.dQdt_prev ? a.dQdt_prev[i] : 0.0;
+  const double q2 = a.Q_prev2 ? a.Q_prev2[i] : 0.0;
+  double t = fr - 0.0;
+  t = t + sc->c0 * (q - 0.0);
+  t = t + sc->c1 * qp;
+  t = t + sc->d1 * dq;
+  t = t + sc->c2 * q2;
+  const double eff = sc->gmin + sc->gshunt;
+  t = t + eff * a.X[i + 1];
+  a.f[i] = t;
+  if (a.Q) a.Q[i] = q;
+  if (a.mrc) a.mrc[i] = mx;
+  fchk = 0.0;
+  const bool checked = a.conv_mask == nullptr || a.conv_mask[i] != 0;
+  if (!checked) return false;
+  const double floor = a.res_floor ? a.res_floor[i] : sc->abstol;
+  const double tol = fmax(mx * sc->reltol, floor);
+  fchk = fabs(t);
+  return !(fchk < tol);
 }
 
 __global__ void __launch_bounds__(256) k_residual(ResidualArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const DevScalars* sc = a.sc;
+  const HubChunks& hub = a.hub;
+  if ((int)blockIdx.x < hub.n_items) {
+    if (threadIdx.x == 0) {
+      const int i = hub.item[blockIdx.x];
+      double f = 0.0, q = 0.0, mx = 0.0;
+      for (int c = hub.ptr[blockIdx.x]; c < hub.ptr[blockIdx.x + 1]; ++c) {
+        f += hub.part[c];
+        q += hub.part2[c];
+        mx = fmax(mx, hub.part3[c]);
+      }
+      double fchk;
+      const bool fail = finish_row(a, i, f, q, mx, fchk);
+      if (a.st) {
+        if (fchk > 0.0) atomicMax(&a.st->maxf_bits, dbits(fchk));
+        if (fail) atomicAdd(&a.st->res_fail, 1u);
+      }
+    }
+    return;
+  }
+  const int i = (blockIdx.x - hub.n_items) * blockDim.x + threadIdx.x;
   bool fail = false;
   double fchk = 0.0;
   if (i < a.n_unk) {
-    double f = 0.0, q = 0.0, mx = 0.0;
-    for (int u = a.res_seg[i], e = a.res_seg[i + 1]; u < e; ++u) {
-      const double v = a.rf[u];
-      f += v;
-      q += a.rq[u];
-      mx = fmax(mx, fabs(v));
-    }
-    double fr = f - 0.0;
-    if (a.m > 0) {
-      double br = 0.0;
-      for (int e = a.br_ptr[i]; e < a.br_ptr[i + 1]; ++e) {
-        const int k = a.br_ent[e];
-        if (k >= 0) {
-          a.I_vs[k] = -fr;
-          br += a.X[a.n_total + k];
-        } else {
-          br += -a.X[a.n_total + ~k];
-        }
+    const int u0 = a.res_seg[i], u1 = a.res_seg[i + 1];
+    if (u1 - u0 <= kHeavySegment) {
+      double f = 0.0, q = 0.0, mx = 0.0;
+      for (int u = u0; u < u1; ++u) {
+        const double v = a.rf[u];
+        f += v;
+        q += a.rq[u];
+        mx = fmax(mx, fabs(v));
       }
-      fr = fr + br;
-    }
-    const double qp = a.Q_prev ? a.Q_prev[i] : 0.0;
-    const double dq = a.dQdt_prev ? a.dQdt_prev[i] : 0.0;
-    const double q2 = a.Q_prev2 ? a.Q_prev2[i] : 0.0;
-    double t = fr - 0.0;
-    t = t + sc->c0 * (q - 0.0);
-    t = t + sc->c1 * qp;
-    t = t + sc->d1 * dq;
-    t = t + sc->c2 * q2;
-    const double eff = sc->gmin + sc->gshunt;
-    t = t + eff * a.X[i + 1];
-    a.f[i] = t;
-    if (a.Q) a.Q[i] = q;
-    if (a.mrc) a.mrc[i] = mx;
-    const bool checked = a.conv_mask == nullptr || a.conv_mask[i] != 0;
-    if (checked) {
-      const double floor = a.res_floor ? a.res_floor[i] : sc->abstol;
-      const double tol = fmax(mx * sc->reltol, floor);
-      fchk = fabs(t);
-      fail = !(fchk < tol);
+      fail = finish_row(a, i, f, q, mx, fchk);
     }
   } else if (i < a.n_unk + a.m) {
     const int k = i - a.n_unk;
@@ -363,12 +500,9 @@ __global__ void __launch_bounds__(256) k_residual(ResidualArgs a) {
     a.f[i] = t;
     if (p == 0) a.I_vs[k] = 0.0;
     fchk = fabs(t);
-    fail = !(fchk < sc->vntol);
+    fail = !(fchk < a.sc->vntol);
   }
-  if (a.st) {
-    block_max_u64(dbits(fchk), &a.st->maxf_bits);
-    block_count(fail, &a.st->res_fail);
-  }
+  if (a.st) block_flags(dbits(fchk), 0, fail ? 1u : 0u, &a.st->maxf_bits, nullptr, &a.st->res_fail);
 }
 
 // ---- LU refactor ------------------------------------------------------------
@@ -382,41 +516,91 @@ __global__ void __launch_bounds__(256) k_lu_gather(int nnz, const int32_t* __res
   lu[e] = s >= 0 ? (T)csr[s] : (T)0;
 }
 
+struct LuProg {
+  const int32_t* tpos;
+  const int32_t* tseg;
+  const int32_t* ul;
+  const int32_t* uu;
+  const int32_t* ud;
+};
+
 template <typename T>
-__device__ __forceinline__ void lu_item(int t, const int32_t* __restrict__ tpos,
-                                        const int32_t* __restrict__ tseg,
-                                        const int32_t* __restrict__ ul,
-                                        const int32_t* __restrict__ uu,
-                                        const int32_t* __restrict__ ud, T* lu) {
+struct LuTerm {
+  const LuProg& P;
+  const T* lu;
+  __device__ __forceinline__ T operator()(int u) const {
+    return lu[P.ul[u]] * lu[P.uu[u]] / lu[P.ud[u]];
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ void lu_light(int t, const LuProg& P, T* lu) {
   T acc = (T)0;
-  for (int u = tseg[t], e = tseg[t + 1]; u < e; ++u) acc += lu[ul[u]] * lu[uu[u]] / lu[ud[u]];
-  lu[tpos[t]] -= acc;
+  const LuTerm<T> term{P, lu};
+  for (int u = P.tseg[t], e = P.tseg[t + 1]; u < e; ++u) acc += term(u);
+  lu[P.tpos[t]] -= acc;
+}
+
+// all kGroupLanes lanes of a group call
+template <typename T>
+__device__ __forceinline__ void lu_mid(int t, int lane, const LuProg& P, T* lu) {
+  const T acc = group_sum<kGroupLanes, T>(P.tseg[t], P.tseg[t + 1], lane, LuTerm<T>{P, lu});
+  if (lane == 0) lu[P.tpos[t]] -= acc;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_lu_wide(int lo, int hi, const int32_t* __restrict__ items,
-                                                 const int32_t* __restrict__ tpos,
-                                                 const int32_t* __restrict__ tseg,
-                                                 const int32_t* __restrict__ ul,
-                                                 const int32_t* __restrict__ uu,
-                                                 const int32_t* __restrict__ ud, T* lu) {
-  const int g = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= hi) return;
-  lu_item<T>(items[g], tpos, tseg, ul, uu, ud, lu);
+__device__ __forceinline__ void lu_heavy(int t, const LuProg& P, T* lu, T* red) {
+  const T acc = block_sum<T>(P.tseg[t], P.tseg[t + 1], LuTerm<T>{P, lu}, red);
+  if (threadIdx.x == 0) lu[P.tpos[t]] -= acc;
 }
 
+constexpr int kGroupsPerBlock = 256 / kGroupLanes;
+
+// One wide level: items are target ids in level order (identity): nh heavy
+// (a workgroup each), nm medium (kGroupLanes lanes each), then short ones.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_lu_narrow(int l0, int l1, const int32_t* __restrict__ lvl,
-                                                    const int32_t* __restrict__ items,
-                                                    const int32_t* __restrict__ tpos,
-                                                    const int32_t* __restrict__ tseg,
-                                                    const int32_t* __restrict__ ul,
-                                                    const int32_t* __restrict__ uu,
-                                                    const int32_t* __restrict__ ud, T* lu) {
-  for (int l = l0; l < l1; ++l) {
-    const int lo = lvl[l], hi = lvl[l + 1];
-    for (int g = lo + (int)threadIdx.x; g < hi; g += blockDim.x)
-      lu_item<T>(items[g], tpos, tseg, ul, uu, ud, lu);
+__global__ void __launch_bounds__(256) k_lu_wide(int lo, int hi, int nh, int nm, LuProg P, T* lu) {
+  __shared__ T red[RB];
+  const int b = blockIdx.x;
+  const int nmb = (nm + kGroupsPerBlock - 1) / kGroupsPerBlock;
+  if (b < nh) {
+    lu_heavy<T>(lo + b, P, lu, red);
+    return;
+  }
+  if (b < nh + nmb) {
+    const int grp = (b - nh) * kGroupsPerBlock + (int)threadIdx.x / kGroupLanes;
+    if (grp < nm) lu_mid<T>(lo + nh + grp, (int)threadIdx.x % kGroupLanes, P, lu);
+    return;
+  }
+  const int g = lo + nh + nm + (b - nh - nmb) * blockDim.x + threadIdx.x;
+  if (g < hi) lu_light<T>(g, P, lu);
+}
+
+// A run of narrow levels in one workgroup; level bounds cached in LDS.
+template <typename T>
+__global__ void __launch_bounds__(512) k_lu_narrow(int l0, int l1, const int32_t* __restrict__ lvl,
+                                                   const int32_t* __restrict__ lheavy,
+                                                   const int32_t* __restrict__ lmid, LuProg P,
+                                                   T* lu) {
+  __shared__ T red[RB];
+  extern __shared__ int32_t s_lv[];
+  const int nl = l1 - l0;
+  int32_t* s_hv = s_lv + nl + 1;
+  int32_t* s_md = s_hv + nl;
+  for (int j = threadIdx.x; j <= nl; j += blockDim.x) s_lv[j] = lvl[l0 + j];
+  for (int j = threadIdx.x; j < nl; j += blockDim.x) {
+    s_hv[j] = lheavy[l0 + j];
+    s_md[j] = lmid[l0 + j];
+  }
+  __syncthreads();
+  const int lane = (int)threadIdx.x % kGroupLanes;
+  const int ngrp = (int)blockDim.x / kGroupLanes;
+  for (int l = 0; l < nl; ++l) {
+    const int lo = s_lv[l], hi = s_lv[l + 1], nh = s_hv[l], nm = s_md[l];
+    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x) lu_light<T>(g, P, lu);
+    for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp)
+      lu_mid<T>(lo + nh + grp, lane, P, lu);
+    for (int h = 0; h < nh; ++h) lu_heavy<T>(lo + h, P, lu, red);
     __syncthreads();
   }
 }
@@ -434,10 +618,8 @@ __global__ void __launch_bounds__(256) k_lu_scale(int n, const int32_t* __restri
     mag = fabs((double)v);
     bad = !(mag <= 1.7976931348623157e308);
   }
-  if (st) {
-    block_max_u64(dbits(bad ? 0.0 : mag), &st->maxl_bits);
-    block_count(bad, &st->nonfinite);
-  }
+  if (st) block_flags(dbits(bad ? 0.0 : mag), 0, bad ? 1u : 0u, &st->maxl_bits, nullptr,
+                        &st->nonfinite);
 }
 
 // ---- triangular solves ------------------------------------------------------
@@ -454,56 +636,106 @@ __global__ void __launch_bounds__(256) k_solve_init(int n, const int32_t* __rest
     y[i] = (T)rhs[p[i]];
 }
 
-template <typename T>
-__device__ __forceinline__ void fwd_row(int i, const int32_t* __restrict__ rp,
-                                        const int32_t* __restrict__ col,
-                                        const int32_t* __restrict__ dg, const T* __restrict__ lu,
-                                        T* y) {
+struct TriProg {
+  const int32_t* items;
+  const int32_t* rp;
+  const int32_t* col;
+  const int32_t* dg;
+};
+
+template <typename T, bool FWD>
+__device__ __forceinline__ void tri_light(int i, const TriProg& P, const T* __restrict__ lu, T* y,
+                                          T* z) {
   T acc = (T)0;
-  for (int e = rp[i], d = dg[i]; e < d; ++e) acc += lu[e] * y[col[e]];
-  y[i] = y[i] - acc;
+  if (FWD) {
+    for (int e = P.rp[i], d = P.dg[i]; e < d; ++e) acc += lu[e] * y[P.col[e]];
+    y[i] = y[i] - acc;
+  } else {
+    const int d = P.dg[i];
+    for (int e = d + 1, end = P.rp[i + 1]; e < end; ++e) acc += lu[e] * z[P.col[e]];
+    z[i] = (y[i] - acc) / lu[d];
+  }
 }
 
 template <typename T>
-__device__ __forceinline__ void bwd_row(int i, const int32_t* __restrict__ rp,
-                                        const int32_t* __restrict__ col,
-                                        const int32_t* __restrict__ dg, const T* __restrict__ lu,
-                                        const T* __restrict__ y, T* z) {
-  T acc = (T)0;
-  const int d = dg[i];
-  for (int e = d + 1, end = rp[i + 1]; e < end; ++e) acc += lu[e] * z[col[e]];
-  z[i] = (y[i] - acc) / lu[d];
+struct TriTerm {
+  const T* lu;
+  const T* v;
+  const int32_t* col;
+  __device__ __forceinline__ T operator()(int e) const { return lu[e] * v[col[e]]; }
+};
+
+template <typename T, bool FWD>
+__device__ __forceinline__ void tri_mid(int i, int lane, const TriProg& P,
+                                        const T* __restrict__ lu, T* y, T* z) {
+  if (FWD) {
+    const T acc = group_sum<kGroupLanes, T>(P.rp[i], P.dg[i], lane, TriTerm<T>{lu, y, P.col});
+    if (lane == 0) y[i] = y[i] - acc;
+  } else {
+    const int d = P.dg[i];
+    const T acc = group_sum<kGroupLanes, T>(d + 1, P.rp[i + 1], lane, TriTerm<T>{lu, z, P.col});
+    if (lane == 0) z[i] = (y[i] - acc) / lu[d];
+  }
 }
 
 template <typename T, bool FWD>
-__global__ void __launch_bounds__(256) k_tri_wide(int lo, int hi, const int32_t* __restrict__ items,
-                                                  const int32_t* __restrict__ rp,
-                                                  const int32_t* __restrict__ col,
-                                                  const int32_t* __restrict__ dg,
+__device__ __forceinline__ void tri_heavy(int i, const TriProg& P, const T* __restrict__ lu, T* y,
+                                          T* z, T* red) {
+  if (FWD) {
+    const T acc = block_sum<T>(P.rp[i], P.dg[i], TriTerm<T>{lu, y, P.col}, red);
+    if (threadIdx.x == 0) y[i] = y[i] - acc;
+  } else {
+    const int d = P.dg[i];
+    const T acc = block_sum<T>(d + 1, P.rp[i + 1], TriTerm<T>{lu, z, P.col}, red);
+    if (threadIdx.x == 0) z[i] = (y[i] - acc) / lu[d];
+  }
+}
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(256) k_tri_wide(int lo, int hi, int nh, int nm, TriProg P,
                                                   const T* __restrict__ lu, T* y, T* z) {
-  const int g = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= hi) return;
-  if (FWD)
-    fwd_row<T>(items[g], rp, col, dg, lu, y);
-  else
-    bwd_row<T>(items[g], rp, col, dg, lu, y, z);
+  __shared__ T red[RB];
+  const int b = blockIdx.x;
+  const int nmb = (nm + kGroupsPerBlock - 1) / kGroupsPerBlock;
+  if (b < nh) {
+    tri_heavy<T, FWD>(P.items[lo + b], P, lu, y, z, red);
+    return;
+  }
+  if (b < nh + nmb) {
+    const int grp = (b - nh) * kGroupsPerBlock + (int)threadIdx.x / kGroupLanes;
+    if (grp < nm)
+      tri_mid<T, FWD>(P.items[lo + nh + grp], (int)threadIdx.x % kGroupLanes, P, lu, y, z);
+    return;
+  }
+  const int g = lo + nh + nm + (b - nh - nmb) * blockDim.x + threadIdx.x;
+  if (g < hi) tri_light<T, FWD>(P.items[g], P, lu, y, z);
 }
 
 template <typename T, bool FWD>
-__global__ void __launch_bounds__(1024) k_tri_narrow(int l0, int l1, const int32_t* __restrict__ lvl,
-                                                     const int32_t* __restrict__ items,
-                                                     const int32_t* __restrict__ rp,
-                                                     const int32_t* __restrict__ col,
-                                                     const int32_t* __restrict__ dg,
-                                                     const T* __restrict__ lu, T* y, T* z) {
-  for (int l = l0; l < l1; ++l) {
-    const int lo = lvl[l], hi = lvl[l + 1];
-    for (int g = lo + (int)threadIdx.x; g < hi; g += blockDim.x) {
-      if (FWD)
-        fwd_row<T>(items[g], rp, col, dg, lu, y);
-      else
-        bwd_row<T>(items[g], rp, col, dg, lu, y, z);
-    }
+__global__ void __launch_bounds__(512) k_tri_narrow(int l0, int l1, const int32_t* __restrict__ lvl,
+                                                    const int32_t* __restrict__ lheavy,
+                                                    const int32_t* __restrict__ lmid, TriProg P,
+                                                    const T* __restrict__ lu, T* y, T* z) {
+  __shared__ T red[RB];
+  extern __shared__ int32_t s_lv[];
+  const int nl = l1 - l0;
+  int32_t* s_hv = s_lv + nl + 1;
+  int32_t* s_md = s_hv + nl;
+  for (int j = threadIdx.x; j <= nl; j += blockDim.x) s_lv[j] = lvl[l0 + j];
+  for (int j = threadIdx.x; j < nl; j += blockDim.x) {
+    s_hv[j] = lheavy[l0 + j];
+    s_md[j] = lmid[l0 + j];
+  }
+  __syncthreads();
+  const int lane = (int)threadIdx.x % kGroupLanes;
+  const int ngrp = (int)blockDim.x / kGroupLanes;
+  for (int l = 0; l < nl; ++l) {
+    const int lo = s_lv[l], hi = s_lv[l + 1], nh = s_hv[l], nm = s_md[l];
+    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x)
+      tri_light<T, FWD>(P.items[g], P, lu, y, z);
+    for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp)
+      tri_mid<T, FWD>(P.items[lo + nh + grp], lane, P, lu, y, z);
+    for (int h = 0; h < nh; ++h) tri_heavy<T, FWD>(P.items[lo + h], P, lu, y, z, red);
     __syncthreads();
   }
 }
@@ -516,32 +748,31 @@ __global__ void __launch_bounds__(256) k_solve_finish(int n, const int32_t* __re
   x[q[i]] = (double)z[i];
 }
 
-// out = base + sgn * (A x)   (base may be null)
+// out = base + sgn * (A x)   (base may be null); long rows by a workgroup
 __global__ void __launch_bounds__(256) k_spmv(int n, const int32_t* __restrict__ rp,
                                               const int32_t* __restrict__ ci,
                                               const double* __restrict__ a,
                                               const double* __restrict__ x,
                                               const double* __restrict__ base, double sgn,
-                                              double* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                              double* out, HubChunks hub) {
+  if ((int)blockIdx.x < hub.n_items) {
+    if (threadIdx.x == 0) {
+      const int i = hub.item[blockIdx.x];
+      double acc = 0.0;
+      for (int c = hub.ptr[blockIdx.x]; c < hub.ptr[blockIdx.x + 1]; ++c) acc += hub.part[c];
+      if (sgn != 1.0) acc = sgn * acc;
+      out[i] = base ? base[i] + acc : acc;
+    }
+    return;
+  }
+  const int i = (blockIdx.x - hub.n_items) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int e0 = rp[i], e1 = rp[i + 1];
+  if (e1 - e0 > kHeavySegment) return;
   double acc = 0.0;
-  for (int e = rp[i], end = rp[i + 1]; e < end; ++e) acc += a[e] * x[ci[e]];
+  for (int e = e0; e < e1; ++e) acc += a[e] * x[ci[e]];
   if (sgn != 1.0) acc = sgn * acc;
   out[i] = base ? base[i] + acc : acc;
-}
-
-__global__ void k_status_reset(DevStatus* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->maxf_bits = 0;
-    st->maxdv_bits = 0;
-    st->maxdelta_bits = 0;
-    st->maxl_bits = 0;
-    st->res_fail = 0;
-    st->delta_fail = 0;
-    st->nonfinite = 0;
-    st->pad = 0;
-  }
 }
 
 __global__ void k_axpy(int n, double* x, const double* d) {
@@ -570,6 +801,19 @@ __global__ void k_sources(int m, const int32_t* __restrict__ wave, const double*
                           double t, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) out[i] = wave_value(wave[i], par, i, m, t);
+}
+
+__global__ void k_status_reset(DevStatus* st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->maxf_bits = 0;
+    st->maxdv_bits = 0;
+    st->maxdelta_bits = 0;
+    st->maxl_bits = 0;
+    st->res_fail = 0;
+    st->delta_fail = 0;
+    st->nonfinite = 0;
+    st->pad = 0;
+  }
 }
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
@@ -631,27 +875,37 @@ void launch_isrc_contrib(int k, const double* isrc, const int32_t* is_dst, doubl
 }
 
 void launch_reduce_jac(int nse, const int32_t* seg, const double* contrib, const int32_t* slot_diag,
-                       const double* slot_b, const DevScalars* sc, double* csr, hipStream_t s) {
-  k_reduce_jac<<<nblk(nse, 256), 256, 0, s>>>(nse, seg, contrib, slot_diag, slot_b, sc, csr);
+                       const double* slot_b, const DevScalars* sc, double* csr,
+                       const HubChunks& hub, hipStream_t s) {
+  if (nse == 0) return;
+  if (hub.n_chunks > 0) k_chunk_sum<<<hub.n_chunks, 256, 0, s>>>(hub.cs, hub.ce, contrib, hub.part);
+  k_reduce_jac<<<hub.n_items + nblk(nse, 256), 256, 0, s>>>(nse, seg, contrib, slot_diag, slot_b,
+                                                            sc, csr, hub);
 }
 
 void launch_residual(const ResidualArgs& a, hipStream_t s) {
   const int n = a.n_unk + a.m;
   if (n == 0) return;
-  k_residual<<<nblk(n, 256), 256, 0, s>>>(a);
+  const HubChunks& h = a.hub;
+  if (h.n_chunks > 0)
+    k_chunk_res<<<h.n_chunks, 256, 0, s>>>(h.cs, h.ce, a.rf, a.rq, h.part, h.part2, h.part3);
+  k_residual<<<h.n_items + nblk(n, 256), 256, 0, s>>>(a);
 }
 
 template <typename T>
-static void run_fact_groups(DevLU& L, T* lu, hipStream_t s, const int32_t* lvl_dev) {
+static void run_fact_groups(DevLU& L, T* lu, hipStream_t s) {
+  const LuProg P{L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
   for (const auto& g : L.fact.groups) {
     if (g.wide) {
       const int lo = L.fact.lvl_ptr[g.lvl_begin], hi = L.fact.lvl_ptr[g.lvl_end];
-      k_lu_wide<T><<<nblk(hi - lo, L.wide_block), L.wide_block, 0, s>>>(
-          lo, hi, L.fact.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p, lu);
+      const int nh = L.fact.lvl_heavy[g.lvl_begin], nm = L.fact.lvl_mid[g.lvl_begin];
+      const int grid = nh + nblk(nm, kGroupsPerBlock) + nblk(hi - lo - nh - nm, 256);
+      k_lu_wide<T><<<grid, 256, 0, s>>>(lo, hi, nh, nm, P, lu);
     } else {
-      k_lu_narrow<T><<<1, L.narrow_block, 0, s>>>(g.lvl_begin, g.lvl_end, lvl_dev, L.fact.items.p,
-                                                  L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p,
-                                                  L.upd_d.p, lu);
+      const int nl = g.lvl_end - g.lvl_begin;
+      const size_t shm = sizeof(int32_t) * (3 * nl + 1);
+      k_lu_narrow<T><<<1, L.narrow_block, shm, s>>>(g.lvl_begin, g.lvl_end, L.fact.lvl_dev.p,
+                                                    L.fact.heavy_dev.p, L.fact.mid_dev.p, P, lu);
     }
   }
 }
@@ -660,9 +914,27 @@ template <typename T>
 void launch_lu_refactor(DevLU& L, const double* csr, T* lu, DevStatus* st, hipStream_t s) {
   const int nnz = (int)L.col.n;
   k_lu_gather<T><<<nblk(nnz, 256), 256, 0, s>>>(nnz, L.src.p, csr, lu);
-  run_fact_groups<T>(L, lu, s, L.fact.lvl_dev.p);
+  run_fact_groups<T>(L, lu, s);
   const int nl = (int)L.l_slot.n;
   if (nl > 0) k_lu_scale<T><<<nblk(nl, 256), 256, 0, s>>>(nl, L.l_slot.p, L.l_diag.p, lu, st);
+}
+
+template <typename T, bool FWD>
+static void run_tri_groups(DevLU& L, const DevSchedule& S, const T* lu, T* y, T* z, hipStream_t s) {
+  const TriProg P{S.items.p, L.rowptr.p, L.col.p, L.diag.p};
+  for (const auto& g : S.groups) {
+    if (g.wide) {
+      const int lo = S.lvl_ptr[g.lvl_begin], hi = S.lvl_ptr[g.lvl_end];
+      const int nh = S.lvl_heavy[g.lvl_begin], nm = S.lvl_mid[g.lvl_begin];
+      const int grid = nh + nblk(nm, kGroupsPerBlock) + nblk(hi - lo - nh - nm, 256);
+      k_tri_wide<T, FWD><<<grid, 256, 0, s>>>(lo, hi, nh, nm, P, lu, y, z);
+    } else {
+      const int nl = g.lvl_end - g.lvl_begin;
+      const size_t shm = sizeof(int32_t) * (3 * nl + 1);
+      k_tri_narrow<T, FWD><<<1, L.narrow_block, shm, s>>>(g.lvl_begin, g.lvl_end, S.lvl_dev.p,
+                                                          S.heavy_dev.p, S.mid_dev.p, P, lu, y, z);
+    }
+  }
 }
 
 template <typename T>
@@ -671,28 +943,8 @@ void launch_lu_solve(DevLU& L, const T* lu, const double* rhs, double scale, dou
   const int n = L.sym.n;
   if (n == 0) return;
   k_solve_init<T><<<nblk(n, 256), 256, 0, s>>>(n, L.p.p, rhs, scale, y);
-  for (const auto& g : L.fwd.groups) {
-    if (g.wide) {
-      const int lo = L.fwd.lvl_ptr[g.lvl_begin], hi = L.fwd.lvl_ptr[g.lvl_end];
-      k_tri_wide<T, true><<<nblk(hi - lo, L.wide_block), L.wide_block, 0, s>>>(
-          lo, hi, L.fwd.items.p, L.rowptr.p, L.col.p, L.diag.p, lu, y, z);
-    } else {
-      k_tri_narrow<T, true><<<1, L.narrow_block, 0, s>>>(g.lvl_begin, g.lvl_end, L.fwd.lvl_dev.p,
-                                                         L.fwd.items.p, L.rowptr.p, L.col.p,
-                                                         L.diag.p, lu, y, z);
-    }
-  }
-  for (const auto& g : L.bwd.groups) {
-    if (g.wide) {
-      const int lo = L.bwd.lvl_ptr[g.lvl_begin], hi = L.bwd.lvl_ptr[g.lvl_end];
-      k_tri_wide<T, false><<<nblk(hi - lo, L.wide_block), L.wide_block, 0, s>>>(
-          lo, hi, L.bwd.items.p, L.rowptr.p, L.col.p, L.diag.p, lu, y, z);
-    } else {
-      k_tri_narrow<T, false><<<1, L.narrow_block, 0, s>>>(g.lvl_begin, g.lvl_end, L.bwd.lvl_dev.p,
-                                                          L.bwd.items.p, L.rowptr.p, L.col.p,
-                                                          L.diag.p, lu, y, z);
-    }
-  }
+  run_tri_groups<T, true>(L, L.fwd, lu, y, z, s);
+  run_tri_groups<T, false>(L, L.bwd, lu, y, z, s);
   k_solve_finish<T><<<nblk(n, 256), 256, 0, s>>>(n, L.q.p, z, x);
 }
 
@@ -704,9 +956,13 @@ template void launch_lu_solve<float>(DevLU&, const float*, const double*, double
                                      float*, hipStream_t);
 
 void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const double* data,
-                 const double* x, const double* base, double sgn, double* out, hipStream_t s) {
+                 const double* x, const double* base, double sgn, double* out,
+                 const HubChunks& hub, hipStream_t s) {
   if (n == 0) return;
-  k_spmv<<<nblk(n, 256), 256, 0, s>>>(n, indptr, indices, data, x, base, sgn, out);
+  if (hub.n_chunks > 0)
+    k_chunk_spmv<<<hub.n_chunks, 256, 0, s>>>(hub.cs, hub.ce, data, indices, x, hub.part);
+  k_spmv<<<hub.n_items + nblk(n, 256), 256, 0, s>>>(n, indptr, indices, data, x, base, sgn, out,
+                                                    hub);
 }
 
 void launch_status_reset(DevStatus* st, hipStream_t s) { k_status_reset<<<1, 64, 0, s>>>(st); }

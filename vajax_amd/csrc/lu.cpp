@@ -23,15 +23,46 @@ int pattern_upload_csr(vjx_pattern* pat) {
   if (pat->d_indptr.p == nullptr) {
     VJX_TRY_HIP(pat->d_indptr.upload(pat->indptr));
     VJX_TRY_HIP(pat->d_indices.upload(pat->indices));
+    VJX_TRY_HIP(pat->hub_rows.build(pat->indptr.data(), pat->n, 1));
   }
   return VJX_OK;
+}
+
+hipError_t HubPlan::build(const int32_t* seg, int n, int nparts) {
+  std::vector<int32_t> hi, hp(1, 0), hcs, hce;
+  for (int i = 0; i < n; ++i) {
+    const int u0 = seg[i], u1 = seg[i + 1];
+    if (u1 - u0 <= kHeavySegment) continue;
+    hi.push_back(i);
+    for (int c = u0; c < u1; c += kHubChunk) {
+      hcs.push_back(c);
+      hce.push_back(u1 - c > kHubChunk ? c + kHubChunk : u1);
+    }
+    hp.push_back((int32_t)hcs.size());
+  }
+  n_items = (int)hi.size();
+  n_chunks = (int)hcs.size();
+  hipError_t e = item.upload(hi);
+  if (e == hipSuccess) e = ptr.upload(hp);
+  if (e == hipSuccess) e = cs.upload(hcs);
+  if (e == hipSuccess) e = ce.upload(hce);
+  if (e == hipSuccess) e = part.alloc(n_chunks);
+  if (e == hipSuccess && nparts > 1) e = part2.alloc(n_chunks);
+  if (e == hipSuccess && nparts > 2) e = part3.alloc(n_chunks);
+  return e;
 }
 
 static hipError_t upload_sched(DevSchedule& d, const LevelSchedule& h) {
   hipError_t e = d.items.upload(h.items);
   if (e != hipSuccess) return e;
   e = d.lvl_dev.upload(h.lvl_ptr);
+  if (e != hipSuccess) return e;
+  e = d.heavy_dev.upload(h.lvl_heavy);
+  if (e != hipSuccess) return e;
+  e = d.mid_dev.upload(h.lvl_mid);
   d.lvl_ptr = h.lvl_ptr;
+  d.lvl_heavy = h.lvl_heavy;
+  d.lvl_mid = h.lvl_mid;
   d.groups = h.groups;
   return e;
 }
@@ -126,7 +157,8 @@ int pattern_solve_async(vjx_pattern* pat, const double* data_dev, const double* 
     for (int r = 0; r < refine_steps; ++r) {
       // residual of A x = scale*rhs, in the reference's form
       // (solver_factories.py:911-921): r = rhs + A x for scale -1.
-      launch_spmv(n, pat->d_indptr.p, pat->d_indices.p, data_dev, x, rhs, -scale, L.r64.p, s);
+      launch_spmv(n, pat->d_indptr.p, pat->d_indices.p, data_dev, x, rhs, -scale, L.r64.p,
+                  pat->hub_rows.view(), s);
       launch_lu_solve<float>(L, L.lu32.p, L.r64.p, scale, L.x64.p, L.y32.p, L.z32.p, s);
       launch_axpy(n, x, L.x64.p, s);
     }

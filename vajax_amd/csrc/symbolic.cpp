@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <set>
@@ -43,12 +44,13 @@ bool csr_validate(int n, int nnz, const int32_t* indptr, const int32_t* indices,
   return true;
 }
 
-// Minimum degree ordering on the graph of A + A^T with explicit elimination
-// (cliques of eliminated neighbourhoods are materialised).  High-degree rows
-// are removed first and appended at the end.
-static void min_degree_order(int n, const int32_t* indptr, const int32_t* indices,
-                             std::vector<int32_t>& order, int& n_dense) {
-  std::vector<std::vector<int32_t>> adj(n);
+// ---- orderings ----------------------------------------------------------
+
+using Adj = std::vector<std::vector<int32_t>>;
+
+// Symmetric adjacency of A + A^T without the diagonal, sorted and unique.
+static void build_graph(int n, const int32_t* indptr, const int32_t* indices, Adj& adj) {
+  adj.assign(n, {});
   for (int r = 0; r < n; ++r)
     for (int e = indptr[r]; e < indptr[r + 1]; ++e) {
       int c = indices[e];
@@ -60,38 +62,62 @@ static void min_degree_order(int n, const int32_t* indptr, const int32_t* indice
     std::sort(a.begin(), a.end());
     a.erase(std::unique(a.begin(), a.end()), a.end());
   }
+}
+
+// Rows with degree above max(16, 10 sqrt(n)) (supply rails) are taken out of
+// the graph; returned in increasing degree order for the end of the ordering.
+static std::vector<int32_t> split_dense(int n, Adj& adj) {
   const size_t thresh = std::max<size_t>(16, (size_t)(10.0 * std::sqrt((double)n)));
   std::vector<char> dense(n, 0);
-  std::vector<std::pair<size_t, int>> dense_list;
-  n_dense = 0;
+  std::vector<std::pair<size_t, int>> dl;
   for (int v = 0; v < n; ++v)
     if (adj[v].size() > thresh) {
       dense[v] = 1;
-      dense_list.push_back({adj[v].size(), v});
-      ++n_dense;
+      dl.push_back({adj[v].size(), v});
     }
-  if (n_dense)
+  if (!dl.empty())
     for (int v = 0; v < n; ++v) {
-      if (dense[v]) continue;
+      if (dense[v]) {
+        adj[v].clear();
+        continue;
+      }
       auto& a = adj[v];
       a.erase(std::remove_if(a.begin(), a.end(), [&](int u) { return dense[u] != 0; }), a.end());
     }
+  std::sort(dl.begin(), dl.end());
+  std::vector<int32_t> out;
+  for (auto& d : dl) out.push_back(d.second);
+  return out;
+}
+
+// Minimum degree with explicit elimination on the subgraph induced by
+// `nodes` (local copy of the adjacency); ties broken by vertex id.
+static void md_subset(const Adj& adj, const std::vector<int32_t>& nodes, std::vector<int32_t>& loc,
+                      std::vector<int32_t>& order) {
+  const int k = (int)nodes.size();
+  if (k == 0) return;
+  for (int i = 0; i < k; ++i) loc[nodes[i]] = i;
+  Adj g(k);
+  for (int i = 0; i < k; ++i) {
+    for (int u : adj[nodes[i]]) {
+      int j = loc[u];
+      if (j >= 0 && j < k && nodes[j] == u) g[i].push_back(j);
+    }
+    std::sort(g[i].begin(), g[i].end());
+  }
   std::set<std::pair<size_t, int>> heap;
-  for (int v = 0; v < n; ++v)
-    if (!dense[v]) heap.insert({adj[v].size(), v});
-  order.clear();
-  order.reserve(n);
+  for (int i = 0; i < k; ++i) heap.insert({g[i].size(), i});
   std::vector<int32_t> nb, merged;
   while (!heap.empty()) {
     int v = heap.begin()->second;
     heap.erase(heap.begin());
-    order.push_back(v);
-    nb.swap(adj[v]);
-    adj[v].clear();
+    order.push_back(nodes[v]);
+    nb.swap(g[v]);
+    g[v].clear();
     for (int u : nb) {
-      heap.erase({adj[u].size(), u});
+      heap.erase({g[u].size(), u});
       merged.clear();
-      const auto& au = adj[u];
+      const auto& au = g[u];
       size_t i = 0, j = 0;
       while (i < au.size() || j < nb.size()) {
         int x;
@@ -106,13 +132,158 @@ static void min_degree_order(int n, const int32_t* indptr, const int32_t* indice
         }
         if (x != u && x != v) merged.push_back(x);
       }
-      adj[u].swap(merged);
-      heap.insert({adj[u].size(), u});
+      g[u].swap(merged);
+      heap.insert({g[u].size(), u});
     }
     nb.clear();
   }
-  std::sort(dense_list.begin(), dense_list.end());
-  for (auto& d : dense_list) order.push_back(d.second);
+  for (int i = 0; i < k; ++i) loc[nodes[i]] = -1;
+}
+
+namespace {
+
+// Nested dissection with BFS level-structure vertex separators.
+struct Dissector {
+  const Adj& adj;
+  int leaf;
+  std::vector<int32_t> stamp, dist, loc;
+  std::vector<int32_t>& order;
+  int next_stamp = 1;
+
+  Dissector(const Adj& a, int n, int leaf_, std::vector<int32_t>& ord)
+      : adj(a), leaf(leaf_), stamp(n, 0), dist(n, -1), loc(n, -1), order(ord) {}
+
+  // BFS inside the current subset (stamp == s); returns visit order and
+  // fills dist.
+  void bfs(int s, int start, std::vector<int32_t>& visit) {
+    visit.clear();
+    visit.push_back(start);
+    dist[start] = 0;
+    for (size_t h = 0; h < visit.size(); ++h) {
+      int v = visit[h];
+      for (int u : adj[v])
+        if (stamp[u] == s && dist[u] < 0) {
+          dist[u] = dist[v] + 1;
+          visit.push_back(u);
+        }
+    }
+  }
+  void clear_dist(const std::vector<int32_t>& visit) {
+    for (int v : visit) dist[v] = -1;
+  }
+
+  void run(std::vector<int32_t> nodes) {
+    if ((int)nodes.size() <= leaf) {
+      md_subset(adj, nodes, loc, order);
+      return;
+    }
+    const int s = next_stamp++;
+    for (int v : nodes) stamp[v] = s;
+    // connected components
+    std::vector<int32_t> visit;
+    std::vector<std::vector<int32_t>> comps;
+    for (int v : nodes) {
+      if (dist[v] >= 0) continue;
+      bfs(s, v, visit);
+      comps.push_back(visit);
+    }
+    for (auto& c : comps) clear_dist(c);
+    if (comps.size() > 1) {
+      for (auto& c : comps) run(c);
+      return;
+    }
+    // pseudo-peripheral start: two sweeps to the far end
+    int start = nodes[0];
+    for (int sweep = 0; sweep < 2; ++sweep) {
+      bfs(s, start, visit);
+      int far = visit.back();
+      const int dmax = dist[far];
+      for (int v : visit)
+        if (dist[v] == dmax && adj[v].size() < adj[far].size()) far = v;
+      clear_dist(visit);
+      start = far;
+    }
+    bfs(s, start, visit);
+    const int nlev = dist[visit.back()] + 1;
+    std::vector<int32_t> lsize(nlev, 0);
+    for (int v : visit) lsize[dist[v]]++;
+    if (nlev < 3) {
+      clear_dist(visit);
+      md_subset(adj, nodes, loc, order);
+      return;
+    }
+    // level where the cumulative count crosses half; then the smallest level
+    // within a window around it
+    const int total = (int)visit.size();
+    int m = 0, cum = 0;
+    while (m < nlev - 1 && cum + lsize[m] < total / 2) cum += lsize[m++];
+    const int w = std::max(1, nlev / 8);
+    int best = m;
+    for (int l = std::max(1, m - w); l <= std::min(nlev - 2, m + w); ++l)
+      if (lsize[l] < lsize[best] || (lsize[l] == lsize[best] && std::abs(l - m) < std::abs(best - m)))
+        best = l;
+    if (best < 1) best = 1;
+    if (best > nlev - 2) best = nlev - 2;
+    std::vector<int32_t> A, B, S;
+    for (int v : visit) {
+      int d = dist[v];
+      if (d < best)
+        A.push_back(v);
+      else if (d > best)
+        B.push_back(v);
+      else
+        S.push_back(v);
+    }
+    // thin the separator: a vertex with no neighbour beyond it joins A
+    std::vector<int32_t> S2;
+    for (int v : S) {
+      bool touches_b = false;
+      for (int u : adj[v])
+        if (stamp[u] == s && dist[u] == best + 1) {
+          touches_b = true;
+          break;
+        }
+      if (touches_b)
+        S2.push_back(v);
+      else
+        A.push_back(v);
+    }
+    clear_dist(visit);
+    if (A.empty() || B.empty()) {
+      md_subset(adj, nodes, loc, order);
+      return;
+    }
+    run(A);
+    run(B);
+    // separator last; minimum degree inside it on the separator graph
+    md_subset(adj, S2, loc, order);
+  }
+};
+
+}  // namespace
+
+static void order_columns(int n, const int32_t* indptr, const int32_t* indices, int method,
+                          int leaf, std::vector<int32_t>& order, int& n_dense) {
+  Adj adj;
+  build_graph(n, indptr, indices, adj);
+  std::vector<int32_t> dense = split_dense(n, adj);
+  n_dense = (int)dense.size();
+  std::vector<char> is_dense(n, 0);
+  for (int v : dense) is_dense[v] = 1;
+  std::vector<int32_t> rest;
+  rest.reserve(n);
+  for (int v = 0; v < n; ++v)
+    if (!is_dense[v]) rest.push_back(v);
+  order.clear();
+  order.reserve(n);
+  if (method == 2) {
+    Dissector nd(adj, n, std::max(4, leaf), order);
+    nd.run(rest);
+  } else {
+    std::vector<int32_t> loc(n, -1);
+    md_subset(adj, rest, loc, order);
+  }
+  for (int v : dense) order.push_back(v);
 }
 
 namespace {
@@ -229,29 +400,43 @@ static bool gp_factor(int n, const std::vector<int32_t>& cp, const std::vector<i
 static void make_groups(LevelSchedule& s, int wide_threshold) {
   s.groups.clear();
   s.max_width = 0;
+  s.max_group_levels = 0;
   int nl = (int)s.lvl_ptr.size() - 1;
   int l = 0;
+  // a level is launched as a grid when it has many items or more than one
+  // long segment (those then run one workgroup each, in parallel)
+  // (one workgroup can overlap ~2 waves of short items or 2 rounds of
+  // lane groups before a separate grid launch pays off)
+  auto is_wide = [&](int lv) {
+    const int w = s.lvl_ptr[lv + 1] - s.lvl_ptr[lv];
+    const int nh = s.lvl_heavy.empty() ? 0 : s.lvl_heavy[lv];
+    const int nm = s.lvl_mid.empty() ? 0 : s.lvl_mid[lv];
+    return w >= wide_threshold || nh > 1 || nm > 64;
+  };
   while (l < nl) {
     int w = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
     s.max_width = std::max(s.max_width, w);
-    if (w >= wide_threshold) {
+    if (is_wide(l)) {
       s.groups.push_back({l, l + 1, 1, w});
       ++l;
       continue;
     }
     int l0 = l, items = 0;
-    while (l < nl) {
+    while (l < nl && l - l0 < kMaxNarrowLevels) {
       int ww = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
-      if (ww >= wide_threshold) break;
+      if (is_wide(l)) break;
       s.max_width = std::max(s.max_width, ww);
       items += ww;
       ++l;
     }
     s.groups.push_back({l0, l, 0, items});
+    s.max_group_levels = std::max(s.max_group_levels, l - l0);
   }
 }
 
-static void rows_by_level(const std::vector<int32_t>& level, LevelSchedule& s) {
+// Rows grouped by level; inside a level heavy rows (long segments) first.
+template <typename SegLen>
+static void rows_by_level(const std::vector<int32_t>& level, SegLen seglen, LevelSchedule& s) {
   int n = (int)level.size();
   int nl = 0;
   for (int v : level) nl = std::max(nl, v + 1);
@@ -259,8 +444,24 @@ static void rows_by_level(const std::vector<int32_t>& level, LevelSchedule& s) {
   for (int v : level) s.lvl_ptr[v + 1]++;
   for (int l = 0; l < nl; ++l) s.lvl_ptr[l + 1] += s.lvl_ptr[l];
   s.items.assign(n, 0);
-  std::vector<int32_t> fill(s.lvl_ptr.begin(), s.lvl_ptr.end() - 1);
-  for (int i = 0; i < n; ++i) s.items[fill[level[i]]++] = i;
+  s.lvl_heavy.assign(nl, 0);
+  s.lvl_mid.assign(nl, 0);
+  for (int i = 0; i < n; ++i) {
+    const int c = reduction_class(seglen(i));
+    if (c == 2) s.lvl_heavy[level[i]]++;
+    if (c == 1) s.lvl_mid[level[i]]++;
+  }
+  std::vector<int32_t> fill[3];
+  for (auto& f : fill) f.resize(nl);
+  for (int l = 0; l < nl; ++l) {
+    fill[2][l] = s.lvl_ptr[l];
+    fill[1][l] = s.lvl_ptr[l] + s.lvl_heavy[l];
+    fill[0][l] = s.lvl_ptr[l] + s.lvl_heavy[l] + s.lvl_mid[l];
+  }
+  for (int i = 0; i < n; ++i) {
+    const int c = reduction_class(seglen(i));
+    s.items[fill[c][level[i]]++] = i;
+  }
 }
 
 bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const double* vals,
@@ -290,7 +491,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     s.q.resize(n);
     std::iota(s.q.begin(), s.q.end(), 0);
   } else {
-    min_degree_order(n, indptr, indices, s.q, s.n_dense);
+    order_columns(n, indptr, indices, opt.ordering, opt.nd_leaf, s.q, s.n_dense);
   }
   s.qinv.assign(n, 0);
   for (int k = 0; k < n; ++k) s.qinv[s.q[k]] = k;
@@ -417,19 +618,35 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
       }
     }
     std::sort(ups.begin(), ups.end());
-    for (size_t a = 0; a < ups.size(); ++a) {
-      const auto& u = ups[a];
-      if (a == 0 || std::get<0>(ups[a - 1]) != std::get<0>(u)) {
-        if (a != 0) s.tgt_seg.push_back((int32_t)s.upd_l.size());
-        s.fact.items.push_back((int32_t)s.tgt_pos.size());
-        s.tgt_pos.push_back(std::get<0>(u));
-      }
-      s.upd_l.push_back(std::get<2>(u));
-      s.upd_u.push_back(std::get<3>(u));
-      s.upd_d.push_back(s.diag[std::get<1>(u)]);
+    // segments of this level: [begin, end) ranges of ups with equal target
+    std::vector<std::pair<size_t, size_t>> segs;
+    for (size_t a0 = 0; a0 < ups.size();) {
+      size_t a1 = a0 + 1;
+      while (a1 < ups.size() && std::get<0>(ups[a1]) == std::get<0>(ups[a0])) ++a1;
+      segs.push_back({a0, a1});
+      a0 = a1;
     }
-    if (!ups.empty()) s.tgt_seg.push_back((int32_t)s.upd_l.size());
-    if (!ups.empty()) s.fact.lvl_ptr.push_back((int32_t)s.fact.items.size());
+    if (segs.empty()) continue;
+    int heavy = 0, mid = 0;
+    for (int pass = 2; pass >= 0; --pass)
+      for (const auto& sg : segs) {
+        const int cls = reduction_class((int)(sg.second - sg.first));
+        if (cls != pass) continue;
+        if (cls == 2) ++heavy;
+        if (cls == 1) ++mid;
+        s.fact.items.push_back((int32_t)s.tgt_pos.size());
+        s.tgt_pos.push_back(std::get<0>(ups[sg.first]));
+        for (size_t a1 = sg.first; a1 < sg.second; ++a1) {
+          const auto& u = ups[a1];
+          s.upd_l.push_back(std::get<2>(u));
+          s.upd_u.push_back(std::get<3>(u));
+          s.upd_d.push_back(s.diag[std::get<1>(u)]);
+        }
+        s.tgt_seg.push_back((int32_t)s.upd_l.size());
+      }
+    s.fact.lvl_ptr.push_back((int32_t)s.fact.items.size());
+    s.fact.lvl_heavy.push_back(heavy);
+    s.fact.lvl_mid.push_back(mid);
   }
   s.n_updates = (int64_t)s.upd_l.size();
   // L scaling list
@@ -452,12 +669,45 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) lv = std::max(lv, lb[s.col[e]] + 1);
     lb[i] = lv;
   }
-  rows_by_level(lf, s.fwd);
-  rows_by_level(lb, s.bwd);
+  rows_by_level(lf, [&](int i) { return s.diag[i] - s.rowptr[i]; }, s.fwd);
+  rows_by_level(lb, [&](int i) { return s.rowptr[i + 1] - s.diag[i] - 1; }, s.bwd);
   make_groups(s.fact, opt.wide_threshold);
   make_groups(s.fwd, opt.wide_threshold);
   make_groups(s.bwd, opt.wide_threshold);
   return true;
+}
+
+// Fixed-order sums of the GPU kernels: G lanes take strided partial sums,
+// then a halving tree (lane t adds lane t + w).  G = 1 is the plain
+// sequential sum.  Emulated here so the CPU program matches the device bit
+// for bit.
+template <int G, typename T, typename F>
+static T lane_sum(int start, int end, F term) {
+  T part[G];
+  for (int t = 0; t < G; ++t) {
+    T acc = (T)0;
+    for (int u = start + t; u < end; u += G) acc += term(u);
+    part[t] = acc;
+  }
+  for (int w = G / 2; w > 0; w >>= 1)
+    for (int t = 0; t < w; ++t) part[t] += part[t + w];
+  return part[0];
+}
+
+template <typename T, typename F>
+static T class_sum(int cls, int start, int end, F term) {
+  if (cls == 2) return lane_sum<kReduceLanes, T>(start, end, term);
+  if (cls == 1) return lane_sum<kGroupLanes, T>(start, end, term);
+  T acc = (T)0;
+  for (int u = start; u < end; ++u) acc += term(u);
+  return acc;
+}
+
+static int item_class(const LevelSchedule& S, int l, int it) {
+  const int lo = S.lvl_ptr[l];
+  if (it < lo + S.lvl_heavy[l]) return 2;
+  if (it < lo + S.lvl_heavy[l] + S.lvl_mid[l]) return 1;
+  return 0;
 }
 
 template <typename T>
@@ -465,12 +715,11 @@ void lu_refactor_host(const LuSymbolic& s, const double* a, T* lu) {
   const int64_t nnz = s.rowptr[s.n];
   for (int64_t e = 0; e < nnz; ++e) lu[e] = s.src[e] >= 0 ? (T)a[s.src[e]] : (T)0;
   const int nl = (int)s.fact.lvl_ptr.size() - 1;
+  auto term = [&](int u) { return lu[s.upd_l[u]] * lu[s.upd_u[u]] / lu[s.upd_d[u]]; };
   for (int l = 0; l < nl; ++l)
     for (int it = s.fact.lvl_ptr[l]; it < s.fact.lvl_ptr[l + 1]; ++it) {
-      int t = s.fact.items[it];
-      T acc = 0;
-      for (int u = s.tgt_seg[t]; u < s.tgt_seg[t + 1]; ++u)
-        acc += lu[s.upd_l[u]] * lu[s.upd_u[u]] / lu[s.upd_d[u]];
+      const int t = s.fact.items[it];
+      const T acc = class_sum<T>(item_class(s.fact, l, it), s.tgt_seg[t], s.tgt_seg[t + 1], term);
       lu[s.tgt_pos[t]] -= acc;
     }
   for (size_t e = 0; e < s.l_slot.size(); ++e) lu[s.l_slot[e]] /= lu[s.l_diag[e]];
@@ -483,16 +732,16 @@ void lu_solve_host(const LuSymbolic& s, const T* lu, const double* b, double* x)
   for (int i = 0; i < n; ++i) y[i] = (T)b[s.p[i]];
   for (int l = 0; l + 1 < (int)s.fwd.lvl_ptr.size(); ++l)
     for (int it = s.fwd.lvl_ptr[l]; it < s.fwd.lvl_ptr[l + 1]; ++it) {
-      int i = s.fwd.items[it];
-      T acc = 0;
-      for (int e = s.rowptr[i]; e < s.diag[i]; ++e) acc += lu[e] * y[s.col[e]];
+      const int i = s.fwd.items[it];
+      auto term = [&](int e) { return lu[e] * y[s.col[e]]; };
+      const T acc = class_sum<T>(item_class(s.fwd, l, it), s.rowptr[i], s.diag[i], term);
       y[i] = y[i] - acc;
     }
   for (int l = 0; l + 1 < (int)s.bwd.lvl_ptr.size(); ++l)
     for (int it = s.bwd.lvl_ptr[l]; it < s.bwd.lvl_ptr[l + 1]; ++it) {
-      int i = s.bwd.items[it];
-      T acc = 0;
-      for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) acc += lu[e] * z[s.col[e]];
+      const int i = s.bwd.items[it];
+      auto term = [&](int e) { return lu[e] * z[s.col[e]]; };
+      const T acc = class_sum<T>(item_class(s.bwd, l, it), s.diag[i] + 1, s.rowptr[i + 1], term);
       z[i] = (y[i] - acc) / lu[s.diag[i]];
     }
   for (int i = 0; i < n; ++i) x[s.q[i]] = (double)z[i];

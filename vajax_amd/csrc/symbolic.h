@@ -5,8 +5,10 @@
 // sequence are chosen once per topology from the first numeric matrix; every
 // later Newton iteration only runs the numeric refactorisation on the GPU.
 //
-//  * column order q: minimum degree on the pattern of A + A^T, with rows
-//    whose degree exceeds max(16, 10 sqrt(n)) (supply rails) ordered last;
+//  * column order q on the pattern of A + A^T, rows whose degree exceeds
+//    max(16, 10 sqrt(n)) (supply rails) ordered last: nested dissection with
+//    BFS level-structure separators (shallow elimination tree, hence few
+//    levels) and minimum degree inside the leaves, or plain minimum degree;
 //  * row pivots p: left-looking Gilbert-Peierls LU of A(:, q) with threshold
 //    partial pivoting (tolerance 0.1, the UMFPACK setting at
 //    umfpack_ffi.cpp:153), preferring the structural diagonal;
@@ -35,12 +37,30 @@ struct LaunchGroup {
   int items;    // number of items over the group
 };
 
+// Items of one level are stored by reduction class: first lvl_heavy[l] items
+// with long segments (one workgroup each), then lvl_mid[l] items with medium
+// segments (kGroupLanes lanes each), then short items (one lane each).  Every
+// class sums in a fixed order (strided lane partials, then a halving tree),
+// which the CPU program reproduces exactly.
 struct LevelSchedule {
-  std::vector<int32_t> items;    // item ids grouped by level
-  std::vector<int32_t> lvl_ptr;  // level l covers items[lvl_ptr[l] .. lvl_ptr[l+1])
+  std::vector<int32_t> items;      // item ids grouped by level
+  std::vector<int32_t> lvl_ptr;    // level l covers items[lvl_ptr[l] .. lvl_ptr[l+1])
+  std::vector<int32_t> lvl_heavy;  // workgroup items at the start of each level
+  std::vector<int32_t> lvl_mid;    // lane-group items after them
   std::vector<LaunchGroup> groups;
   int max_width = 0;
+  int max_group_levels = 0;
 };
+
+constexpr int kHeavySegment = 128;    // longer segments: one workgroup
+constexpr int kShortSegment = 3;      // shorter or equal: one lane
+constexpr int kGroupLanes = 16;       // lanes per medium segment
+constexpr int kReduceLanes = 256;     // lanes of the fixed-order workgroup sum
+constexpr int kMaxNarrowLevels = 4096;  // levels per single-block launch (LDS cache)
+
+inline int reduction_class(int len) {
+  return len > kHeavySegment ? 2 : (len > kShortSegment ? 1 : 0);
+}
 
 struct LuSymbolic {
   int n = 0;
@@ -71,8 +91,9 @@ struct LuSymbolic {
 
 struct AnalyzeOptions {
   double pivot_tol = 0.1;
-  int wide_threshold = 4096;  // levels with at least this many items get a grid
-  int ordering = 0;           // 0 = minimum degree, 1 = natural
+  int wide_threshold = 2048;  // levels with at least this many items get a grid
+  int ordering = 2;           // 0 = minimum degree, 1 = natural, 2 = nested dissection
+  int nd_leaf = 48;           // nested-dissection leaf size (min-degree inside)
 };
 
 // Full analysis from a CSR matrix (host arrays).  Returns false and sets err on

@@ -20,6 +20,9 @@ namespace vjx {
 
 namespace {
 
+bool g_profile = false;
+vjx_phase_times g_phase = {0.0, 0.0, 0.0, 0};
+
 double bits_to_double(unsigned long long b) {
   double d;
   std::memcpy(&d, &b, sizeof(d));
@@ -97,29 +100,113 @@ struct Driver {
     return VJX_OK;
   }
 
-  // One Newton iteration (solver_factories.py:230-325) on Xw.
-  int nr_iteration(const double* Qp, const double* dQ, const double* Q2, const double* floor) {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const double* graph_args[4] = {nullptr, nullptr, nullptr, nullptr};
+  int graph_launches = 0;
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
+  ~Driver() {
+    drop_graph();
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+
+  // after a stream sync: fold the event pairs of the last iteration in
+  void collect_phase_times() {
+    if (!ev_pending) return;
+    float a = 0.f, b = 0.f, d = 0.f;
+    if (hipEventElapsedTime(&a, ev[0], ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&b, ev[1], ev[2]) == hipSuccess &&
+        hipEventElapsedTime(&d, ev[2], ev[3]) == hipSuccess) {
+      g_phase.build_ms += a;
+      g_phase.lu_ms += b;
+      g_phase.solve_ms += d;
+      g_phase.n_iterations += 1;
+    }
+    ev_pending = false;
+  }
+
+  // Symbolic analysis (ordering + pivots) from the Jacobian at Xw with the
+  // current scalars; run before the timed loop.
+  int ensure_analysis(const double* Qp, const double* dQ, const double* Q2) {
+    vjx_pattern* pat = c->pat;
+    if (pat->lu.analyzed && pat->uploaded) return VJX_OK;
+    circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, nullptr, w.csr.p, w.f.p,
+                        w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, s);
+    VJX_TRY_HIP(hipGetLastError());
+    VJX_TRY_HIP(hipStreamSynchronize(s));
+    return pattern_ensure_analyzed(pat, w.csr.p);
+  }
+
+  // One Newton iteration (solver_factories.py:230-325) on Xw, kernels only
+  // (capturable): save X, reset status, build, refactor, solve, delta test,
+  // damped update.
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool ev_pending = false;
+
+  int enqueue_iteration(const double* Qp, const double* dQ, const double* Q2,
+                        const double* floor) {
+    vjx_pattern* pat = c->pat;
+    const bool prof = g_profile && !cfg->use_graph;
+    if (prof && !ev[0])
+      for (auto& e : ev) VJX_TRY_HIP(hipEventCreate(&e));
+    launch_copy(n_total + m, w.Xw.p, w.Xsave.p, s);
+    launch_status_reset(pat->d_status.p, s);
     launch_status_reset(w.st.p, s);
+    if (prof) VJX_TRY_HIP(hipEventRecord(ev[0], s));
     circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, floor, w.csr.p, w.f.p, w.Qn.p,
                         w.mrc.p, w.Ivs.p, w.st.p, s);
-    ++total_evals;
-    vjx_pattern* pat = c->pat;
-    // analysis on first use (pivots from the first numeric matrix)
-    if (!pat->lu.analyzed) {
-      VJX_TRY_HIP(hipStreamSynchronize(s));
-      int rc = pattern_ensure_analyzed(pat, w.csr.p);
-      if (rc < 0) return rc;
-    }
-    pat->stream = s;
+    if (prof) VJX_TRY_HIP(hipEventRecord(ev[1], s));
     int rc = pattern_refactor_async(pat, w.csr.p, cfg->precision, false);
     if (rc < 0) return rc;
-    // pivot growth is checked from the status of this iteration
+    if (prof) VJX_TRY_HIP(hipEventRecord(ev[2], s));
     rc = pattern_solve_async(pat, w.csr.p, w.f.p, -1.0, w.delta.p, cfg->refine_steps);
     if (rc < 0) return rc;
-    // LU status lives in the pattern's record; the NR status in w.st
+    if (prof) {
+      VJX_TRY_HIP(hipEventRecord(ev[3], s));
+      ev_pending = true;
+    }
     launch_delta(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
     launch_update(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
-    VJX_TRY_HIP(hipGetLastError());
+    return VJX_OK;
+  }
+
+  int nr_iteration(const double* Qp, const double* dQ, const double* Q2, const double* floor) {
+    vjx_pattern* pat = c->pat;
+    if (!pat->lu.analyzed || !pat->uploaded) {
+      int rc = ensure_analysis(Qp, dQ, Q2);
+      if (rc < 0) return rc;
+      drop_graph();
+    }
+    pat->stream = s;
+    ++total_evals;
+    if (!cfg->use_graph) {
+      int rc = enqueue_iteration(Qp, dQ, Q2, floor);
+      if (rc < 0) return rc;
+      VJX_TRY_HIP(hipGetLastError());
+      return VJX_OK;
+    }
+    const double* args[4] = {Qp, dQ, Q2, floor};
+    if (graph_exec && std::memcmp(args, graph_args, sizeof(args)) != 0) drop_graph();
+    if (!graph_exec) {
+      VJX_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_iteration(Qp, dQ, Q2, floor);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc < 0) return rc;
+      VJX_TRY_HIP(e);
+      graph = g;
+      VJX_TRY_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+      std::memcpy(graph_args, args, sizeof(args));
+    }
+    VJX_TRY_HIP(hipGraphLaunch(graph_exec, s));
+    ++graph_launches;
     return VJX_OK;
   }
 
@@ -134,14 +221,13 @@ struct Driver {
     while (!conv && iters < cfg->tran_itl) {
       int rc = upload_scalars(c0, c1, d1, c2, gshunt, iters);
       if (rc < 0) return rc;
-      launch_copy(n_total + m, w.Xw.p, w.Xsave.p, s);
-      launch_status_reset(pat->d_status.p, s);
       rc = nr_iteration(Qp, dQ, Q2, floor);
       if (rc < 0) return rc;
       VJX_TRY_HIP(hipMemcpyAsync(pat->h_status, pat->d_status.p, sizeof(DevStatus),
                                  hipMemcpyDeviceToHost, s));
       rc = read_status();
       if (rc < 0) return rc;
+      collect_phase_times();
       // pivot monitor: re-pivot on this matrix and redo the iteration from
       // the saved iterate
       const double maxl = bits_to_double(pat->h_status->maxl_bits);
@@ -152,6 +238,7 @@ struct Driver {
         if (rc < 0) return rc;
         rc = pattern_upload(pat);
         if (rc < 0) return rc;
+        drop_graph();  // factor buffers were reallocated
         ++repivots;
         launch_copy(n_total + m, w.Xsave.p, w.Xw.p, s);
         --total_evals;
@@ -273,6 +360,16 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     launch_accept(n_total, 0, 0, w.Xa.p, tmp.p, tmp.p, w.Vmax.p, nullptr, nullptr, nullptr,
                   nullptr, nullptr, nullptr, nullptr, s);
     VJX_TRY_HIP(hipStreamSynchronize(s));
+  }
+  // symbolic analysis before the timed loop, from a transient-like Jacobian
+  // (backward-Euler c0 at the first step size) when the DC solve did not
+  // already provide one
+  {
+    const double dt0 = cfg->tran_fs != 1.0 ? cfg->dt * cfg->tran_fs : cfg->dt;
+    rc = D.upload_scalars(1.0 / dt0, -1.0 / dt0, 0.0, 0.0, 0.0, 0);
+    if (rc < 0) return rc;
+    rc = D.ensure_analysis(w.Qp.p, w.dQ.p, w.Q2.p);
+    if (rc < 0) return rc;
   }
   VJX_TRY_HIP(hipGetLastError());
   VJX_TRY_HIP(hipStreamSynchronize(s));
@@ -476,4 +573,23 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   VJX_TRY_HIP(hipMemcpy(res->I_out, w.iout.p, sizeof(double) * (size_t)step_idx * mo,
                         hipMemcpyDeviceToHost));
   return dc_conv ? VJX_OK : VJX_WARN_NOCONV;
+}
+
+extern "C" int vjx_profile_enable(int32_t enable) {
+  vjx::g_profile = enable != 0;
+  return VJX_OK;
+}
+
+extern "C" int vjx_profile_reset(void) {
+  vjx::g_phase = vjx_phase_times{0.0, 0.0, 0.0, 0};
+  return VJX_OK;
+}
+
+extern "C" int vjx_profile_read(vjx_phase_times* out) {
+  if (!out) {
+    vjx::set_error("vjx_profile_read: null argument");
+    return VJX_ERR_ARG;
+  }
+  *out = vjx::g_phase;
+  return VJX_OK;
 }

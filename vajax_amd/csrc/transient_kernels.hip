@@ -1,4 +1,5 @@
 // Element-wise kernels of the Newton / time-step loop (FP64 throughout).
+#include "block_flags.h"
 #include "circuit.h"
 #include "transient.h"
 
@@ -20,18 +21,6 @@ __device__ __forceinline__ double jmax(double a, double b) {
 __device__ __forceinline__ double jmin(double a, double b) {
   if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000LL);
   return a < b ? a : b;
-}
-
-__device__ __forceinline__ void wave_max(unsigned long long v, unsigned long long* out) {
-  for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long w = __shfl_xor(v, o);
-    v = w > v ? w : v;
-  }
-  if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
-}
-__device__ __forceinline__ void wave_count(bool flag, unsigned int* out) {
-  unsigned long long m = __ballot(flag);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (unsigned int)__popcll(m));
 }
 
 // delta test of solver_factories.py:283-299
@@ -59,9 +48,8 @@ __global__ void __launch_bounds__(256) k_delta(int n_unk, int m, int n_total,
     fail = !(fabs(cd) < tol);
     ad = fabs(d);
   }
-  wave_count(fail, &st->delta_fail);
-  wave_max(dbits(adv), &st->maxdv_bits);
-  wave_max(dbits(ad), &st->maxdelta_bits);
+  block_flags(dbits(adv), dbits(ad), fail ? 1u : 0u, &st->maxdv_bits, &st->maxdelta_bits,
+              &st->delta_fail);
 }
 
 // damped update of solver_factories.py:301-309
@@ -133,7 +121,7 @@ __global__ void __launch_bounds__(256) k_lte(int n, const double* __restrict__ V
     r = fabs(lte) / tol;
     if (r != r) r = __longlong_as_double(0x7ff8000000000000LL);
   }
-  wave_max(dbits(r), &st->maxf_bits);
+  block_flags(dbits(r), 0, 0u, &st->maxf_bits, nullptr, nullptr);
 }
 
 // state update on use_new_solution (full_mna.py:2078-2113)
