@@ -150,7 +150,8 @@ typedef struct {
   const double* vs_params;  /* [8][m] dc, val0, val1, rise, fall, width, period, delay */
   const int32_t* is_wave;   /* [k] */
   const double* is_params;  /* [8][k] */
-  const int32_t* is_dst;    /* [2][k] residual contribution slots of +I at p, -I at n */
+  const int32_t* is_dst;    /* [k][2] residual contribution slots of +I at p, -I at n
+                               (interleaved: is_dst[2i], is_dst[2i+1]; -1 = ground) */
   int32_t n_jac_contrib;
   int32_t n_res_contrib;
   const int32_t* jac_seg;   /* [nse+1] */
