@@ -94,6 +94,10 @@ typedef struct {
   int32_t n_dense_rows;
   int32_t n_offdiag_pivots;
   double min_pivot_ratio;
+  /* block-diagonal (batched) systems: diagonal blocks and the number of
+   * distinct block patterns, each analysed once and shared */
+  int32_t n_blocks;
+  int32_t n_block_patterns;
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);

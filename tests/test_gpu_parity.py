@@ -251,3 +251,27 @@ def test_transient_gate_dag_adaptive():
     e_all = waveform_error(ref.times, ref.V, res.times, res.stats["V_out"])
     assert e_all <= 3.5e-3, e_all
     assert res.times[-1] == pytest.approx(0.2e-9, rel=1e-12)
+
+
+def test_packed_batch_fixed_step_fp64_and_mixed():
+    """Monte Carlo copies plus ragged topologies packed into one block-diagonal
+    system (one analysis per distinct block pattern, one launch per level for
+    all instances): same time grid, same bars as a single circuit, and each
+    instance's waveform equals the packed oracle's."""
+    nets = [G.monte_carlo(G.gate_dag(120, seed=3), 40 + b, 0.03) for b in range(3)]
+    nets += [G.gate_dag(50, seed=11), G.gate_dag(80, seed=12)]
+    net = G.pack(nets)
+    _, ref = run_oracle(net, 0.12e-9, 1e-12, fixed=True, tran_fs=1.0)
+    eng, r64 = run_gpu(net, 0.12e-9, 1e-12, fixed=True, tran_fs=1.0)
+    info = eng.device.lu.info()
+    assert info["n_block_patterns"] < info["n_blocks"]
+    assert np.array_equal(r64.times, ref.times)
+    e64 = float(np.max(np.abs(r64.stats["V_out"] - ref.V)))
+    assert e64 <= 1e-9, e64
+    _, r32 = run_gpu(net, 0.12e-9, 1e-12, fixed=True, tran_fs=1.0, precision="f32")
+    e32 = float(np.max(np.abs(r32.stats["V_out"] - ref.V)))
+    assert e32 <= 1e-6, e32
+    # instances differ (device variation), so the comparison is not vacuous
+    y0 = r64.voltage("i000.g000119y")
+    y1 = r64.voltage("i001.g000119y")
+    assert np.max(np.abs(y0 - y1)) > 1e-6

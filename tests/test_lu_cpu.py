@@ -95,3 +95,64 @@ def test_level_program_statistics():
     assert info["fact_levels"] >= 1 and info["fwd_levels"] >= 1
     assert info["n_updates"] > 0 and info["flops"] >= 2 * info["n_updates"]
     assert info["n_dense_rows"] == 1  # the supply rail
+
+
+def test_packed_batch_shares_block_analysis():
+    # Monte Carlo copies + ragged topologies packed block-diagonally: one
+    # analysis per distinct block pattern, laid out block after block
+    nets = [G.monte_carlo(G.gate_dag(150, seed=4), 10 + b, 0.03) for b in range(3)]
+    nets += [G.gate_dag(60, seed=7), G.gate_dag(90, seed=8)]
+    packed = G.pack(nets)
+    cir, csr, A = _system(packed, seed=3)
+    lu = SparseLU(cir.indptr, cir.indices)
+    lu.analyze(csr)
+    info = lu.info()
+    assert info["n_blocks"] > len(nets)  # instances + isolated unused-input branches
+    # 3 MC copies share one pattern; the two ragged DAGs add two more
+    assert info["n_block_patterns"] <= info["n_blocks"] - 2
+    b = np.random.default_rng(2).standard_normal(cir.n_aug)
+    x = lu.solve_cpu(csr, b)
+    assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(b)
+    x32 = lu.solve_cpu(csr, b, 1)
+    assert np.linalg.norm(A @ x32 - b) <= 1e-5 * np.linalg.norm(b)
+    # the merged level program runs as many levels as the deepest block
+    single = []
+    for net in nets:
+        c1, d1, _ = _system(net, seed=3)
+        l1 = SparseLU(c1.indptr, c1.indices)
+        l1.analyze(d1)
+        single.append(l1.info())
+    assert info["fact_levels"] <= max(s["fact_levels"] for s in single)
+    assert info["nnz_l"] + info["nnz_u"] >= sum(s["nnz_l"] + s["nnz_u"] for s in single) * 0.5
+
+
+def test_packed_batch_block_equals_standalone():
+    # the first Monte Carlo instance's block of the packed LU equals the
+    # standalone LU of that instance bit for bit (same pattern, same values,
+    # same pivot order, same fixed-order update program)
+    nets = [G.monte_carlo(G.gate_dag(120, seed=9), 30 + b, 0.03) for b in range(3)]
+    packed = G.pack(nets)
+    cir = mna.setup(packed)
+    c0 = mna.setup(nets[0])
+    rng = np.random.default_rng(5)
+    X0 = np.zeros(c0.n_total + c0.m)
+    X0[1:c0.n_total] = rng.uniform(0.2, 0.8, c0.n_unk)
+    # same node voltages for every instance, mapped through the prefixed names
+    X = np.zeros(cir.n_total + cir.m)
+    for name, k in c0.node_names.items():
+        if k == 0:
+            continue  # ground is shared
+        for b in range(3):
+            X[cir.node_names[f"i{b:03d}.{name}"]] = X0[k]
+    d0, *_ = mna.build_system(c0, X0, np.zeros(c0.m), np.zeros(0), np.zeros(c0.n_unk), 1e12)
+    dp, *_ = mna.build_system(cir, X, np.zeros(cir.m), np.zeros(0), np.zeros(cir.n_unk), 1e12)
+    l0 = SparseLU(c0.indptr, c0.indices)
+    l0.analyze(d0)
+    v0 = l0.refactor_cpu(d0)
+    lp = SparseLU(cir.indptr, cir.indices)
+    lp.analyze(dp)
+    vp = lp.refactor_cpu(dp)
+    # instance 0 owns the lowest-numbered vertices, so its block comes first
+    i0 = lp.info()
+    assert i0["n_blocks"] >= 3
+    assert np.array_equal(vp[: len(v0)], v0)

@@ -35,6 +35,8 @@ class LuInfo(C.Structure):
         ("fact_launches", C.c_int32), ("fwd_launches", C.c_int32), ("bwd_launches", C.c_int32),
         ("n_dense_rows", C.c_int32), ("n_offdiag_pivots", C.c_int32),
         ("min_pivot_ratio", C.c_double),
+        ("n_blocks", C.c_int32),
+        ("n_block_patterns", C.c_int32),
     ]
 
     def as_dict(self):

@@ -109,6 +109,8 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->n_dense_rows = s.n_dense;
   o->n_offdiag_pivots = s.n_offdiag_pivots;
   o->min_pivot_ratio = s.min_pivot_ratio;
+  o->n_blocks = s.n_blocks;
+  o->n_block_patterns = s.n_block_patterns;
   return VJX_OK;
 }
 

@@ -464,10 +464,10 @@ static void rows_by_level(const std::vector<int32_t>& level, SegLen seglen, Leve
   }
 }
 
-bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const double* vals,
-                const AnalyzeOptions& opt, LuSymbolic& s, std::string& err) {
+static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
+                           const double* vals, const AnalyzeOptions& opt, LuSymbolic& s,
+                           std::string& err) {
   const int nnz = indptr[n];
-  if (!csr_validate(n, nnz, indptr, indices, err)) return false;
   s = LuSymbolic();
   s.n = n;
   s.nnz_a = nnz;
@@ -671,6 +671,247 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
   }
   rows_by_level(lf, [&](int i) { return s.diag[i] - s.rowptr[i]; }, s.fwd);
   rows_by_level(lb, [&](int i) { return s.rowptr[i + 1] - s.diag[i] - 1; }, s.bwd);
+  make_groups(s.fact, opt.wide_threshold);
+  make_groups(s.fwd, opt.wide_threshold);
+  make_groups(s.bwd, opt.wide_threshold);
+  return true;
+}
+
+// ---- block-diagonal systems --------------------------------------------------
+//
+// A batch of circuits packed into one system (Monte Carlo instances, several
+// topologies) has a block-diagonal matrix: one block per connected component
+// of the pattern of A + A^T.  Each distinct block pattern is analysed once
+// (from the values of its first instance) and the results are laid out block
+// after block; the level programs are merged level by level, so one launch
+// per level serves every instance.
+
+// Connected components of the pattern of A + A^T, labelled in order of their
+// smallest vertex.
+static int pattern_components(int n, const int32_t* indptr, const int32_t* indices,
+                              std::vector<int32_t>& comp) {
+  std::vector<int32_t> parent(n);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto root = [&](int v) {
+    while (parent[v] != v) {
+      parent[v] = parent[parent[v]];
+      v = parent[v];
+    }
+    return v;
+  };
+  for (int r = 0; r < n; ++r)
+    for (int e = indptr[r]; e < indptr[r + 1]; ++e) {
+      int a = root(r), b = root(indices[e]);
+      if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+  comp.assign(n, -1);
+  std::vector<int32_t> label(n, -1);
+  int nc = 0;
+  for (int v = 0; v < n; ++v) {
+    const int r = root(v);
+    if (label[r] < 0) label[r] = nc++;
+    comp[v] = label[r];
+  }
+  return nc;
+}
+
+struct BlockPattern {
+  std::vector<int32_t> verts;            // global vertices, ascending
+  std::vector<int32_t> indptr, indices;  // local CSR (same entry order as global)
+  uint64_t hash = 0;
+};
+
+static void local_pattern(const int32_t* indptr, const int32_t* indices,
+                          const std::vector<int32_t>& g2l, BlockPattern& b) {
+  b.indptr.assign(1, 0);
+  b.indices.clear();
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t x) {
+    h ^= x;
+    h *= 1099511628211ull;
+  };
+  mix((uint64_t)b.verts.size());
+  for (int v : b.verts) {
+    for (int e = indptr[v]; e < indptr[v + 1]; ++e) {
+      b.indices.push_back(g2l[indices[e]]);
+      mix((uint64_t)g2l[indices[e]] + 1);
+    }
+    b.indptr.push_back((int32_t)b.indices.size());
+    mix(0);
+  }
+  b.hash = h;
+}
+
+// Append the level program `src` (item ids offset by `off`) into per-level,
+// per-class buckets.
+static void bucket_schedule(const LevelSchedule& src, int32_t off,
+                            std::vector<std::vector<int32_t>> (&bk)[3]) {
+  const int nl = (int)src.lvl_ptr.size() - 1;
+  for (auto& b : bk)
+    if ((int)b.size() < nl) b.resize(nl);
+  for (int l = 0; l < nl; ++l) {
+    const int a = src.lvl_ptr[l], h = a + src.lvl_heavy[l], m = h + src.lvl_mid[l];
+    for (int i = a; i < h; ++i) bk[2][l].push_back(src.items[i] + off);
+    for (int i = h; i < m; ++i) bk[1][l].push_back(src.items[i] + off);
+    for (int i = m; i < src.lvl_ptr[l + 1]; ++i) bk[0][l].push_back(src.items[i] + off);
+  }
+}
+
+static void merge_schedule(std::vector<std::vector<int32_t>> (&bk)[3], LevelSchedule& s) {
+  size_t nl = std::max({bk[0].size(), bk[1].size(), bk[2].size()});
+  s.items.clear();
+  s.lvl_ptr.assign(1, 0);
+  s.lvl_heavy.clear();
+  s.lvl_mid.clear();
+  for (size_t l = 0; l < nl; ++l) {
+    size_t before = s.items.size();
+    for (int c = 2; c >= 0; --c)
+      if (l < bk[c].size()) s.items.insert(s.items.end(), bk[c][l].begin(), bk[c][l].end());
+    if (s.items.size() == before) continue;
+    s.lvl_heavy.push_back(l < bk[2].size() ? (int32_t)bk[2][l].size() : 0);
+    s.lvl_mid.push_back(l < bk[1].size() ? (int32_t)bk[1][l].size() : 0);
+    s.lvl_ptr.push_back((int32_t)s.items.size());
+  }
+}
+
+bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                const AnalyzeOptions& opt, LuSymbolic& s, std::string& err) {
+  const int nnz = indptr[n];
+  if (!csr_validate(n, nnz, indptr, indices, err)) return false;
+  std::vector<int32_t> comp;
+  const int nc = n > 0 ? pattern_components(n, indptr, indices, comp) : 0;
+  if (nc <= 1) {
+    if (!analyze_single(n, indptr, indices, vals, opt, s, err)) return false;
+    s.n_blocks = 1;
+    s.n_block_patterns = 1;
+    return true;
+  }
+  // blocks and their local patterns
+  std::vector<BlockPattern> blk(nc);
+  for (int v = 0; v < n; ++v) blk[comp[v]].verts.push_back(v);
+  std::vector<int32_t> g2l(n);
+  for (auto& b : blk)
+    for (size_t i = 0; i < b.verts.size(); ++i) g2l[b.verts[i]] = (int32_t)i;
+  for (auto& b : blk) local_pattern(indptr, indices, g2l, b);
+  // distinct patterns: analyse the first instance of each
+  std::vector<int32_t> kind(nc, -1), rep;
+  for (int c = 0; c < nc; ++c) {
+    for (size_t k = 0; k < rep.size(); ++k) {
+      const BlockPattern& r = blk[rep[k]];
+      if (r.hash == blk[c].hash && r.indptr == blk[c].indptr && r.indices == blk[c].indices) {
+        kind[c] = (int32_t)k;
+        break;
+      }
+    }
+    if (kind[c] < 0) {
+      kind[c] = (int32_t)rep.size();
+      rep.push_back(c);
+    }
+  }
+  std::vector<LuSymbolic> sym(rep.size());
+  for (size_t k = 0; k < rep.size(); ++k) {
+    const BlockPattern& b = blk[rep[k]];
+    std::vector<double> lv(b.indices.size());
+    size_t u = 0;
+    for (int v : b.verts)
+      for (int e = indptr[v]; e < indptr[v + 1]; ++e) lv[u++] = vals[e];
+    if (!analyze_single((int)b.verts.size(), b.indptr.data(), b.indices.data(), lv.data(), opt,
+                        sym[k], err))
+      return false;
+  }
+  // lay the blocks out one after another
+  s = LuSymbolic();
+  s.n = n;
+  s.nnz_a = nnz;
+  s.n_blocks = nc;
+  s.n_block_patterns = (int)rep.size();
+  int64_t nnz_lu = 0, n_upd = 0, n_tgt = 0, n_l = 0;
+  for (int c = 0; c < nc; ++c) {
+    const LuSymbolic& b = sym[kind[c]];
+    nnz_lu += (int64_t)b.col.size();
+    n_upd += (int64_t)b.upd_l.size();
+    n_tgt += (int64_t)b.tgt_pos.size();
+    n_l += (int64_t)b.l_slot.size();
+  }
+  if (nnz_lu > (int64_t)INT32_MAX || n_upd > (int64_t)INT32_MAX) {
+    err = "LU of the packed system exceeds 2^31 entries";
+    return false;
+  }
+  s.q.resize(n);
+  s.p.resize(n);
+  s.rowptr.assign(1, 0);
+  s.col.reserve(nnz_lu);
+  s.diag.reserve(n);
+  s.src.reserve(nnz_lu);
+  s.tgt_pos.reserve(n_tgt);
+  s.tgt_seg.assign(1, 0);
+  s.upd_l.reserve(n_upd);
+  s.upd_u.reserve(n_upd);
+  s.upd_d.reserve(n_upd);
+  s.l_slot.reserve(n_l);
+  s.l_diag.reserve(n_l);
+  s.min_pivot_ratio = 1.0;
+  std::vector<std::vector<int32_t>> bf[3], bfw[3], bbw[3];
+  int32_t roff = 0, soff = 0, toff = 0, uoff = 0;
+  std::vector<int32_t> gmap;
+  for (int c = 0; c < nc; ++c) {
+    const LuSymbolic& b = sym[kind[c]];
+    const BlockPattern& bp = blk[c];
+    const int nb = b.n;
+    // local A entry -> global A entry
+    gmap.resize(bp.indices.size());
+    for (int r = 0; r < nb; ++r)
+      for (int e = bp.indptr[r]; e < bp.indptr[r + 1]; ++e)
+        gmap[e] = indptr[bp.verts[r]] + (e - bp.indptr[r]);
+    for (int k = 0; k < nb; ++k) {
+      s.q[roff + k] = bp.verts[b.q[k]];
+      s.p[roff + k] = bp.verts[b.p[k]];
+    }
+    for (int r = 0; r < nb; ++r) {
+      s.rowptr.push_back(b.rowptr[r + 1] + soff);
+      s.diag.push_back(b.diag[r] + soff);
+    }
+    for (size_t e = 0; e < b.col.size(); ++e) {
+      s.col.push_back(b.col[e] + roff);
+      s.src.push_back(b.src[e] < 0 ? -1 : gmap[b.src[e]]);
+    }
+    for (size_t t = 0; t < b.tgt_pos.size(); ++t) {
+      s.tgt_pos.push_back(b.tgt_pos[t] + soff);
+      s.tgt_seg.push_back(b.tgt_seg[t + 1] + uoff);
+    }
+    for (size_t u = 0; u < b.upd_l.size(); ++u) {
+      s.upd_l.push_back(b.upd_l[u] + soff);
+      s.upd_u.push_back(b.upd_u[u] + soff);
+      s.upd_d.push_back(b.upd_d[u] + soff);
+    }
+    for (size_t e = 0; e < b.l_slot.size(); ++e) {
+      s.l_slot.push_back(b.l_slot[e] + soff);
+      s.l_diag.push_back(b.l_diag[e] + soff);
+    }
+    bucket_schedule(b.fact, toff, bf);
+    bucket_schedule(b.fwd, roff, bfw);
+    bucket_schedule(b.bwd, roff, bbw);
+    s.nnz_l += b.nnz_l;
+    s.nnz_u += b.nnz_u;
+    s.n_updates += b.n_updates;
+    s.flops += b.flops;
+    s.n_dense += b.n_dense;
+    s.n_offdiag_pivots += b.n_offdiag_pivots;
+    s.min_pivot_ratio = std::min(s.min_pivot_ratio, b.min_pivot_ratio);
+    roff += nb;
+    soff += (int32_t)b.col.size();
+    toff += (int32_t)b.tgt_pos.size();
+    uoff += (int32_t)b.upd_l.size();
+  }
+  s.qinv.assign(n, 0);
+  s.pinv.assign(n, 0);
+  for (int k = 0; k < n; ++k) {
+    s.qinv[s.q[k]] = k;
+    s.pinv[s.p[k]] = k;
+  }
+  merge_schedule(bf, s.fact);
+  merge_schedule(bfw, s.fwd);
+  merge_schedule(bbw, s.bwd);
   make_groups(s.fact, opt.wide_threshold);
   make_groups(s.fwd, opt.wide_threshold);
   make_groups(s.bwd, opt.wide_threshold);

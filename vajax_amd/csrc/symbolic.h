@@ -85,6 +85,8 @@ struct LuSymbolic {
   int64_t nnz_l = 0, nnz_u = 0, n_updates = 0;
   double flops = 0.0;
   int n_dense = 0;
+  int n_blocks = 1;          // diagonal blocks (connected components)
+  int n_block_patterns = 1;  // distinct block patterns analysed
   int n_offdiag_pivots = 0;
   double min_pivot_ratio = 1.0;
 };

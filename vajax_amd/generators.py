@@ -273,3 +273,81 @@ def monte_carlo(net: Netlist, seed: int, sigma: float = 0.03) -> Netlist:
         g.params["vto"] = g.params["vto"] + rng.normal(0.0, 1.0, n) * sigma * g.params["vto"]
         g.params["kp"] = g.params["kp"] * rng.normal(1.0, sigma, n)
     return out
+
+
+def pack(nets, title: str = "batch") -> Netlist:
+    """Pack several circuits into one block-diagonal netlist.
+
+    Instance ``b`` keeps its devices and sources; its node ``x`` becomes
+    ``i<b>.x`` (ground stays shared).  The prefixes keep each instance's
+    nodes contiguous in the sorted node numbering, so the packed MNA matrix
+    has one diagonal block per instance and the sparse LU analyses each
+    distinct block pattern once.  All instances advance on the packed
+    system's common time grid.
+    """
+    bw = max(3, len(str(max(len(nets) - 1, 0))))
+    nodes = ["0"]
+    groups: Dict[str, list] = {}
+    vs: list = []
+    is_: list = []
+    for b, net in enumerate(nets):
+        pre = f"i{b:0{bw}d}."
+        gidx = net.nodes.index(net.ground)
+        remap = np.empty(len(net.nodes), dtype=np.int64)
+        for k, name in enumerate(net.nodes):
+            if k == gidx:
+                remap[k] = 0
+            else:
+                remap[k] = len(nodes)
+                nodes.append(pre + name)
+        for model, g in net.groups.items():
+            groups.setdefault(model, []).append((remap[g.terminals], g.params))
+        for src, acc in ((net.vsources, vs), (net.isources, is_)):
+            if src.count:
+                acc.append((pre, remap[src.terminals], src))
+
+    def cat_sources(kind, acc):
+        if not acc:
+            return empty_sources(kind)
+        return SourceGroup(
+            kind=kind,
+            names=[pre + n for pre, _, s in acc for n in s.names],
+            terminals=np.concatenate([t for _, t, _ in acc]),
+            wave=np.concatenate([s.wave for _, _, s in acc]).astype(np.int32),
+            params={k: np.concatenate([s.params[k] for _, _, s in acc]) for k in SOURCE_DEFAULTS},
+        )
+
+    out_groups = {}
+    for model, parts in groups.items():
+        keys = parts[0][1].keys()
+        out_groups[model] = DeviceGroup(
+            model=model,
+            terminals=np.concatenate([t for t, _ in parts]),
+            params={k: np.concatenate([p[k] for _, p in parts]) for k in keys},
+            names=None)
+    first = nets[0]
+    return Netlist(
+        title=title,
+        nodes=nodes,
+        ground="0",
+        groups=out_groups,
+        vsources=cat_sources("vsource", vs),
+        isources=cat_sources("isource", is_),
+        analysis=dict(first.analysis),
+        options=dict(first.options),
+    )
+
+
+def batch_mc(n_instances: int = 256, gates: int = 2000, ragged=(500, 1000, 2000, 4000),
+             seed: int = 0, sigma: float = 0.03) -> Netlist:
+    """Monte Carlo batch plus ragged topologies, packed into one netlist.
+
+    ``n_instances`` copies of the ``gates``-gate DAG (seed ``seed``) with
+    per-device Vth / mobility variation (instance ``b`` drawn with seed
+    ``seed + 1 + b``), followed by one DAG of each size in ``ragged`` (seeds
+    ``seed + 1000 + j``).
+    """
+    base = gate_dag(gates, seed=seed)
+    nets = [monte_carlo(base, seed + 1 + b, sigma) for b in range(n_instances)]
+    nets += [gate_dag(g, seed=seed + 1000 + j) for j, g in enumerate(ragged)]
+    return pack(nets, title=f"batch_mc{n_instances}x{gates}_ragged{len(ragged)}_s{seed}")

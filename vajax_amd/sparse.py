@@ -46,8 +46,10 @@ class SparseLU:
             self._owned = True
 
     def __del__(self):
-        if getattr(self, "_owned", False) and self.h:
-            L.lib.vjx_pattern_destroy(self.h)
+        # at interpreter shutdown the module globals may already be gone
+        lib = getattr(L, "lib", None) if L is not None else None
+        if getattr(self, "_owned", False) and self.h and lib is not None:
+            lib.vjx_pattern_destroy(self.h)
             self.h = None
 
     # ---- host-side analysis ----
