@@ -42,11 +42,12 @@ def build_circuit(gates: int, seed: int):
     return G.gate_dag(gates, seed=seed)
 
 
-def run_gpu(eng, steps: int, precision: str, use_graph: bool):
+def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None):
     from vajax_amd.engine import AdaptiveConfig
 
     eng.prepare(t_stop=1e-9, dt=1e-12, adaptive_config=AdaptiveConfig(integration_method="gear2"),
-                precision=precision, max_steps=steps + 1, use_graph=use_graph)
+                precision=precision, max_steps=steps + 1, use_graph=use_graph,
+                band_width=band_width)
     return eng.run_transient()
 
 
@@ -98,6 +99,65 @@ def cpu_baseline(net, steps: int):
     }
 
 
+def run_batched(args):
+    """Batched throughput: Monte Carlo instances of a 2k-gate DAG plus ragged
+    topologies packed into one block-diagonal system (shared analysis per
+    distinct block pattern, one launch per level for every instance), all
+    instances advancing on the packed system's common time grid."""
+    import torch
+
+    from vajax_amd import generators as G
+    from vajax_amd.engine import CircuitEngine
+
+    t0 = time.time()
+    ragged = tuple(int(g) for g in args.batch_ragged.split(",")) if args.batch_ragged else ()
+    net = G.batch_mc(args.batch, args.batch_gates, ragged=ragged, seed=args.seed)
+    n_inst = args.batch + len(ragged)
+    eng = CircuitEngine(net)
+    eng.parse()
+    topo = eng.topology
+    log(f"batch setup {time.time() - t0:.1f}s: {n_inst} instances, "
+        f"n_unknowns={topo.n_augmented} nse={topo.nse}")
+    out = {"workload": net.title, "instances": n_inst, "unknowns": topo.n_augmented,
+           "mosfets": int(net.groups["mosfet"].count), "unit": "instance·steps/s"}
+    for prec in ("f32", "f64"):
+        t0 = time.time()
+        run_gpu(eng, args.warmup, prec, True)
+        log(f"batch warm-up {prec} {time.time() - t0:.1f}s")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = run_gpu(eng, args.batch_steps, prec, True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        st = res.stats
+        steps = st["n_steps"] - 1
+        out[prec] = {"value": round(n_inst * steps / el, 3), "ms_per_step": round(el / max(steps, 1) * 1e3, 4),
+                     "accepted_steps": steps, "newton_iterations": st["total_nr_iterations"],
+                     "newton_iters_per_s": round(st["total_nr_iterations"] / el, 3)}
+        log(f"batch {prec}: {steps} steps in {el:.3f}s")
+    # the plain form: one instance at a time, FP64
+    single = CircuitEngine(G.gate_dag(args.batch_gates, seed=args.seed))
+    single.parse()
+    run_gpu(single, args.warmup, "f64", True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = run_gpu(single, args.batch_steps, "f64", True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    steps = res.stats["n_steps"] - 1
+    out["single_f64"] = {"value": round(steps / el, 3), "ms_per_step": round(el / max(steps, 1) * 1e3, 4),
+                         "accepted_steps": steps, "workload": f"gate_dag_{args.batch_gates}_s{args.seed}"}
+    info = eng.device.lu.info()
+    out["lu"] = {k: info[k] for k in ("n_blocks", "n_block_patterns", "nnz_l", "nnz_u", "n_updates",
+                                      "fact_levels")}
+    out["value"] = out["f32"]["value"]
+    out["precision"] = "f32-lu+f64"
+    out["mixed_vs_fp64_batched"] = round(out["f32"]["value"] / max(out["f64"]["value"], 1e-30), 3)
+    out["mixed_batched_vs_fp64_single"] = round(out["f32"]["value"] / max(out["single_f64"]["value"], 1e-30), 3)
+    out["time_grid"] = "common to all instances (dt = smallest LTE step over the batch)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,8 +167,14 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=("f64", "f32"), default="f64")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--band-width", type=int, default=None,
+                    help="levels per triangular-solve band (0: one launch per level)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=256, help="Monte Carlo instances (0: skip)")
+    ap.add_argument("--batch-gates", type=int, default=2000)
+    ap.add_argument("--batch-ragged", default="500,1000,2000,4000")
+    ap.add_argument("--batch-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -130,13 +196,13 @@ def main():
 
     use_graph = not args.no_graph
     t0 = time.time()
-    run_gpu(eng, args.warmup, args.precision, use_graph)
+    run_gpu(eng, args.warmup, args.precision, use_graph, args.band_width)
     log(f"warm-up ({args.warmup} steps incl. symbolic analysis) {time.time() - t0:.1f}s")
     lu_info = eng.device.lu.info()
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = run_gpu(eng, args.steps, args.precision, use_graph)
+    res = run_gpu(eng, args.steps, args.precision, use_graph, args.band_width)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = res.stats
@@ -148,7 +214,7 @@ def main():
     # eager pass with phase events on the library stream
     L.lib.vjx_profile_enable(1)
     L.lib.vjx_profile_reset()
-    run_gpu(eng, args.steps, args.precision, False)
+    run_gpu(eng, args.steps, args.precision, False, args.band_width)
     pt = L.PhaseTimes()
     L.lib.vjx_profile_read(C.byref(pt))
     L.lib.vjx_profile_enable(0)
@@ -180,6 +246,10 @@ def main():
         cpu = cpu_baseline(net, args.cpu_steps)
         log(f"cpu baseline {time.time() - t0:.1f}s: {cpu['ms_per_step']:.1f} ms/step")
 
+    batched = None
+    if args.batch > 0:
+        batched = run_batched(args)
+
     line = {
         "metric": METRIC,
         "value": round(nr / elapsed, 3),
@@ -201,6 +271,7 @@ def main():
             "csr_nnz": topo.nse,
             "lu_nnz": lu_info["nnz_l"] + lu_info["nnz_u"],
             "lu_levels": lu_info["fact_levels"],
+            "solve_bands": [lu_info["fwd_bands"], lu_info["bwd_bands"]],
             "integration": "gear2 adaptive",
             "precision": args.precision,
             "hip_graph": use_graph,
@@ -210,6 +281,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     if cpu:
         line["speedup_vs_cpu_ms_per_step"] = round(cpu["ms_per_step"] / ms_per_step, 2)

@@ -98,6 +98,8 @@ typedef struct {
    * distinct block patterns, each analysed once and shared */
   int32_t n_blocks;
   int32_t n_block_patterns;
+  /* triangular-solve band programs: bands (launches) and hub rows */
+  int32_t fwd_bands, bwd_bands, fwd_hub_rows, bwd_hub_rows;
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
@@ -105,6 +107,14 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
 /* Host-side copies of the analysis (for tests): permutations, LU pattern. */
 int vjx_lu_get_perm(const vjx_pattern* pat, int32_t* p_host, int32_t* q_host);
 int vjx_lu_get_pattern(const vjx_pattern* pat, int32_t* rowptr_host, int32_t* col_host);
+/* Dependency check of the analysed solve programs: every row is scheduled
+ * once and after every row it reads.  VJX_OK or VJX_ERR_STATE with a message. */
+int vjx_lu_check_schedules(const vjx_pattern* pat);
+/* Triangular solves in bands of `width` levels, one workgroup per group of
+ * connected rows (default 16; 0 = one launch per level).  A new value drops
+ * the analysis; the next refactorisation analyses again.  No counterpart in
+ * the reference (a launch-plan knob). */
+int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width);
 /* Factorise / solve on the CPU with the same numeric program as the GPU path. */
 int vjx_lu_refactor_cpu(vjx_pattern* pat, const double* data_host, int32_t precision,
                         double* lu_host);

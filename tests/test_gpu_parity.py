@@ -275,3 +275,18 @@ def test_packed_batch_fixed_step_fp64_and_mixed():
     y0 = r64.voltage("i000.g000119y")
     y1 = r64.voltage("i001.g000119y")
     assert np.max(np.abs(y0 - y1)) > 1e-6
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_band_solves_match_level_solves(precision):
+    """Triangular solves as band programs (one launch per band of levels, one
+    workgroup per group of connected rows) give exactly the waveforms of the
+    one-launch-per-level program: each row's sum is the same fixed-order sum."""
+    net = G.gate_dag(300, seed=13)
+    eng_a, a = run_gpu(net, 0.05e-9, 1e-12, precision=precision, band_width=16)
+    eng_b, b = run_gpu(net, 0.05e-9, 1e-12, precision=precision, band_width=0)
+    ia, ib = eng_a.device.lu.info(), eng_b.device.lu.info()
+    assert ia["fwd_bands"] > 0 and ib["fwd_bands"] == 0
+    assert ia["fwd_bands"] < ib["fwd_levels"]
+    assert np.array_equal(a.times, b.times)
+    assert np.array_equal(a.stats["V_out"], b.stats["V_out"])

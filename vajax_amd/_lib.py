@@ -37,6 +37,8 @@ class LuInfo(C.Structure):
         ("min_pivot_ratio", C.c_double),
         ("n_blocks", C.c_int32),
         ("n_block_patterns", C.c_int32),
+        ("fwd_bands", C.c_int32), ("bwd_bands", C.c_int32),
+        ("fwd_hub_rows", C.c_int32), ("bwd_hub_rows", C.c_int32),
     ]
 
     def as_dict(self):
@@ -119,6 +121,8 @@ _PROTOS = {
     "vjx_lu_get_info": (C.c_int, [vp, C.POINTER(LuInfo)]),
     "vjx_lu_get_perm": (C.c_int, [vp, i32p, i32p]),
     "vjx_lu_get_pattern": (C.c_int, [vp, i32p, i32p]),
+    "vjx_lu_check_schedules": (C.c_int, [vp]),
+    "vjx_pattern_set_band_width": (C.c_int, [vp, C.c_int32]),
     "vjx_lu_refactor_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p]),
     "vjx_lu_solve_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p, f64p]),
     "vjx_lu_download": (C.c_int, [vp, f64p]),

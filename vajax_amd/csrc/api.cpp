@@ -111,6 +111,10 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->min_pivot_ratio = s.min_pivot_ratio;
   o->n_blocks = s.n_blocks;
   o->n_block_patterns = s.n_block_patterns;
+  o->fwd_bands = (int32_t)s.fwd_band.band_w.size();
+  o->bwd_bands = (int32_t)s.bwd_band.band_w.size();
+  o->fwd_hub_rows = (int32_t)s.fwd_band.hub_rows.size();
+  o->bwd_hub_rows = (int32_t)s.bwd_band.hub_rows.size();
   return VJX_OK;
 }
 
@@ -121,6 +125,35 @@ int vjx_lu_get_perm(const vjx_pattern* pat, int32_t* p, int32_t* q) {
   }
   if (p) std::memcpy(p, pat->lu.sym.p.data(), sizeof(int32_t) * pat->n);
   if (q) std::memcpy(q, pat->lu.sym.q.data(), sizeof(int32_t) * pat->n);
+  return VJX_OK;
+}
+
+int vjx_lu_check_schedules(const vjx_pattern* pat) {
+  if (!pat || !pat->lu.analyzed) {
+    set_error("vjx_lu_check_schedules: no analysis yet");
+    return VJX_ERR_STATE;
+  }
+  std::string err;
+  if (!lu_check_schedules(pat->lu.sym, err)) {
+    set_error("vjx_lu_check_schedules: " + err);
+    return VJX_ERR_STATE;
+  }
+  return VJX_OK;
+}
+
+int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width) {
+  if (!pat || width < 0 || width > kMaxBandWidth) {
+    set_error("vjx_pattern_set_band_width: width must be in [0, " +
+              std::to_string(kMaxBandWidth) + "]");
+    return VJX_ERR_ARG;
+  }
+  if (pat->opts.band_width != width) {
+    pat->opts.band_width = width;
+    // the schedules change: analyse again on next use
+    pat->lu.analyzed = false;
+    pat->lu.factored_prec = -1;
+    pat->uploaded = false;
+  }
   return VJX_OK;
 }
 

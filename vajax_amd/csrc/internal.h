@@ -93,12 +93,20 @@ struct DevSchedule {
   std::vector<LaunchGroup> groups;
 };
 
+// Band program of a triangular solve on the device (host plan in sym).
+struct DevBand {
+  DBuf<int32_t> items, tl_ptr, tl_heavy, tl_mid, hub_rows, hub_cptr, hub_cs, hub_ce;
+  DBuf<double> part;  // hub chunk partials (FP32 solves use it as float storage)
+  bool on = false;
+};
+
 struct DevLU {
   LuSymbolic sym;
   bool analyzed = false;
   int factored_prec = -1;  // precision of the current device factors
   DBuf<int32_t> rowptr, col, diag, src, tgt_pos, tgt_seg, upd_l, upd_u, upd_d, l_slot, l_diag, p, q;
   DevSchedule fact, fwd, bwd;
+  DevBand fwd_band, bwd_band;
   DBuf<double> lu64;
   DBuf<float> lu32;
   DBuf<double> y64, z64, r64, x64;
@@ -111,7 +119,6 @@ struct DevLU {
 // are summed in kHubChunk-element chunks by one workgroup each; the consuming
 // kernel then adds a hub's chunk partials in chunk order, so the result is
 // fixed by the plan and not by scheduling.
-constexpr int kHubChunk = 4096;
 
 struct HubChunks {
   const int32_t* item = nullptr;  // hub item (slot or row) ids

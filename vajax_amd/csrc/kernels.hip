@@ -740,6 +740,73 @@ __global__ void __launch_bounds__(512) k_tri_narrow(int l0, int l1, const int32_
   }
 }
 
+// One band of a triangular solve: workgroup = task, walking the band's w
+// levels of its own rows (connected components packed by the host) with a
+// barrier between levels.  Rows of earlier bands are final at launch.
+struct BandProg {
+  const int32_t* items;
+  const int32_t* tl_ptr;
+  const int32_t* tl_heavy;
+  const int32_t* tl_mid;
+};
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(256) k_tri_band(int w, int ptr_off, int hm_off, BandProg B,
+                                                  TriProg P, const T* __restrict__ lu, T* y, T* z) {
+  __shared__ T red[RB];
+  __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
+  const int t = blockIdx.x;
+  const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
+  const int32_t* th = B.tl_heavy + hm_off + (size_t)t * w;
+  const int32_t* tm = B.tl_mid + hm_off + (size_t)t * w;
+  for (int j = threadIdx.x; j <= w; j += blockDim.x) s_ptr[j] = tp[j];
+  for (int j = threadIdx.x; j < w; j += blockDim.x) {
+    s_h[j] = th[j];
+    s_m[j] = tm[j];
+  }
+  __syncthreads();
+  const int lane = (int)threadIdx.x % kGroupLanes;
+  const int ngrp = (int)blockDim.x / kGroupLanes;
+  for (int j = 0; j < w; ++j) {
+    const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
+    if (lo == hi) continue;  // uniform across the block
+    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x)
+      tri_light<T, FWD>(B.items[g], P, lu, y, z);
+    for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp)
+      tri_mid<T, FWD>(B.items[lo + nh + grp], lane, P, lu, y, z);
+    for (int h = 0; h < nh; ++h) tri_heavy<T, FWD>(B.items[lo + h], P, lu, y, z, red);
+    __syncthreads();
+  }
+}
+
+// Rows longer than kHubChunk: chunk partials over the grid, then one lane per
+// row adds them in chunk order.
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(256) k_tri_hub_part(const int32_t* __restrict__ cs,
+                                                      const int32_t* __restrict__ ce, TriProg P,
+                                                      const T* __restrict__ lu, const T* y,
+                                                      const T* z, T* part) {
+  __shared__ T red[RB];
+  const int c = blockIdx.x;
+  const T s = block_sum<T>(cs[c], ce[c], TriTerm<T>{lu, FWD ? y : z, P.col}, red);
+  if (threadIdx.x == 0) part[c] = s;
+}
+
+template <typename T, bool FWD>
+__global__ void k_tri_hub_finish(int h0, int h1, const int32_t* __restrict__ rows,
+                                 const int32_t* __restrict__ cptr, const T* __restrict__ part,
+                                 TriProg P, const T* __restrict__ lu, T* y, T* z) {
+  const int h = h0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= h1) return;
+  const int i = rows[h];
+  T acc = (T)0;
+  for (int c = cptr[h]; c < cptr[h + 1]; ++c) acc += part[c];
+  if (FWD)
+    y[i] = y[i] - acc;
+  else
+    z[i] = (y[i] - acc) / lu[P.dg[i]];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_solve_finish(int n, const int32_t* __restrict__ q,
                                                       const T* __restrict__ z, double* x) {
@@ -937,14 +1004,42 @@ static void run_tri_groups(DevLU& L, const DevSchedule& S, const T* lu, T* y, T*
   }
 }
 
+template <typename T, bool FWD>
+static void run_tri_bands(DevLU& L, const BandSchedule& H, DevBand& D, const T* lu, T* y, T* z,
+                          hipStream_t s) {
+  const TriProg P{nullptr, L.rowptr.p, L.col.p, L.diag.p};
+  const BandProg B{D.items.p, D.tl_ptr.p, D.tl_heavy.p, D.tl_mid.p};
+  T* part = reinterpret_cast<T*>(D.part.p);
+  for (size_t b = 0; b + 1 < H.band_task.size(); ++b) {
+    const int ntask = H.band_task[b + 1] - H.band_task[b];
+    if (ntask > 0)
+      k_tri_band<T, FWD><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b],
+                                               B, P, lu, y, z);
+    const int h0 = H.band_hub[b], h1 = H.band_hub[b + 1];
+    if (h1 > h0) {
+      const int c0 = H.hub_cptr[h0], c1 = H.hub_cptr[h1];
+      k_tri_hub_part<T, FWD><<<c1 - c0, 256, 0, s>>>(D.hub_cs.p + c0, D.hub_ce.p + c0, P, lu, y, z,
+                                                      part + c0);
+      k_tri_hub_finish<T, FWD><<<nblk(h1 - h0, 64), 64, 0, s>>>(h0, h1, D.hub_rows.p,
+                                                                 D.hub_cptr.p, part, P, lu, y, z);
+    }
+  }
+}
+
 template <typename T>
 void launch_lu_solve(DevLU& L, const T* lu, const double* rhs, double scale, double* x, T* y, T* z,
                      hipStream_t s) {
   const int n = L.sym.n;
   if (n == 0) return;
   k_solve_init<T><<<nblk(n, 256), 256, 0, s>>>(n, L.p.p, rhs, scale, y);
-  run_tri_groups<T, true>(L, L.fwd, lu, y, z, s);
-  run_tri_groups<T, false>(L, L.bwd, lu, y, z, s);
+  if (L.fwd_band.on)
+    run_tri_bands<T, true>(L, L.sym.fwd_band, L.fwd_band, lu, y, z, s);
+  else
+    run_tri_groups<T, true>(L, L.fwd, lu, y, z, s);
+  if (L.bwd_band.on)
+    run_tri_bands<T, false>(L, L.sym.bwd_band, L.bwd_band, lu, y, z, s);
+  else
+    run_tri_groups<T, false>(L, L.bwd, lu, y, z, s);
   k_solve_finish<T><<<nblk(n, 256), 256, 0, s>>>(n, L.q.p, z, x);
 }
 

@@ -67,6 +67,21 @@ static hipError_t upload_sched(DevSchedule& d, const LevelSchedule& h) {
   return e;
 }
 
+static hipError_t upload_band(DevBand& d, const BandSchedule& h) {
+  d.on = !h.band_w.empty();
+  if (!d.on) return hipSuccess;
+  hipError_t e = d.items.upload(h.items);
+  if (e == hipSuccess) e = d.tl_ptr.upload(h.tl_ptr);
+  if (e == hipSuccess) e = d.tl_heavy.upload(h.tl_heavy);
+  if (e == hipSuccess) e = d.tl_mid.upload(h.tl_mid);
+  if (e == hipSuccess) e = d.hub_rows.upload(h.hub_rows);
+  if (e == hipSuccess) e = d.hub_cptr.upload(h.hub_cptr);
+  if (e == hipSuccess) e = d.hub_cs.upload(h.hub_cs);
+  if (e == hipSuccess) e = d.hub_ce.upload(h.hub_ce);
+  if (e == hipSuccess) e = d.part.alloc(h.hub_cs.size());
+  return e;
+}
+
 int pattern_upload(vjx_pattern* pat) {
   if (pat->uploaded) return VJX_OK;
   int rc = pattern_upload_csr(pat);
@@ -89,6 +104,8 @@ int pattern_upload(vjx_pattern* pat) {
   VJX_TRY_HIP(upload_sched(L.fact, s.fact));
   VJX_TRY_HIP(upload_sched(L.fwd, s.fwd));
   VJX_TRY_HIP(upload_sched(L.bwd, s.bwd));
+  VJX_TRY_HIP(upload_band(L.fwd_band, s.fwd_band));
+  VJX_TRY_HIP(upload_band(L.bwd_band, s.bwd_band));
   const size_t nnz = s.col.size();
   const size_t n = (size_t)s.n;
   VJX_TRY_HIP(L.lu64.alloc(nnz));

@@ -657,7 +657,228 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
       s.l_slot.push_back(e);
       s.l_diag.push_back(s.diag[s.col[e]]);
     }
-  // solve level sets
+  make_groups(s.fact, opt.wide_threshold);
+  return true;
+}
+
+// ---- band programs of the triangular solves ----------------------------------
+
+// level: per row; seg(i) -> (begin, end) of row i's off-diagonal segment in the
+// LU arrays; deps(i, f) calls f(j) for every row j that row i reads.
+template <typename Seg, typename Deps>
+static void build_bands(const std::vector<int32_t>& level, Seg seg, Deps deps, int width,
+                        int max_tasks, BandSchedule& B) {
+  B = BandSchedule();
+  const int n = (int)level.size();
+  if (n == 0 || width <= 0) return;
+  width = std::min(width, kMaxBandWidth);
+  int nl = 0;
+  for (int v : level) nl = std::max(nl, v + 1);
+  std::vector<int32_t> lptr(nl + 1, 0), lrows(n);
+  for (int v : level) lptr[v + 1]++;
+  for (int l = 0; l < nl; ++l) lptr[l + 1] += lptr[l];
+  {
+    std::vector<int32_t> fill(lptr.begin(), lptr.end() - 1);
+    for (int i = 0; i < n; ++i) lrows[fill[level[i]]++] = i;
+  }
+  auto seglen = [&](int i) {
+    auto r = seg(i);
+    return r.second - r.first;
+  };
+  std::vector<char> has_hub(nl, 0);
+  for (int i = 0; i < n; ++i)
+    if (seglen(i) > kHubChunk) has_hub[level[i]] = 1;
+  std::vector<int32_t> parent(n, -1), comp_id(n, -1);
+  auto root = [&](int v) {
+    while (parent[v] != v) {
+      parent[v] = parent[parent[v]];
+      v = parent[v];
+    }
+    return v;
+  };
+  B.band_task.assign(1, 0);
+  B.band_hub.assign(1, 0);
+  B.hub_cptr.assign(1, 0);
+  std::vector<int32_t> rows, comps, task_of_comp;
+  std::vector<int64_t> comp_work;
+  for (int b0 = 0; b0 < nl;) {
+    int b1 = std::min(nl, b0 + width);
+    for (int l = b0; l < b1; ++l)
+      if (has_hub[l]) {
+        b1 = l + 1;
+        break;
+      }
+    const int w = b1 - b0;
+    // components of the band's non-hub rows
+    rows.assign(lrows.begin() + lptr[b0], lrows.begin() + lptr[b1]);
+    for (int i : rows) parent[i] = i;
+    for (int i : rows) {
+      if (seglen(i) > kHubChunk) continue;
+      deps(i, [&](int j) {
+        if (level[j] >= b0 && seglen(j) <= kHubChunk) {
+          const int a = root(i), c = root(j);
+          if (a != c) parent[std::max(a, c)] = std::min(a, c);
+        }
+      });
+    }
+    comps.clear();
+    comp_work.clear();
+    for (int i : rows) {
+      if (seglen(i) > kHubChunk) continue;
+      const int r = root(i);
+      if (comp_id[r] < 0 || comp_id[r] >= (int)comps.size() || comps[comp_id[r]] != r) {
+        comp_id[r] = (int32_t)comps.size();
+        comps.push_back(r);
+        comp_work.push_back(0);
+      }
+      comp_work[comp_id[r]] += seglen(i) + 8;
+    }
+    // pack components into tasks: longest first onto the least loaded task
+    const int nc = (int)comps.size();
+    const int ntask = std::min(nc, std::max(1, max_tasks));
+    std::vector<int32_t> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return comp_work[a] > comp_work[c]; });
+    task_of_comp.assign(nc, 0);
+    {
+      std::vector<std::pair<int64_t, int32_t>> heap;  // (load, task) min-heap
+      for (int t = 0; t < ntask; ++t) heap.push_back({0, t});
+      auto cmp = [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& c) {
+        return a > c;
+      };
+      std::make_heap(heap.begin(), heap.end(), cmp);
+      for (int k : order) {
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        auto& top = heap.back();
+        task_of_comp[k] = top.second;
+        top.first += comp_work[k];
+        std::push_heap(heap.begin(), heap.end(), cmp);
+      }
+    }
+    // per task, per level, per class: rows in ascending order
+    std::vector<std::vector<int32_t>> bucket((size_t)ntask * w * 3);
+    std::vector<int32_t> sorted_rows = rows;
+    std::sort(sorted_rows.begin(), sorted_rows.end());
+    for (int i : sorted_rows) {
+      if (seglen(i) > kHubChunk) continue;
+      const int t = task_of_comp[comp_id[root(i)]];
+      const int cls = reduction_class(seglen(i));
+      bucket[((size_t)t * w + (level[i] - b0)) * 3 + cls].push_back(i);
+    }
+    B.band_w.push_back(w);
+    B.band_ptr_off.push_back((int32_t)B.tl_ptr.size());
+    B.band_hm_off.push_back((int32_t)B.tl_heavy.size());
+    for (int t = 0; t < ntask; ++t) {
+      for (int j = 0; j < w; ++j) {
+        B.tl_ptr.push_back((int32_t)B.items.size());
+        const size_t k = ((size_t)t * w + j) * 3;
+        for (int cls = 2; cls >= 0; --cls)
+          B.items.insert(B.items.end(), bucket[k + cls].begin(), bucket[k + cls].end());
+        B.tl_heavy.push_back((int32_t)bucket[k + 2].size());
+        B.tl_mid.push_back((int32_t)bucket[k + 1].size());
+      }
+      B.tl_ptr.push_back((int32_t)B.items.size());
+    }
+    B.band_task.push_back(B.band_task.back() + ntask);
+    B.max_tasks = std::max(B.max_tasks, ntask);
+    // hub rows of this band, finished after its tasks
+    for (int i : sorted_rows) {
+      if (seglen(i) <= kHubChunk) continue;
+      B.hub_rows.push_back(i);
+      auto r = seg(i);
+      for (int c = r.first; c < r.second; c += kHubChunk) {
+        B.hub_cs.push_back(c);
+        B.hub_ce.push_back(std::min(r.second, c + kHubChunk));
+      }
+      B.hub_cptr.push_back((int32_t)B.hub_cs.size());
+    }
+    B.band_hub.push_back((int32_t)B.hub_rows.size());
+    for (int i : rows) comp_id[root(i)] = -1;
+    b0 = b1;
+  }
+}
+
+// Checks that a band program runs every row once and only after the rows it
+// reads: earlier bands, or the same task at an earlier level; a hub row after
+// every task row of its band.
+template <typename Deps>
+static bool check_bands(int n, const BandSchedule& B, Deps deps, std::string& err) {
+  std::vector<int32_t> band(n, -1), task(n, -1), lvl(n, -1);
+  std::vector<char> hub(n, 0);
+  const int nb = (int)B.band_w.size();
+  for (int b = 0; b < nb; ++b) {
+    const int w = B.band_w[b];
+    for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
+      const int tl = t - B.band_task[b];
+      const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)tl * (w + 1)];
+      for (int j = 0; j < w; ++j)
+        for (int g = tp[j]; g < tp[j + 1]; ++g) {
+          const int i = B.items[g];
+          if (band[i] >= 0) {
+            err = "row scheduled twice";
+            return false;
+          }
+          band[i] = b;
+          task[i] = t;
+          lvl[i] = j;
+        }
+    }
+    for (int h = B.band_hub[b]; h < B.band_hub[b + 1]; ++h) {
+      const int i = B.hub_rows[h];
+      if (band[i] >= 0) {
+        err = "hub row scheduled twice";
+        return false;
+      }
+      band[i] = b;
+      hub[i] = 1;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    if (band[i] < 0) {
+      err = "row " + std::to_string(i) + " not scheduled";
+      return false;
+    }
+    bool ok = true;
+    deps(i, [&](int j) {
+      if (band[j] < band[i]) return;
+      if (band[j] > band[i] || hub[j]) {
+        ok = false;
+        return;
+      }
+      if (hub[i]) return;  // task rows of the same band run before its hubs
+      if (task[j] != task[i] || lvl[j] >= lvl[i]) ok = false;
+    });
+    if (!ok) {
+      err = "row " + std::to_string(i) + " runs before a row it reads";
+      return false;
+    }
+  }
+  return true;
+}
+
+bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
+  const int n = s.n;
+  auto fdeps = [&](int i, auto f) {
+    for (int e = s.rowptr[i]; e < s.diag[i]; ++e) f(s.col[e]);
+  };
+  auto bdeps = [&](int i, auto f) {
+    for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) f(s.col[e]);
+  };
+  if (!s.fwd_band.band_w.empty() && !check_bands(n, s.fwd_band, fdeps, err)) {
+    err = "forward band program: " + err;
+    return false;
+  }
+  if (!s.bwd_band.band_w.empty() && !check_bands(n, s.bwd_band, bdeps, err)) {
+    err = "backward band program: " + err;
+    return false;
+  }
+  return true;
+}
+
+// Level sets and band programs of both solves from the LU pattern.
+static void solve_schedules(LuSymbolic& s, const AnalyzeOptions& opt) {
+  const int n = s.n;
   std::vector<int32_t> lf(n, 0), lb(n, 0);
   for (int i = 0; i < n; ++i) {
     int lv = 0;
@@ -671,10 +892,18 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
   }
   rows_by_level(lf, [&](int i) { return s.diag[i] - s.rowptr[i]; }, s.fwd);
   rows_by_level(lb, [&](int i) { return s.rowptr[i + 1] - s.diag[i] - 1; }, s.bwd);
-  make_groups(s.fact, opt.wide_threshold);
-  make_groups(s.fwd, opt.wide_threshold);
-  make_groups(s.bwd, opt.wide_threshold);
-  return true;
+  build_bands(
+      lf, [&](int i) { return std::make_pair((int)s.rowptr[i], (int)s.diag[i]); },
+      [&](int i, auto f) {
+        for (int e = s.rowptr[i]; e < s.diag[i]; ++e) f(s.col[e]);
+      },
+      opt.band_width, opt.max_band_tasks, s.fwd_band);
+  build_bands(
+      lb, [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); },
+      [&](int i, auto f) {
+        for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) f(s.col[e]);
+      },
+      opt.band_width, opt.max_band_tasks, s.bwd_band);
 }
 
 // ---- block-diagonal systems --------------------------------------------------
@@ -784,6 +1013,9 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     if (!analyze_single(n, indptr, indices, vals, opt, s, err)) return false;
     s.n_blocks = 1;
     s.n_block_patterns = 1;
+    solve_schedules(s, opt);
+    make_groups(s.fwd, opt.wide_threshold);
+    make_groups(s.bwd, opt.wide_threshold);
     return true;
   }
   // blocks and their local patterns
@@ -851,7 +1083,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
   s.l_slot.reserve(n_l);
   s.l_diag.reserve(n_l);
   s.min_pivot_ratio = 1.0;
-  std::vector<std::vector<int32_t>> bf[3], bfw[3], bbw[3];
+  std::vector<std::vector<int32_t>> bf[3];
   int32_t roff = 0, soff = 0, toff = 0, uoff = 0;
   std::vector<int32_t> gmap;
   for (int c = 0; c < nc; ++c) {
@@ -889,8 +1121,6 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
       s.l_diag.push_back(b.l_diag[e] + soff);
     }
     bucket_schedule(b.fact, toff, bf);
-    bucket_schedule(b.fwd, roff, bfw);
-    bucket_schedule(b.bwd, roff, bbw);
     s.nnz_l += b.nnz_l;
     s.nnz_u += b.nnz_u;
     s.n_updates += b.n_updates;
@@ -910,8 +1140,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     s.pinv[s.p[k]] = k;
   }
   merge_schedule(bf, s.fact);
-  merge_schedule(bfw, s.fwd);
-  merge_schedule(bbw, s.bwd);
+  solve_schedules(s, opt);
   make_groups(s.fact, opt.wide_threshold);
   make_groups(s.fwd, opt.wide_threshold);
   make_groups(s.bwd, opt.wide_threshold);
@@ -944,6 +1173,18 @@ static T class_sum(int cls, int start, int end, F term) {
   return acc;
 }
 
+// Sum of one triangular-solve row: by length class, or for rows longer than
+// kHubChunk under band programs, chunk sums (workgroup class) added in order.
+template <typename T, typename F>
+static T row_sum(int start, int end, bool chunk_hubs, F term) {
+  const int len = end - start;
+  if (!chunk_hubs || len <= kHubChunk) return class_sum<T>(reduction_class(len), start, end, term);
+  T acc = (T)0;
+  for (int c = start; c < end; c += kHubChunk)
+    acc += class_sum<T>(2, c, std::min(end, c + kHubChunk), term);
+  return acc;
+}
+
 static int item_class(const LevelSchedule& S, int l, int it) {
   const int lo = S.lvl_ptr[l];
   if (it < lo + S.lvl_heavy[l]) return 2;
@@ -971,20 +1212,20 @@ void lu_solve_host(const LuSymbolic& s, const T* lu, const double* b, double* x)
   const int n = s.n;
   std::vector<T> y(n), z(n);
   for (int i = 0; i < n; ++i) y[i] = (T)b[s.p[i]];
-  for (int l = 0; l + 1 < (int)s.fwd.lvl_ptr.size(); ++l)
-    for (int it = s.fwd.lvl_ptr[l]; it < s.fwd.lvl_ptr[l + 1]; ++it) {
-      const int i = s.fwd.items[it];
-      auto term = [&](int e) { return lu[e] * y[s.col[e]]; };
-      const T acc = class_sum<T>(item_class(s.fwd, l, it), s.rowptr[i], s.diag[i], term);
-      y[i] = y[i] - acc;
-    }
-  for (int l = 0; l + 1 < (int)s.bwd.lvl_ptr.size(); ++l)
-    for (int it = s.bwd.lvl_ptr[l]; it < s.bwd.lvl_ptr[l + 1]; ++it) {
-      const int i = s.bwd.items[it];
-      auto term = [&](int e) { return lu[e] * z[s.col[e]]; };
-      const T acc = class_sum<T>(item_class(s.bwd, l, it), s.diag[i] + 1, s.rowptr[i + 1], term);
-      z[i] = (y[i] - acc) / lu[s.diag[i]];
-    }
+  // Each row's sum depends only on its length class (and, under band
+  // programs, on the chunking of rows longer than kHubChunk), not on the
+  // order rows are visited, so rows are taken in dependency order here.
+  const bool chunk_f = !s.fwd_band.band_w.empty(), chunk_b = !s.bwd_band.band_w.empty();
+  for (int i = 0; i < n; ++i) {
+    auto term = [&](int e) { return lu[e] * y[s.col[e]]; };
+    const T acc = row_sum<T>(s.rowptr[i], s.diag[i], chunk_f, term);
+    y[i] = y[i] - acc;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    auto term = [&](int e) { return lu[e] * z[s.col[e]]; };
+    const T acc = row_sum<T>(s.diag[i] + 1, s.rowptr[i + 1], chunk_b, term);
+    z[i] = (y[i] - acc) / lu[s.diag[i]];
+  }
   for (int i = 0; i < n; ++i) x[s.q[i]] = (double)z[i];
 }
 

@@ -62,6 +62,35 @@ inline int reduction_class(int len) {
   return len > kHeavySegment ? 2 : (len > kShortSegment ? 1 : 0);
 }
 
+// Segments longer than kHubChunk (supply rails, dense LU rows) are summed in
+// kHubChunk-entry chunks, one workgroup per chunk (fixed-order workgroup sum),
+// and the chunk partials are then added in chunk order.
+constexpr int kHubChunk = 4096;
+constexpr int kMaxBandWidth = 64;  // levels per band (LDS table of one task)
+
+// Band program of a triangular solve.  The levels are cut into bands of at
+// most `width` levels; inside a band the rows form connected components
+// (a row depends only on rows at lower levels, and on rows of the same band
+// only through edges inside its component), which are packed into tasks.
+// One launch per band runs one workgroup per task; the workgroup walks the
+// band's levels with a barrier between them.  Rows longer than kHubChunk are
+// taken out of the tasks and finished after them by a chunked grid
+// reduction (a band that holds one ends at that row's level).
+struct BandSchedule {
+  std::vector<int32_t> items;                // rows by band, task, level, class
+  std::vector<int32_t> band_task;            // band b: tasks [band_task[b], band_task[b+1])
+  std::vector<int32_t> band_w;               // levels in band b
+  std::vector<int32_t> band_ptr_off;         // task t of band b: tl_ptr[band_ptr_off[b] + (t-t0)*(w+1) ...]
+  std::vector<int32_t> band_hm_off;          //                  tl_heavy/mid[band_hm_off[b] + (t-t0)*w ...]
+  std::vector<int32_t> tl_ptr;               // per task: w+1 item offsets
+  std::vector<int32_t> tl_heavy, tl_mid;     // per task and level: workgroup / lane-group rows
+  std::vector<int32_t> band_hub;             // band b: hub rows [band_hub[b], band_hub[b+1])
+  std::vector<int32_t> hub_rows;
+  std::vector<int32_t> hub_cptr;             // hub h: chunks [hub_cptr[h], hub_cptr[h+1])
+  std::vector<int32_t> hub_cs, hub_ce;       // chunk c covers LU entries [hub_cs[c], hub_ce[c])
+  int max_tasks = 0;
+};
+
 struct LuSymbolic {
   int n = 0;
   int nnz_a = 0;
@@ -81,6 +110,8 @@ struct LuSymbolic {
   std::vector<int32_t> l_diag;   // ... and the slot of its pivot U(k,k)
   LevelSchedule fwd;             // items = rows, L solve
   LevelSchedule bwd;             // items = rows, U solve
+  BandSchedule fwd_band;         // band programs of the two solves (empty: level launches)
+  BandSchedule bwd_band;
   // statistics
   int64_t nnz_l = 0, nnz_u = 0, n_updates = 0;
   double flops = 0.0;
@@ -96,6 +127,8 @@ struct AnalyzeOptions {
   int wide_threshold = 2048;  // levels with at least this many items get a grid
   int ordering = 2;           // 0 = minimum degree, 1 = natural, 2 = nested dissection
   int nd_leaf = 48;           // nested-dissection leaf size (min-degree inside)
+  int band_width = 16;        // solve bands of this many levels (0: one launch per level)
+  int max_band_tasks = 2048;  // workgroups per band launch
 };
 
 // Full analysis from a CSR matrix (host arrays).  Returns false and sets err on
@@ -105,6 +138,9 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
 
 // CSR structure validation as in vajax/sparse/umfpack_ffi.cpp:234-251.
 bool csr_validate(int n, int nnz, const int32_t* indptr, const int32_t* indices, std::string& err);
+
+// Dependency check of the band programs (every row once, after what it reads).
+bool lu_check_schedules(const LuSymbolic& s, std::string& err);
 
 // CPU execution of the same numeric program (reference for the GPU kernels).
 template <typename T>

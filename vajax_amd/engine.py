@@ -296,9 +296,14 @@ class CircuitEngine:
     def prepare(self, *, t_stop: Optional[float] = None, dt: Optional[float] = None,
                 adaptive_config: Optional[AdaptiveConfig] = None, precision: str = "f64",
                 refine_steps: Optional[int] = None, max_steps: Optional[int] = None,
-                use_graph: bool = False, **_ignored) -> None:
+                use_graph: bool = False, band_width: Optional[int] = None,
+                **_ignored) -> None:
         if self.topology is None:
             self.parse()
+        if band_width is not None:
+            # launch plan of the triangular solves (levels per band; 0 = one
+            # launch per level); a change re-runs the symbolic analysis
+            self.device.lu.set_band_width(band_width)
         if t_stop is None:
             t_stop = float(self.analysis_params.get("stop", 1e-3))
         if dt is None:

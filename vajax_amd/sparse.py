@@ -57,6 +57,14 @@ class SparseLU:
         d = _f64(data)
         L.check(L.lib.vjx_lu_analyze(self.h, L.ptr(d, C.c_double)), "vjx_lu_analyze")
 
+    def set_band_width(self, width: int) -> None:
+        """Levels per triangular-solve band (0: one launch per level); applies
+        from the next analysis."""
+        L.check(L.lib.vjx_pattern_set_band_width(self.h, int(width)), "vjx_pattern_set_band_width")
+
+    def check_schedules(self) -> None:
+        L.check(L.lib.vjx_lu_check_schedules(self.h), "vjx_lu_check_schedules")
+
     def info(self) -> dict:
         info = L.LuInfo()
         L.check(L.lib.vjx_lu_get_info(self.h, C.byref(info)), "vjx_lu_get_info")
