@@ -186,3 +186,34 @@ def test_band_program_with_hub_row():
     b = np.random.default_rng(8).standard_normal(cir.n_aug)
     x = lu.solve_cpu(csr, b)
     assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("width", [1, 3, 16])
+def test_band_refactorisation_program(width):
+    # refactorisation in bands: each update reads only final slots and no two
+    # tasks of one launch write the same slot; L*U reproduces P*A*Q
+    net = G.gate_dag(400, seed=21)
+    cir, csr, A = _system(net)
+    lu = SparseLU(cir.indptr, cir.indices)
+    lu.set_band_width(width)
+    lu.analyze(csr)
+    lu.check_schedules()
+    info = lu.info()
+    assert info["fact_launches"] <= 2 * ((info["fact_levels"] + width - 1) // width)
+    p, q = lu.perm()
+    rp, col = lu.lu_pattern()
+    vals = lu.refactor_cpu(csr)
+    n = cir.n_aug
+    F = sp.csr_matrix((vals, col, rp), shape=(n, n)).toarray()
+    PAQ = A.toarray()[p][:, q]
+    LU = (np.tril(F, -1) + np.eye(n)) @ np.triu(F)
+    assert np.allclose(LU, PAQ, rtol=0, atol=1e-12 * np.max(np.abs(PAQ)))
+
+
+def test_packed_batch_band_programs_checked():
+    nets = [G.monte_carlo(G.gate_dag(150, seed=4), 10 + b, 0.03) for b in range(3)]
+    nets += [G.gate_dag(60, seed=7), G.gate_dag(90, seed=8)]
+    cir, csr, A = _system(G.pack(nets), seed=3)
+    lu = SparseLU(cir.indptr, cir.indices)
+    lu.analyze(csr)
+    lu.check_schedules()

@@ -100,10 +100,18 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->nnz_u = s.nnz_u;
   o->n_updates = s.n_updates;
   o->flops = s.flops;
-  o->fact_levels = (int32_t)s.fact.lvl_ptr.size() - 1;
+  o->fact_levels = s.n_fact_levels;
   o->fwd_levels = (int32_t)s.fwd.lvl_ptr.size() - 1;
   o->bwd_levels = (int32_t)s.bwd.lvl_ptr.size() - 1;
-  o->fact_launches = (int32_t)s.fact.groups.size();
+  if (!s.fact_band.band_w.empty()) {
+    int nlaunch = 0;
+    for (size_t b = 0; b + 1 < s.fact_band.band_task.size(); ++b)
+      nlaunch += (s.fact_band.band_task[b + 1] > s.fact_band.band_task[b]) +
+                 (s.fact_schur_hi[b] > s.fact_schur_lo[b]);
+    o->fact_launches = nlaunch;
+  } else {
+    o->fact_launches = (int32_t)s.fact.groups.size();
+  }
   o->fwd_launches = (int32_t)s.fwd.groups.size();
   o->bwd_launches = (int32_t)s.bwd.groups.size();
   o->n_dense_rows = s.n_dense;

@@ -106,7 +106,7 @@ struct DevLU {
   int factored_prec = -1;  // precision of the current device factors
   DBuf<int32_t> rowptr, col, diag, src, tgt_pos, tgt_seg, upd_l, upd_u, upd_d, l_slot, l_diag, p, q;
   DevSchedule fact, fwd, bwd;
-  DevBand fwd_band, bwd_band;
+  DevBand fwd_band, bwd_band, fact_band;
   DBuf<double> lu64;
   DBuf<float> lu32;
   DBuf<double> y64, z64, r64, x64;

@@ -516,7 +516,17 @@ __global__ void __launch_bounds__(256) k_lu_gather(int nnz, const int32_t* __res
   lu[e] = s >= 0 ? (T)csr[s] : (T)0;
 }
 
+// Task tables of a band program: task t of a band with w levels has item
+// offsets tl_ptr[t*(w+1) ...] and per-level workgroup / lane-group counts.
+struct BandProg {
+  const int32_t* items;
+  const int32_t* tl_ptr;
+  const int32_t* tl_heavy;
+  const int32_t* tl_mid;
+};
+
 struct LuProg {
+  const int32_t* items;  // item -> target id
   const int32_t* tpos;
   const int32_t* tseg;
   const int32_t* ul;
@@ -564,16 +574,16 @@ __global__ void __launch_bounds__(256) k_lu_wide(int lo, int hi, int nh, int nm,
   const int b = blockIdx.x;
   const int nmb = (nm + kGroupsPerBlock - 1) / kGroupsPerBlock;
   if (b < nh) {
-    lu_heavy<T>(lo + b, P, lu, red);
+    lu_heavy<T>(P.items[lo + b], P, lu, red);
     return;
   }
   if (b < nh + nmb) {
     const int grp = (b - nh) * kGroupsPerBlock + (int)threadIdx.x / kGroupLanes;
-    if (grp < nm) lu_mid<T>(lo + nh + grp, (int)threadIdx.x % kGroupLanes, P, lu);
+    if (grp < nm) lu_mid<T>(P.items[lo + nh + grp], (int)threadIdx.x % kGroupLanes, P, lu);
     return;
   }
   const int g = lo + nh + nm + (b - nh - nmb) * blockDim.x + threadIdx.x;
-  if (g < hi) lu_light<T>(g, P, lu);
+  if (g < hi) lu_light<T>(P.items[g], P, lu);
 }
 
 // A run of narrow levels in one workgroup; level bounds cached in LDS.
@@ -597,10 +607,44 @@ __global__ void __launch_bounds__(512) k_lu_narrow(int l0, int l1, const int32_t
   const int ngrp = (int)blockDim.x / kGroupLanes;
   for (int l = 0; l < nl; ++l) {
     const int lo = s_lv[l], hi = s_lv[l + 1], nh = s_hv[l], nm = s_md[l];
-    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x) lu_light<T>(g, P, lu);
+    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x)
+      lu_light<T>(P.items[g], P, lu);
     for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp)
-      lu_mid<T>(lo + nh + grp, lane, P, lu);
-    for (int h = 0; h < nh; ++h) lu_heavy<T>(lo + h, P, lu, red);
+      lu_mid<T>(P.items[lo + nh + grp], lane, P, lu);
+    for (int h = 0; h < nh; ++h) lu_heavy<T>(P.items[lo + h], P, lu, red);
+    __syncthreads();
+  }
+}
+
+// One band of the refactorisation: workgroup = task (a group of connected
+// pivots of the band), walking the band's levels; its items are the targets
+// owned by the task (rows or columns of its own pivots).  The band's updates
+// of later pivots' rows and columns (the Schur part) follow as one wide launch.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_band(int w, int ptr_off, int hm_off, BandProg B,
+                                                 LuProg P, T* lu) {
+  __shared__ T red[RB];
+  __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
+  const int t = blockIdx.x;
+  const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
+  const int32_t* th = B.tl_heavy + hm_off + (size_t)t * w;
+  const int32_t* tm = B.tl_mid + hm_off + (size_t)t * w;
+  for (int j = threadIdx.x; j <= w; j += blockDim.x) s_ptr[j] = tp[j];
+  for (int j = threadIdx.x; j < w; j += blockDim.x) {
+    s_h[j] = th[j];
+    s_m[j] = tm[j];
+  }
+  __syncthreads();
+  const int lane = (int)threadIdx.x % kGroupLanes;
+  const int ngrp = (int)blockDim.x / kGroupLanes;
+  for (int j = 0; j < w; ++j) {
+    const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
+    if (lo == hi) continue;  // uniform across the block
+    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x)
+      lu_light<T>(B.items[g], P, lu);
+    for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp)
+      lu_mid<T>(B.items[lo + nh + grp], lane, P, lu);
+    for (int h = 0; h < nh; ++h) lu_heavy<T>(B.items[lo + h], P, lu, red);
     __syncthreads();
   }
 }
@@ -743,13 +787,6 @@ __global__ void __launch_bounds__(512) k_tri_narrow(int l0, int l1, const int32_
 // One band of a triangular solve: workgroup = task, walking the band's w
 // levels of its own rows (connected components packed by the host) with a
 // barrier between levels.  Rows of earlier bands are final at launch.
-struct BandProg {
-  const int32_t* items;
-  const int32_t* tl_ptr;
-  const int32_t* tl_heavy;
-  const int32_t* tl_mid;
-};
-
 template <typename T, bool FWD>
 __global__ void __launch_bounds__(256) k_tri_band(int w, int ptr_off, int hm_off, BandProg B,
                                                   TriProg P, const T* __restrict__ lu, T* y, T* z) {
@@ -960,8 +997,33 @@ void launch_residual(const ResidualArgs& a, hipStream_t s) {
 }
 
 template <typename T>
+static void run_fact_bands(DevLU& L, T* lu, hipStream_t s) {
+  const LuSymbolic& S = L.sym;
+  const BandSchedule& H = S.fact_band;
+  const LuProg P{L.fact_band.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
+  const BandProg B{L.fact_band.items.p, L.fact_band.tl_ptr.p, L.fact_band.tl_heavy.p,
+                   L.fact_band.tl_mid.p};
+  for (size_t b = 0; b + 1 < H.band_task.size(); ++b) {
+    const int ntask = H.band_task[b + 1] - H.band_task[b];
+    if (ntask > 0)
+      k_lu_band<T><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], B, P,
+                                         lu);
+    const int lo = S.fact_schur_lo[b], hi = S.fact_schur_hi[b];
+    if (hi > lo) {
+      const int nh = S.fact_schur_heavy[b], nm = S.fact_schur_mid[b];
+      const int grid = nh + nblk(nm, kGroupsPerBlock) + nblk(hi - lo - nh - nm, 256);
+      k_lu_wide<T><<<grid, 256, 0, s>>>(lo, hi, nh, nm, P, lu);
+    }
+  }
+}
+
+template <typename T>
 static void run_fact_groups(DevLU& L, T* lu, hipStream_t s) {
-  const LuProg P{L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
+  if (L.fact_band.on) {
+    run_fact_bands<T>(L, lu, s);
+    return;
+  }
+  const LuProg P{L.fact.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
   for (const auto& g : L.fact.groups) {
     if (g.wide) {
       const int lo = L.fact.lvl_ptr[g.lvl_begin], hi = L.fact.lvl_ptr[g.lvl_end];

@@ -106,6 +106,7 @@ int pattern_upload(vjx_pattern* pat) {
   VJX_TRY_HIP(upload_sched(L.bwd, s.bwd));
   VJX_TRY_HIP(upload_band(L.fwd_band, s.fwd_band));
   VJX_TRY_HIP(upload_band(L.bwd_band, s.bwd_band));
+  VJX_TRY_HIP(upload_band(L.fact_band, s.fact_band));
   const size_t nnz = s.col.size();
   const size_t n = (size_t)s.n;
   VJX_TRY_HIP(L.lu64.alloc(nnz));

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <numeric>
 #include <set>
+#include <array>
 #include <tuple>
 
 namespace vjx {
@@ -582,6 +583,7 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
     level[k] = lv;
     nlev = std::max(nlev, lv + 1);
   }
+  s.n_fact_levels = nlev;
   // update contributions grouped by level, sorted by (target slot, pivot)
   std::vector<std::vector<int32_t>> by_level(nlev);
   for (int k = 0; k < n; ++k) by_level[level[k]].push_back(k);
@@ -593,8 +595,188 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
   s.fact.lvl_ptr.assign(1, 0);
   s.fact.items.clear();
   s.flops = 0.0;
+  if (opt.band_width > 0) {
+    // ---- band program ----
+    const int W = std::min(opt.band_width, kMaxBandWidth);
+    BandSchedule& B = s.fact_band;
+    B = BandSchedule();
+    B.band_task.assign(1, 0);
+    B.band_hub.assign(1, 0);
+    B.hub_cptr.assign(1, 0);
+    s.fact_schur_lo.clear();
+    s.fact_schur_hi.clear();
+    s.fact_schur_heavy.clear();
+    s.fact_schur_mid.clear();
+    std::vector<int32_t> parent(n), comp_id(n, -1);
+    auto root = [&](int v) {
+      while (parent[v] != v) {
+        parent[v] = parent[parent[v]];
+        v = parent[v];
+      }
+      return v;
+    };
+    auto first_l = [&](int k) {  // first CSC entry of column k strictly below the diagonal
+      int d0 = ccnt[k];
+      while (d0 < ccnt[k + 1] && crow[d0] <= k) ++d0;
+      return d0;
+    };
+    // owned: (task, level in band, target, pivot, L slot, U slot); schur: (target, pivot, L, U)
+    std::vector<std::array<int32_t, 6>> own;
+    std::vector<std::array<int32_t, 4>> sch;
+    std::vector<int32_t> piv, comps, task_of;
+    std::vector<int64_t> cwork;
+    for (int b0 = 0; b0 < nlev; b0 += W) {
+      const int b1 = std::min(nlev, b0 + W), w = b1 - b0;
+      piv.clear();
+      for (int l = b0; l < b1; ++l) piv.insert(piv.end(), by_level[l].begin(), by_level[l].end());
+      for (int k : piv) parent[k] = k;
+      for (int k : piv) {
+        auto unite = [&](int a) {
+          if (level[a] < b0) return;
+          const int x = root(k), y = root(a);
+          if (x != y) parent[std::max(x, y)] = std::min(x, y);
+        };
+        for (int e = s.rowptr[k]; e < s.diag[k]; ++e) unite(s.col[e]);  // L(k, k')
+        for (int d = ccnt[k]; d < ccnt[k + 1]; ++d) {                      // U(k', k), k' < k
+          if (crow[d] >= k) break;
+          unite(crow[d]);
+        }
+      }
+      comps.clear();
+      cwork.clear();
+      for (int k : piv) {
+        const int r = root(k);
+        if (comp_id[r] < 0) {
+          comp_id[r] = (int32_t)comps.size();
+          comps.push_back(r);
+          cwork.push_back(0);
+        }
+        const int64_t nlk = ccnt[k + 1] - first_l(k), nuk = s.rowptr[k + 1] - s.diag[k] - 1;
+        cwork[comp_id[r]] += nlk * nuk + 8;
+      }
+      const int nc = (int)comps.size();
+      const int ntask = std::min(nc, std::max(1, opt.max_band_tasks));
+      {
+        std::vector<int32_t> order(nc);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int c) { return cwork[a] > cwork[c]; });
+        task_of.assign(nc, 0);
+        std::vector<std::pair<int64_t, int32_t>> heap;
+        for (int t = 0; t < ntask; ++t) heap.push_back({0, t});
+        auto cmp = [](const std::pair<int64_t, int32_t>& a, const std::pair<int64_t, int32_t>& c) {
+          return a > c;
+        };
+        std::make_heap(heap.begin(), heap.end(), cmp);
+        for (int k : order) {
+          std::pop_heap(heap.begin(), heap.end(), cmp);
+          task_of[k] = heap.back().second;
+          heap.back().first += cwork[k];
+          std::push_heap(heap.begin(), heap.end(), cmp);
+        }
+      }
+      own.clear();
+      sch.clear();
+      for (int k : piv) {
+        const int task = task_of[comp_id[root(k)]];
+        const int d0 = first_l(k);
+        const int nlk = ccnt[k + 1] - d0, nuk = s.rowptr[k + 1] - s.diag[k] - 1;
+        s.flops += (double)nlk + 2.0 * (double)nlk * (double)nuk;
+        for (int d = d0; d < ccnt[k + 1]; ++d) {
+          const int i = crow[d], lpos = cslot[d];
+          for (int e = s.diag[k] + 1; e < s.rowptr[k + 1]; ++e) {
+            const int j = s.col[e];
+            const int32_t t = find(i, j);
+            if (t < 0) {
+              err = "internal: fill closure violated";
+              return false;
+            }
+            // a row or column of a pivot in this band belongs to the pivot's task
+            if (level[i] < b1 || level[j] < b1)
+              own.push_back({task, level[k] - b0, t, k, lpos, e});
+            else
+              sch.push_back({t, k, lpos, e});
+          }
+        }
+      }
+      for (int k : piv) comp_id[root(k)] = -1;
+      std::sort(own.begin(), own.end());
+      std::sort(sch.begin(), sch.end());
+      // emit one target per (task, level, slot) with its contributions by pivot
+      auto push_target = [&](int32_t slot) {
+        B.items.push_back((int32_t)s.tgt_pos.size());
+        s.tgt_pos.push_back(slot);
+      };
+      B.band_w.push_back(w);
+      B.band_ptr_off.push_back((int32_t)B.tl_ptr.size());
+      B.band_hm_off.push_back((int32_t)B.tl_heavy.size());
+      size_t a0 = 0;
+      std::vector<std::pair<size_t, size_t>> segs;
+      for (int task = 0; task < ntask; ++task)
+        for (int j = 0; j < w; ++j) {
+          B.tl_ptr.push_back((int32_t)B.items.size());
+          segs.clear();
+          while (a0 < own.size() && own[a0][0] == task && own[a0][1] == j) {
+            size_t a1 = a0 + 1;
+            while (a1 < own.size() && own[a1][0] == task && own[a1][1] == j &&
+                   own[a1][2] == own[a0][2])
+              ++a1;
+            segs.push_back({a0, a1});
+            a0 = a1;
+          }
+          int nh = 0, nm = 0;
+          for (int pass = 2; pass >= 0; --pass)
+            for (const auto& sg : segs) {
+              const int cls = reduction_class((int)(sg.second - sg.first));
+              if (cls != pass) continue;
+              nh += cls == 2;
+              nm += cls == 1;
+              push_target(own[sg.first][2]);
+              for (size_t u = sg.first; u < sg.second; ++u) {
+                s.upd_l.push_back(own[u][4]);
+                s.upd_u.push_back(own[u][5]);
+                s.upd_d.push_back(s.diag[own[u][3]]);
+              }
+              s.tgt_seg.push_back((int32_t)s.upd_l.size());
+            }
+          B.tl_heavy.push_back(nh);
+          B.tl_mid.push_back(nm);
+          if (j == w - 1) B.tl_ptr.push_back((int32_t)B.items.size());
+        }
+      B.band_task.push_back(B.band_task.back() + ntask);
+      B.band_hub.push_back(0);
+      B.max_tasks = std::max(B.max_tasks, ntask);
+      // the band's Schur step: one target per slot, by class
+      segs.clear();
+      for (size_t c0 = 0; c0 < sch.size();) {
+        size_t c1 = c0 + 1;
+        while (c1 < sch.size() && sch[c1][0] == sch[c0][0]) ++c1;
+        segs.push_back({c0, c1});
+        c0 = c1;
+      }
+      s.fact_schur_lo.push_back((int32_t)B.items.size());
+      int nh = 0, nm = 0;
+      for (int pass = 2; pass >= 0; --pass)
+        for (const auto& sg : segs) {
+          const int cls = reduction_class((int)(sg.second - sg.first));
+          if (cls != pass) continue;
+          nh += cls == 2;
+          nm += cls == 1;
+          push_target(sch[sg.first][0]);
+          for (size_t u = sg.first; u < sg.second; ++u) {
+            s.upd_l.push_back(sch[u][2]);
+            s.upd_u.push_back(sch[u][3]);
+            s.upd_d.push_back(s.diag[sch[u][1]]);
+          }
+          s.tgt_seg.push_back((int32_t)s.upd_l.size());
+        }
+      s.fact_schur_hi.push_back((int32_t)B.items.size());
+      s.fact_schur_heavy.push_back(nh);
+      s.fact_schur_mid.push_back(nm);
+    }
+  }
   std::vector<std::tuple<int32_t, int32_t, int32_t, int32_t>> ups;
-  for (int l = 0; l < nlev; ++l) {
+  for (int l = 0; l < nlev && opt.band_width <= 0; ++l) {
     ups.clear();
     for (int k : by_level[l]) {
       // L(:,k): rows > k in column k; U(k,:): cols > k in row k
@@ -873,6 +1055,54 @@ bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
     err = "backward band program: " + err;
     return false;
   }
+  // refactorisation band program: stage = (band, phase, level) with phase 0 the
+  // tasks and 1 the Schur step.  Every update reads slots whose last write is in
+  // an earlier stage, and within a band's task phase each slot is written by
+  // one task only (tasks of one launch run concurrently).
+  const BandSchedule& B = s.fact_band;
+  if (B.band_w.empty()) return true;
+  const int64_t nnz = (int64_t)s.col.size();
+  std::vector<int64_t> last_write(nnz, -1);
+  std::vector<int32_t> writer(nnz, -1);
+  auto stage = [](int64_t b, int64_t phase, int64_t j) { return (b * 2 + phase) * 128 + j; };
+  std::vector<std::pair<int32_t, int64_t>> items;  // (item, stage) in execution order
+  std::vector<int32_t> item_task;
+  for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
+    const int w = B.band_w[b];
+    for (int j = 0; j < w; ++j)
+      for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
+        const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+        for (int g = tp[j]; g < tp[j + 1]; ++g) {
+          items.push_back({B.items[g], stage((int64_t)b, 0, j)});
+          item_task.push_back(t);
+        }
+      }
+    for (int g = s.fact_schur_lo[b]; g < s.fact_schur_hi[b]; ++g) {
+      items.push_back({B.items[g], stage((int64_t)b, 1, 0)});
+      item_task.push_back(-1);
+    }
+  }
+  for (size_t k = 0; k < items.size(); ++k) {  // writes
+    const int pos = s.tgt_pos[items[k].first];
+    const int64_t st = items[k].second;
+    if (item_task[k] >= 0 && last_write[pos] >= 0 && last_write[pos] / 128 == st / 128 &&
+        writer[pos] != item_task[k]) {
+      err = "refactorisation: two tasks of one launch write the same slot";
+      return false;
+    }
+    last_write[pos] = std::max(last_write[pos], st);
+    writer[pos] = item_task[k];
+  }
+  for (size_t k = 0; k < items.size(); ++k) {  // reads see final values
+    const int t = items[k].first;
+    const int64_t st = items[k].second;
+    for (int u = s.tgt_seg[t]; u < s.tgt_seg[t + 1]; ++u)
+      for (int32_t slot : {s.upd_l[u], s.upd_u[u], s.upd_d[u]})
+        if (last_write[slot] >= st) {
+          err = "refactorisation: an update reads a slot not final at its stage";
+          return false;
+        }
+  }
   return true;
 }
 
@@ -1003,6 +1233,77 @@ static void merge_schedule(std::vector<std::vector<int32_t>> (&bk)[3], LevelSche
   }
 }
 
+// Merge the refactorisation band programs of the diagonal blocks (target ids
+// offset by each block's first target): band b runs every block's band-b
+// tasks in one launch (a narrower block band is padded with empty levels),
+// then every block's band-b Schur items, class by class.
+static void merge_fact_bands(const std::vector<std::pair<const LuSymbolic*, int32_t>>& parts,
+                             LuSymbolic& s) {
+  BandSchedule& M = s.fact_band;
+  M = BandSchedule();
+  s.fact_schur_lo.clear();
+  s.fact_schur_hi.clear();
+  s.fact_schur_heavy.clear();
+  s.fact_schur_mid.clear();
+  size_t nb = 0;
+  for (const auto& p : parts) nb = std::max(nb, p.first->fact_band.band_w.size());
+  if (nb == 0) return;
+  M.band_task.assign(1, 0);
+  M.band_hub.assign(1, 0);
+  M.hub_cptr.assign(1, 0);
+  for (size_t b = 0; b < nb; ++b) {
+    int w = 1;
+    for (const auto& p : parts) {
+      const BandSchedule& B = p.first->fact_band;
+      if (b < B.band_w.size()) w = std::max(w, B.band_w[b]);
+    }
+    M.band_w.push_back(w);
+    M.band_ptr_off.push_back((int32_t)M.tl_ptr.size());
+    M.band_hm_off.push_back((int32_t)M.tl_heavy.size());
+    int ntask = 0;
+    for (const auto& p : parts) {
+      const BandSchedule& B = p.first->fact_band;
+      if (b >= B.band_w.size()) continue;
+      const int bw = B.band_w[b];
+      for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t, ++ntask) {
+        const size_t tl = (size_t)(t - B.band_task[b]);
+        const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + tl * (bw + 1)];
+        for (int j = 0; j < w; ++j) {
+          M.tl_ptr.push_back((int32_t)M.items.size());
+          if (j < bw) {
+            for (int g = tp[j]; g < tp[j + 1]; ++g) M.items.push_back(B.items[g] + p.second);
+            M.tl_heavy.push_back(B.tl_heavy[B.band_hm_off[b] + tl * bw + j]);
+            M.tl_mid.push_back(B.tl_mid[B.band_hm_off[b] + tl * bw + j]);
+          } else {
+            M.tl_heavy.push_back(0);
+            M.tl_mid.push_back(0);
+          }
+        }
+        M.tl_ptr.push_back((int32_t)M.items.size());
+      }
+    }
+    M.band_task.push_back(M.band_task.back() + ntask);
+    M.band_hub.push_back(0);
+    M.max_tasks = std::max(M.max_tasks, ntask);
+    s.fact_schur_lo.push_back((int32_t)M.items.size());
+    int nh = 0, nm = 0;
+    for (int cls = 2; cls >= 0; --cls)
+      for (const auto& p : parts) {
+        const LuSymbolic& P = *p.first;
+        if (b >= P.fact_band.band_w.size()) continue;
+        const int lo = P.fact_schur_lo[b], h = lo + P.fact_schur_heavy[b],
+                  m = h + P.fact_schur_mid[b], hi = P.fact_schur_hi[b];
+        const int a = cls == 2 ? lo : (cls == 1 ? h : m), e = cls == 2 ? h : (cls == 1 ? m : hi);
+        for (int g = a; g < e; ++g) M.items.push_back(P.fact_band.items[g] + p.second);
+        if (cls == 2) nh += e - a;
+        if (cls == 1) nm += e - a;
+      }
+    s.fact_schur_hi.push_back((int32_t)M.items.size());
+    s.fact_schur_heavy.push_back(nh);
+    s.fact_schur_mid.push_back(nm);
+  }
+}
+
 bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const double* vals,
                 const AnalyzeOptions& opt, LuSymbolic& s, std::string& err) {
   const int nnz = indptr[n];
@@ -1084,6 +1385,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
   s.l_diag.reserve(n_l);
   s.min_pivot_ratio = 1.0;
   std::vector<std::vector<int32_t>> bf[3];
+  std::vector<std::pair<const LuSymbolic*, int32_t>> fact_parts;
   int32_t roff = 0, soff = 0, toff = 0, uoff = 0;
   std::vector<int32_t> gmap;
   for (int c = 0; c < nc; ++c) {
@@ -1121,11 +1423,13 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
       s.l_diag.push_back(b.l_diag[e] + soff);
     }
     bucket_schedule(b.fact, toff, bf);
+    fact_parts.push_back({&b, toff});
     s.nnz_l += b.nnz_l;
     s.nnz_u += b.nnz_u;
     s.n_updates += b.n_updates;
     s.flops += b.flops;
     s.n_dense += b.n_dense;
+    s.n_fact_levels = std::max(s.n_fact_levels, b.n_fact_levels);
     s.n_offdiag_pivots += b.n_offdiag_pivots;
     s.min_pivot_ratio = std::min(s.min_pivot_ratio, b.min_pivot_ratio);
     roff += nb;
@@ -1140,6 +1444,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     s.pinv[s.p[k]] = k;
   }
   merge_schedule(bf, s.fact);
+  merge_fact_bands(fact_parts, s);
   solve_schedules(s, opt);
   make_groups(s.fact, opt.wide_threshold);
   make_groups(s.fwd, opt.wide_threshold);
@@ -1198,6 +1503,20 @@ void lu_refactor_host(const LuSymbolic& s, const double* a, T* lu) {
   for (int64_t e = 0; e < nnz; ++e) lu[e] = s.src[e] >= 0 ? (T)a[s.src[e]] : (T)0;
   const int nl = (int)s.fact.lvl_ptr.size() - 1;
   auto term = [&](int u) { return lu[s.upd_l[u]] * lu[s.upd_u[u]] / lu[s.upd_d[u]]; };
+  auto apply = [&](int t) {
+    const int a = s.tgt_seg[t], e = s.tgt_seg[t + 1];
+    lu[s.tgt_pos[t]] -= class_sum<T>(reduction_class(e - a), a, e, term);
+  };
+  const BandSchedule& B = s.fact_band;
+  for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
+    // tasks are independent within a band: task by task, level by level
+    const int w = B.band_w[b];
+    for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
+      const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+      for (int g = tp[0]; g < tp[w]; ++g) apply(B.items[g]);
+    }
+    for (int g = s.fact_schur_lo[b]; g < s.fact_schur_hi[b]; ++g) apply(B.items[g]);
+  }
   for (int l = 0; l < nl; ++l)
     for (int it = s.fact.lvl_ptr[l]; it < s.fact.lvl_ptr[l + 1]; ++it) {
       const int t = s.fact.items[it];

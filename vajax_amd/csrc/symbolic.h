@@ -105,7 +105,14 @@ struct LuSymbolic {
   std::vector<int32_t> upd_l;    // per contribution: slot of L(i,k)
   std::vector<int32_t> upd_u;    // per contribution: slot of U(k,j)
   std::vector<int32_t> upd_d;    // per contribution: slot of U(k,k)
-  LevelSchedule fact;            // items = target ids
+  LevelSchedule fact;            // items = target ids (level program)
+  // Band program of the refactorisation (used when fact_band has bands):
+  // per band, tasks of connected pivots update the targets in their own rows
+  // and columns level by level (items = target ids), then the band's updates
+  // of later rows and columns run as one wide step, items
+  // fact_band.items[fact_schur_lo[b] .. fact_schur_hi[b]) ordered by class.
+  BandSchedule fact_band;
+  std::vector<int32_t> fact_schur_lo, fact_schur_hi, fact_schur_heavy, fact_schur_mid;
   std::vector<int32_t> l_slot;   // every L slot (i,k) ...
   std::vector<int32_t> l_diag;   // ... and the slot of its pivot U(k,k)
   LevelSchedule fwd;             // items = rows, L solve
@@ -116,6 +123,7 @@ struct LuSymbolic {
   int64_t nnz_l = 0, nnz_u = 0, n_updates = 0;
   double flops = 0.0;
   int n_dense = 0;
+  int n_fact_levels = 0;     // pivot levels of the refactorisation
   int n_blocks = 1;          // diagonal blocks (connected components)
   int n_block_patterns = 1;  // distinct block patterns analysed
   int n_offdiag_pivots = 0;
