@@ -51,28 +51,52 @@ def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None):
     return eng.run_transient()
 
 
-def algorithmic_bytes(topo, lu_info, precision: str):
-    """Per-Newton-iteration algorithmic bytes of each phase (formulas of the
-    build's measurement plan): eval+stamp, LU refactor, solves."""
-    n_aug = topo.n_augmented
-    nse = topo.nse
-    b_eval = 0
-    b_stamp = 8 * nse + 8 * n_aug * 4
-    for name, ms in topo.models.items():
-        n = ms.count
-        n_v = ms.v_node1.shape[1]
-        n_par = {"resistor": 1, "capacitor": 1, "mosfet": 11}[name]
-        n_jac = ms.csr_pos.shape[1]
-        n_res = ms.res_idx.shape[1]
-        b_eval += n * (8 * n_par + n_v * (2 * 4 + 2 * 8))
-        b_stamp += n * (4 * n_jac + 4 * n_res)
-    nnz_lu = lu_info["nnz_l"] + lu_info["nnz_u"]
+# model parameter columns read by k_eval (the MOSFET's include its init cache)
+N_PAR = {"resistor": 1, "capacitor": 1, "mosfet": 11}
+
+
+def kernel_bytes(topo, info, precision: str):
+    """Algorithmic HBM bytes per Newton iteration of each kernel class (the
+    data each kernel must move at least once: indices, gathered operands,
+    results), from the circuit topology and the LU analysis."""
     vb = 4 if precision == "f32" else 8
-    b_lu = (vb + 4) * nnz_lu
-    b_solve = (vb + 4) * nnz_lu + 16 * n_aug
-    if precision == "f32":
-        b_solve = 2 * b_solve + 12 * nse + 16 * n_aug
-    return {"build": b_eval + b_stamp, "lu": b_lu, "solve": b_solve}
+    n = topo.n_augmented
+    nse = topo.nse
+    ev = 0
+    n_jc = 0
+    n_rc = 0
+    for name, ms in topo.models.items():
+        N = ms.count
+        n_v = ms.v_node1.shape[1]
+        jv = int((ms.csr_pos >= 0).sum())
+        rv = int((ms.res_idx >= 0).sum())
+        n_jc += jv
+        n_rc += rv
+        # params, node indices + voltage gathers, destination indices, writes
+        ev += N * (8 * N_PAR[name] + n_v * (2 * 4 + 2 * 8) + 4 * (ms.csr_pos.shape[1] + ms.res_idx.shape[1]))
+        ev += 8 * jv + 16 * rv
+    # stamping: contributions + segment pointers + per-slot constants + csr write;
+    # residual: rf/rq + segments + history vectors + f/Q/mrc writes
+    stamp = 8 * n_jc + 4 * (nse + 1) + 12 * nse + 8 * nse + 16 * n_rc + 4 * n + 8 * 8 * n
+    nnz_lu = info["nnz_l"] + info["nnz_u"]
+    upd = 12 + 3 * vb            # three slot indices + three gathered values
+    tgt = 4 + 4 + 8 + 2 * vb     # item, target slot, segment pair, read-modify-write
+    lu_tasks = upd * info["fact_task_updates"] + tgt * info["fact_task_targets"]
+    lu_schur = upd * info["fact_schur_updates"] + tgt * info["fact_schur_targets"]
+    lu_gather = (4 + 8 + vb) * nnz_lu
+    lu_scale = (8 + 3 * vb) * info["nnz_l"]
+    offd = nnz_lu - n            # off-diagonal entries, each read once per solve
+    solves = (4 + 2 * vb) * offd + (4 + 8 + 3 * vb) * 2 * n
+    if precision == "f32":       # second solve of the refinement step + FP64 SpMV
+        solves = 2 * solves
+    return {"eval": ev, "stamp": stamp, "lu_gather": lu_gather, "lu_tasks": lu_tasks,
+            "lu_schur": lu_schur, "lu_scale": lu_scale, "solve_tasks": solves, "solve_hub": 0}
+
+
+KERNEL_NAMES = {"eval": "k_eval<MOSFET> (+R/C)", "stamp": "k_chunk_sum/k_reduce_jac/k_chunk_res/k_residual",
+                "lu_gather": "k_lu_gather", "lu_tasks": "k_lu_band", "lu_schur": "k_lu_wide (Schur step)",
+                "lu_scale": "k_lu_scale", "solve_tasks": "k_tri_band (forward + backward)",
+                "solve_hub": "k_tri_hub_part/k_tri_hub_finish"}
 
 
 def cpu_baseline(net, steps: int):
@@ -175,7 +199,19 @@ def main():
     ap.add_argument("--batch-gates", type=int, default=2000)
     ap.add_argument("--batch-ragged", default="500,1000,2000,4000")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device buffers (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": a.numel() * 8, "copies": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "note only (see DESIGN.md)", "n_gpus": args.gpus}))
@@ -222,22 +258,32 @@ def main():
     iters = max(ph["n_iterations"], 1)
     per_iter = {"build": ph["build_ms"] / iters, "lu": ph["lu_ms"] / iters,
                 "solve": ph["solve_ms"] / iters}
-    dom = max(per_iter, key=per_iter.get)
-    bytes_ = algorithmic_bytes(topo, lu_info, args.precision)
-    achieved = bytes_[dom] / (per_iter[dom] * 1e-3) / 1e9
-    kernels = {"build": "k_eval<mosfet> + k_reduce_jac + k_residual",
-               "lu": "k_lu_gather + k_lu_wide/k_lu_narrow levels + k_lu_scale",
-               "solve": "k_solve_init + k_tri_wide/k_tri_narrow levels + k_solve_finish"}
+    kb = kernel_bytes(topo, lu_info, args.precision)
+    kernels = {}
+    for name, kt in ph["kernels"].items():
+        if kt["launches"] == 0:
+            continue
+        avg_ms = kt["ms"] / kt["launches"]
+        per_launch = kb[name] / (kt["launches"] / iters)
+        kernels[name] = {"kernel": KERNEL_NAMES[name], "ms_per_newton_iter": round(kt["ms"] / iters, 4),
+                         "launches_per_newton_iter": round(kt["launches"] / iters, 2),
+                         "avg_launch_us": round(avg_ms * 1e3, 2),
+                         "alg_bytes_per_launch": int(per_launch),
+                         "achieved_GBs": round(per_launch / (avg_ms * 1e-3) / 1e9, 2)}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_newton_iter"])
+    d = kernels[dom]
     roofline = {
         "bound": "hbm",
-        "kernel": kernels[dom],
-        "achieved": round(achieved, 3),
+        "kernel": d["kernel"],
+        "achieved": d["achieved_GBs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 6),
+        "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6),
         "traffic": None,
+        "avg_launch_us": d["avg_launch_us"],
+        "alg_bytes_per_launch": d["alg_bytes_per_launch"],
         "phase_ms_per_newton_iter": {k: round(v, 4) for k, v in per_iter.items()},
-        "algorithmic_bytes_per_newton_iter": bytes_,
+        "kernels": kernels,
     }
 
     cpu = None

@@ -100,6 +100,10 @@ typedef struct {
   int32_t n_block_patterns;
   /* triangular-solve band programs: bands (launches) and hub rows */
   int32_t fwd_bands, bwd_bands, fwd_hub_rows, bwd_hub_rows;
+  /* refactorisation band program: update contributions and targets run by
+   * the band tasks and by the Schur steps (all in the tasks' counts when the
+   * level program is used) */
+  int64_t fact_task_updates, fact_schur_updates, fact_task_targets, fact_schur_targets;
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
@@ -276,6 +280,12 @@ typedef struct {
   double lu_ms;     /* numeric refactorisation */
   double solve_ms;  /* triangular solves (+ refinement) */
   int64_t n_iterations;
+  /* per kernel class, from event pairs around each launch: 0 device eval,
+   * 1 stamping reductions + residual, 2 LU gather, 3 LU band tasks (or LU
+   * level launches), 4 LU Schur steps, 5 LU scaling, 6 solve band tasks (or
+   * solve level launches), 7 solve hub rows */
+  double kernel_ms[8];
+  int64_t kernel_launches[8];
 } vjx_phase_times;
 
 int vjx_profile_enable(int32_t enable);

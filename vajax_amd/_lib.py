@@ -39,6 +39,8 @@ class LuInfo(C.Structure):
         ("n_block_patterns", C.c_int32),
         ("fwd_bands", C.c_int32), ("bwd_bands", C.c_int32),
         ("fwd_hub_rows", C.c_int32), ("bwd_hub_rows", C.c_int32),
+        ("fact_task_updates", C.c_int64), ("fact_schur_updates", C.c_int64),
+        ("fact_task_targets", C.c_int64), ("fact_schur_targets", C.c_int64),
     ]
 
     def as_dict(self):
@@ -100,10 +102,17 @@ class TranResult(C.Structure):
 
 class PhaseTimes(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("lu_ms", C.c_double), ("solve_ms", C.c_double),
-                ("n_iterations", C.c_int64)]
+                ("n_iterations", C.c_int64), ("kernel_ms", C.c_double * 8),
+                ("kernel_launches", C.c_int64 * 8)]
+
+    KERNEL_CLASSES = ("eval", "stamp", "lu_gather", "lu_tasks", "lu_schur", "lu_scale",
+                      "solve_tasks", "solve_hub")
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k in ("build_ms", "lu_ms", "solve_ms", "n_iterations")}
+        d["kernels"] = {name: {"ms": self.kernel_ms[i], "launches": self.kernel_launches[i]}
+                        for i, name in enumerate(self.KERNEL_CLASSES)}
+        return d
 
 
 _PROTOS = {

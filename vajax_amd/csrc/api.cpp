@@ -123,6 +123,26 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->bwd_bands = (int32_t)s.bwd_band.band_w.size();
   o->fwd_hub_rows = (int32_t)s.fwd_band.hub_rows.size();
   o->bwd_hub_rows = (int32_t)s.bwd_band.hub_rows.size();
+  o->fact_task_updates = o->fact_schur_updates = o->fact_task_targets = o->fact_schur_targets = 0;
+  if (!s.fact_band.band_w.empty()) {
+    std::vector<char> schur(s.fact_band.items.size(), 0);
+    for (size_t b = 0; b < s.fact_schur_lo.size(); ++b)
+      for (int g = s.fact_schur_lo[b]; g < s.fact_schur_hi[b]; ++g) schur[g] = 1;
+    for (size_t g = 0; g < s.fact_band.items.size(); ++g) {
+      const int t = s.fact_band.items[g];
+      const int64_t u = s.tgt_seg[t + 1] - s.tgt_seg[t];
+      if (schur[g]) {
+        o->fact_schur_updates += u;
+        o->fact_schur_targets += 1;
+      } else {
+        o->fact_task_updates += u;
+        o->fact_task_targets += 1;
+      }
+    }
+  } else {
+    o->fact_task_updates = s.n_updates;
+    o->fact_task_targets = (int64_t)s.tgt_pos.size();
+  }
   return VJX_OK;
 }
 

@@ -153,6 +153,26 @@ struct HubPlan {
   }
 };
 
+// ----- per-kernel timing (profiling passes only, never under graph capture) --
+enum KernelClass {
+  KC_EVAL = 0,     // k_eval<model>
+  KC_STAMP,        // chunk pre-passes + k_reduce_jac + k_residual
+  KC_LU_GATHER,    // k_lu_gather
+  KC_LU_TASKS,     // k_lu_band (and level launches of the level program)
+  KC_LU_SCHUR,     // k_lu_wide of a band's Schur step
+  KC_LU_SCALE,     // k_lu_scale
+  KC_TRI_TASKS,    // k_tri_band (and level launches of the level program)
+  KC_TRI_HUB,      // chunked hub rows of the solves
+  KC_COUNT
+};
+void ktimer_enable(bool on);
+bool ktimer_on();
+void ktimer_begin(hipStream_t s);
+void ktimer_end(int cls, hipStream_t s);
+void ktimer_collect();  // after a stream synchronisation
+void ktimer_reset();
+void ktimer_read(double* ms, int64_t* launches);  // KC_COUNT entries each
+
 // ----- kernel launchers (kernels.hip) -----
 struct EvalOut {
   double* jac;  // contribution arrays (sorted slots), may be null

@@ -19,6 +19,7 @@
 #include "circuit.h"
 #include "internal.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace vjx {
@@ -653,17 +654,19 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_lu_scale(int n, const int32_t* __restrict__ ls,
                                                   const int32_t* __restrict__ ld, T* lu,
                                                   DevStatus* st) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  // grid-stride over a capped grid: few workgroups, hence few status atomics
   double mag = 0.0;
-  bool bad = false;
-  if (e < n) {
+  unsigned int bad = 0;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const T v = lu[ls[e]] / lu[ld[e]];
     lu[ls[e]] = v;
-    mag = fabs((double)v);
-    bad = !(mag <= 1.7976931348623157e308);
+    const double m = fabs((double)v);
+    if (m <= 1.7976931348623157e308)
+      mag = fmax(mag, m);
+    else
+      ++bad;
   }
-  if (st) block_flags(dbits(bad ? 0.0 : mag), 0, bad ? 1u : 0u, &st->maxl_bits, nullptr,
-                        &st->nonfinite);
+  if (st) block_flags(dbits(mag), 0, bad, &st->maxl_bits, nullptr, &st->nonfinite);
 }
 
 // ---- triangular solves ------------------------------------------------------
@@ -936,6 +939,7 @@ void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalO
   if (m.count == 0) return;
   const int B = 256;
   const int G = nblk(m.count, B);
+  ktimer_begin(s);
   switch (m.model_id) {
     case VJX_MODEL_RESISTOR:
       k_eval<VJX_MODEL_RESISTOR><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
@@ -953,6 +957,7 @@ void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalO
                                                out.rq);
       break;
   }
+  ktimer_end(KC_EVAL, s);
 }
 
 void launch_model_eval_raw(int id, int count, const double* vb, const double* par, double* rr,
@@ -982,18 +987,22 @@ void launch_reduce_jac(int nse, const int32_t* seg, const double* contrib, const
                        const double* slot_b, const DevScalars* sc, double* csr,
                        const HubChunks& hub, hipStream_t s) {
   if (nse == 0) return;
+  ktimer_begin(s);
   if (hub.n_chunks > 0) k_chunk_sum<<<hub.n_chunks, 256, 0, s>>>(hub.cs, hub.ce, contrib, hub.part);
   k_reduce_jac<<<hub.n_items + nblk(nse, 256), 256, 0, s>>>(nse, seg, contrib, slot_diag, slot_b,
                                                             sc, csr, hub);
+  ktimer_end(KC_STAMP, s);
 }
 
 void launch_residual(const ResidualArgs& a, hipStream_t s) {
   const int n = a.n_unk + a.m;
   if (n == 0) return;
   const HubChunks& h = a.hub;
+  ktimer_begin(s);
   if (h.n_chunks > 0)
     k_chunk_res<<<h.n_chunks, 256, 0, s>>>(h.cs, h.ce, a.rf, a.rq, h.part, h.part2, h.part3);
   k_residual<<<h.n_items + nblk(n, 256), 256, 0, s>>>(a);
+  ktimer_end(KC_STAMP, s);
 }
 
 template <typename T>
@@ -1005,14 +1014,19 @@ static void run_fact_bands(DevLU& L, T* lu, hipStream_t s) {
                    L.fact_band.tl_mid.p};
   for (size_t b = 0; b + 1 < H.band_task.size(); ++b) {
     const int ntask = H.band_task[b + 1] - H.band_task[b];
-    if (ntask > 0)
+    if (ntask > 0) {
+      ktimer_begin(s);
       k_lu_band<T><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], B, P,
                                          lu);
+      ktimer_end(KC_LU_TASKS, s);
+    }
     const int lo = S.fact_schur_lo[b], hi = S.fact_schur_hi[b];
     if (hi > lo) {
       const int nh = S.fact_schur_heavy[b], nm = S.fact_schur_mid[b];
       const int grid = nh + nblk(nm, kGroupsPerBlock) + nblk(hi - lo - nh - nm, 256);
+      ktimer_begin(s);
       k_lu_wide<T><<<grid, 256, 0, s>>>(lo, hi, nh, nm, P, lu);
+      ktimer_end(KC_LU_SCHUR, s);
     }
   }
 }
@@ -1025,6 +1039,7 @@ static void run_fact_groups(DevLU& L, T* lu, hipStream_t s) {
   }
   const LuProg P{L.fact.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
   for (const auto& g : L.fact.groups) {
+    ktimer_begin(s);
     if (g.wide) {
       const int lo = L.fact.lvl_ptr[g.lvl_begin], hi = L.fact.lvl_ptr[g.lvl_end];
       const int nh = L.fact.lvl_heavy[g.lvl_begin], nm = L.fact.lvl_mid[g.lvl_begin];
@@ -1036,22 +1051,30 @@ static void run_fact_groups(DevLU& L, T* lu, hipStream_t s) {
       k_lu_narrow<T><<<1, L.narrow_block, shm, s>>>(g.lvl_begin, g.lvl_end, L.fact.lvl_dev.p,
                                                     L.fact.heavy_dev.p, L.fact.mid_dev.p, P, lu);
     }
+    ktimer_end(KC_LU_TASKS, s);
   }
 }
 
 template <typename T>
 void launch_lu_refactor(DevLU& L, const double* csr, T* lu, DevStatus* st, hipStream_t s) {
   const int nnz = (int)L.col.n;
+  ktimer_begin(s);
   k_lu_gather<T><<<nblk(nnz, 256), 256, 0, s>>>(nnz, L.src.p, csr, lu);
+  ktimer_end(KC_LU_GATHER, s);
   run_fact_groups<T>(L, lu, s);
   const int nl = (int)L.l_slot.n;
-  if (nl > 0) k_lu_scale<T><<<nblk(nl, 256), 256, 0, s>>>(nl, L.l_slot.p, L.l_diag.p, lu, st);
+  if (nl > 0) {
+    ktimer_begin(s);
+    k_lu_scale<T><<<std::min(nblk(nl, 256), 2048), 256, 0, s>>>(nl, L.l_slot.p, L.l_diag.p, lu, st);
+    ktimer_end(KC_LU_SCALE, s);
+  }
 }
 
 template <typename T, bool FWD>
 static void run_tri_groups(DevLU& L, const DevSchedule& S, const T* lu, T* y, T* z, hipStream_t s) {
   const TriProg P{S.items.p, L.rowptr.p, L.col.p, L.diag.p};
   for (const auto& g : S.groups) {
+    ktimer_begin(s);
     if (g.wide) {
       const int lo = S.lvl_ptr[g.lvl_begin], hi = S.lvl_ptr[g.lvl_end];
       const int nh = S.lvl_heavy[g.lvl_begin], nm = S.lvl_mid[g.lvl_begin];
@@ -1063,6 +1086,7 @@ static void run_tri_groups(DevLU& L, const DevSchedule& S, const T* lu, T* y, T*
       k_tri_narrow<T, FWD><<<1, L.narrow_block, shm, s>>>(g.lvl_begin, g.lvl_end, S.lvl_dev.p,
                                                           S.heavy_dev.p, S.mid_dev.p, P, lu, y, z);
     }
+    ktimer_end(KC_TRI_TASKS, s);
   }
 }
 
@@ -1074,16 +1098,21 @@ static void run_tri_bands(DevLU& L, const BandSchedule& H, DevBand& D, const T* 
   T* part = reinterpret_cast<T*>(D.part.p);
   for (size_t b = 0; b + 1 < H.band_task.size(); ++b) {
     const int ntask = H.band_task[b + 1] - H.band_task[b];
-    if (ntask > 0)
+    if (ntask > 0) {
+      ktimer_begin(s);
       k_tri_band<T, FWD><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b],
                                                B, P, lu, y, z);
+      ktimer_end(KC_TRI_TASKS, s);
+    }
     const int h0 = H.band_hub[b], h1 = H.band_hub[b + 1];
     if (h1 > h0) {
       const int c0 = H.hub_cptr[h0], c1 = H.hub_cptr[h1];
+      ktimer_begin(s);
       k_tri_hub_part<T, FWD><<<c1 - c0, 256, 0, s>>>(D.hub_cs.p + c0, D.hub_ce.p + c0, P, lu, y, z,
                                                       part + c0);
       k_tri_hub_finish<T, FWD><<<nblk(h1 - h0, 64), 64, 0, s>>>(h0, h1, D.hub_rows.p,
                                                                  D.hub_cptr.p, part, P, lu, y, z);
+      ktimer_end(KC_TRI_HUB, s);
     }
   }
 }

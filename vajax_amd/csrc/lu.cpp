@@ -8,6 +8,68 @@
 
 namespace vjx {
 
+// Event pairs around individual launches, folded in after a synchronisation.
+namespace {
+struct KTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<size_t, int>> pending;  // (start event index, class)
+  double ms[KC_COUNT] = {};
+  int64_t launches[KC_COUNT] = {};
+};
+KTimer g_kt;
+hipEvent_t kt_event() {
+  if (g_kt.used == g_kt.pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    g_kt.pool.push_back(e);
+  }
+  return g_kt.pool[g_kt.used++];
+}
+}  // namespace
+
+void ktimer_enable(bool on) { g_kt.on = on; }
+bool ktimer_on() { return g_kt.on; }
+void ktimer_begin(hipStream_t s) {
+  if (!g_kt.on) return;
+  hipEvent_t e = kt_event();
+  if (e) (void)hipEventRecord(e, s);
+}
+void ktimer_end(int cls, hipStream_t s) {
+  if (!g_kt.on || g_kt.used == 0) return;
+  const size_t start = g_kt.used - 1;
+  hipEvent_t e = kt_event();
+  if (!e) return;
+  (void)hipEventRecord(e, s);
+  g_kt.pending.push_back({start, cls});
+}
+void ktimer_collect() {
+  for (const auto& p : g_kt.pending) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, g_kt.pool[p.first], g_kt.pool[p.first + 1]) == hipSuccess) {
+      g_kt.ms[p.second] += t;
+      g_kt.launches[p.second] += 1;
+    }
+  }
+  g_kt.pending.clear();
+  g_kt.used = 0;
+}
+void ktimer_reset() {
+  g_kt.pending.clear();
+  g_kt.used = 0;
+  for (int c = 0; c < KC_COUNT; ++c) {
+    g_kt.ms[c] = 0.0;
+    g_kt.launches[c] = 0;
+  }
+}
+void ktimer_read(double* ms, int64_t* launches) {
+  for (int c = 0; c < KC_COUNT; ++c) {
+    ms[c] = g_kt.ms[c];
+    launches[c] = g_kt.launches[c];
+  }
+}
+
 int pattern_stream(vjx_pattern* pat) {
   if (pat->stream) return VJX_OK;
   VJX_TRY_HIP(hipStreamCreateWithFlags(&pat->stream, hipStreamNonBlocking));

@@ -21,7 +21,7 @@ namespace vjx {
 namespace {
 
 bool g_profile = false;
-vjx_phase_times g_phase = {0.0, 0.0, 0.0, 0};
+vjx_phase_times g_phase = {};
 
 double bits_to_double(unsigned long long b) {
   double d;
@@ -119,6 +119,7 @@ struct Driver {
 
   // after a stream sync: fold the event pairs of the last iteration in
   void collect_phase_times() {
+    ktimer_collect();
     if (!ev_pending) return;
     float a = 0.f, b = 0.f, d = 0.f;
     if (hipEventElapsedTime(&a, ev[0], ev[1]) == hipSuccess &&
@@ -156,6 +157,8 @@ struct Driver {
     const bool prof = g_profile && !cfg->use_graph;
     if (prof && !ev[0])
       for (auto& e : ev) VJX_TRY_HIP(hipEventCreate(&e));
+    // per-launch events only in eager profiling passes (never under capture)
+    ktimer_enable(prof);
     launch_copy(n_total + m, w.Xw.p, w.Xsave.p, s);
     launch_status_reset(pat->d_status.p, s);
     launch_status_reset(w.st.p, s);
@@ -273,6 +276,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     set_error("vjx_transient_run: null argument");
     return VJX_ERR_ARG;
   }
+  ktimer_enable(false);
   if (cfg->max_steps < 2 || !(cfg->dt > 0.0) || cfg->tran_itl < 1 ||
       (cfg->precision != VJX_PREC_F64 && cfg->precision != VJX_PREC_F32)) {
     set_error("vjx_transient_run: invalid configuration");
@@ -581,7 +585,8 @@ extern "C" int vjx_profile_enable(int32_t enable) {
 }
 
 extern "C" int vjx_profile_reset(void) {
-  vjx::g_phase = vjx_phase_times{0.0, 0.0, 0.0, 0};
+  vjx::g_phase = vjx_phase_times{};
+  vjx::ktimer_reset();
   return VJX_OK;
 }
 
@@ -591,5 +596,6 @@ extern "C" int vjx_profile_read(vjx_phase_times* out) {
     return VJX_ERR_ARG;
   }
   *out = vjx::g_phase;
+  vjx::ktimer_read(out->kernel_ms, out->kernel_launches);
   return VJX_OK;
 }
