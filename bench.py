@@ -79,14 +79,21 @@ def kernel_bytes(topo, info, precision: str):
     # residual: rf/rq + segments + history vectors + f/Q/mrc writes
     stamp = 8 * n_jc + 4 * (nse + 1) + 12 * nse + 8 * nse + 16 * n_rc + 4 * n + 8 * 8 * n
     nnz_lu = info["nnz_l"] + info["nnz_u"]
-    upd = 12 + 3 * vb            # three slot indices + three gathered values
-    tgt = 4 + 4 + 8 + 2 * vb     # item, target slot, segment pair, read-modify-write
+    # LU update programs: the three slot indices of every contribution are
+    # streamed once; per target the item, slot, segment pair and the value's
+    # read-modify-write.  Operand values (L, U, pivot) are gathered many times
+    # but are counted once, by k_lu_gather / k_lu_scale (the minimum traffic).
+    upd = 12
+    tgt = 4 + 4 + 8 + 2 * vb
     lu_tasks = upd * info["fact_task_updates"] + tgt * info["fact_task_targets"]
     lu_schur = upd * info["fact_schur_updates"] + tgt * info["fact_schur_targets"]
     lu_gather = (4 + 8 + vb) * nnz_lu
     lu_scale = (8 + 3 * vb) * info["nnz_l"]
     offd = nnz_lu - n            # off-diagonal entries, each read once per solve
-    solves = (4 + 2 * vb) * offd + (4 + 8 + 3 * vb) * 2 * n
+    # solves: each off-diagonal entry's column index and value once (the y
+    # operands it gathers are counted once per row), per row the item, row
+    # pointers, the right-hand side / result and the pivot
+    solves = (4 + vb) * offd + (4 + 8 + 3 * vb) * 2 * n
     if precision == "f32":       # second solve of the refinement step + FP64 SpMV
         solves = 2 * solves
     return {"eval": ev, "stamp": stamp, "lu_gather": lu_gather, "lu_tasks": lu_tasks,
@@ -97,6 +104,25 @@ KERNEL_NAMES = {"eval": "k_eval<MOSFET> (+R/C)", "stamp": "k_chunk_sum/k_reduce_
                 "lu_gather": "k_lu_gather", "lu_tasks": "k_lu_band", "lu_schur": "k_lu_wide (Schur step)",
                 "lu_scale": "k_lu_scale", "solve_tasks": "k_tri_band (forward + backward)",
                 "solve_hub": "k_tri_hub_part/k_tri_hub_finish"}
+
+
+# kernel class -> kernel name in the rocprofv3 traffic summaries
+TRAFFIC_KERNEL = {"eval": "k_eval<2>", "stamp": "k_reduce_jac", "lu_gather": "k_lu_gather<double>",
+                  "lu_tasks": "k_lu_band<double>", "lu_schur": "k_lu_wide<double>",
+                  "lu_scale": "k_lu_scale<double>", "solve_tasks": "k_tri_band<double, true>"}
+
+
+def measured_traffic(args, cls):
+    """HBM bytes per dispatch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the
+    copy-kernel calibration), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_gate_dag_{args.gates}.json")
+    if args.precision != "f64" or args.seed != 0 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        data = json.load(fh)
+    k = data["kernels"].get(TRAFFIC_KERNEL.get(cls, ""))
+    return int(k["traffic_bytes"]) if k else None
 
 
 def cpu_baseline(net, steps: int):
@@ -272,6 +298,7 @@ def main():
                          "achieved_GBs": round(per_launch / (avg_ms * 1e-3) / 1e9, 2)}
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_newton_iter"])
     d = kernels[dom]
+    traffic = measured_traffic(args, dom)
     roofline = {
         "bound": "hbm",
         "kernel": d["kernel"],
@@ -279,7 +306,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6),
-        "traffic": None,
+        "traffic": traffic,
         "avg_launch_us": d["avg_launch_us"],
         "alg_bytes_per_launch": d["alg_bytes_per_launch"],
         "phase_ms_per_newton_iter": {k: round(v, 4) for k, v in per_iter.items()},
