@@ -104,6 +104,9 @@ typedef struct {
    * the band tasks and by the Schur steps (all in the tasks' counts when the
    * level program is used) */
   int64_t fact_task_updates, fact_schur_updates, fact_task_targets, fact_schur_targets;
+  /* analysis time in ms: ordering, pivoting, LU pattern, refactorisation
+   * program, solve programs (single-block systems; zero for packed batches) */
+  double analyze_ms[5];
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);

@@ -41,10 +41,13 @@ class LuInfo(C.Structure):
         ("fwd_hub_rows", C.c_int32), ("bwd_hub_rows", C.c_int32),
         ("fact_task_updates", C.c_int64), ("fact_schur_updates", C.c_int64),
         ("fact_task_targets", C.c_int64), ("fact_schur_targets", C.c_int64),
+        ("analyze_ms", C.c_double * 5),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "analyze_ms"}
+        d["analyze_ms"] = list(self.analyze_ms)
+        return d
 
 
 class ModelDesc(C.Structure):

@@ -123,6 +123,7 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->bwd_bands = (int32_t)s.bwd_band.band_w.size();
   o->fwd_hub_rows = (int32_t)s.fwd_band.hub_rows.size();
   o->bwd_hub_rows = (int32_t)s.bwd_band.hub_rows.size();
+  for (int k = 0; k < 5; ++k) o->analyze_ms[k] = s.analyze_ms[k];
   o->fact_task_updates = o->fact_schur_updates = o->fact_task_targets = o->fact_schur_targets = 0;
   if (!s.fact_band.band_w.empty()) {
     std::vector<char> schur(s.fact_band.items.size(), 0);

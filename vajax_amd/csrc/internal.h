@@ -97,7 +97,10 @@ struct DevSchedule {
 struct DevBand {
   DBuf<int32_t> items, tl_ptr, tl_heavy, tl_mid, hub_rows, hub_cptr, hub_cs, hub_ce;
   DBuf<double> part;  // hub chunk partials (FP32 solves use it as float storage)
+  DBuf<int32_t> lds_rec, lds_task, lds_slots;  // LDS form (solves)
+  DBuf<uint16_t> lds_col;
   bool on = false;
+  bool use_lds = true;
 };
 
 struct DevLU {
@@ -107,6 +110,9 @@ struct DevLU {
   DBuf<int32_t> rowptr, col, diag, src, tgt_pos, tgt_seg, upd_l, upd_u, upd_d, l_slot, l_diag, p, q;
   DevSchedule fact, fwd, bwd;
   DevBand fwd_band, bwd_band, fact_band;
+  DBuf<int32_t> lds_item, lds_task, lds_slots;  // LDS form of the LU band tasks
+  DBuf<uint32_t> lds_upd;
+  bool use_lds = true;  // run LU band tasks that fit from LDS
   DBuf<double> lu64;
   DBuf<float> lu32;
   DBuf<double> y64, z64, r64, x64;

@@ -650,6 +650,71 @@ __global__ void __launch_bounds__(256) k_lu_band(int w, int ptr_off, int hm_off,
   }
 }
 
+// The same band task with its slots in LDS: the task's targets and every
+// operand its updates read are loaded once, the levels run on LDS (one
+// dependent global load per item for its record, the rest on-chip), and the
+// targets are written back.  Updates reference slots by 16-bit local index:
+// packed word a = (L index | U index << 16), word b = pivot index.  Same
+// fixed-order sums as the HBM form, so the factors are bit-identical.
+template <typename T>
+struct LdsTerm {
+  const T* v;
+  const uint2* up;
+  __device__ __forceinline__ T operator()(int k) const {
+    const uint2 a = up[k];
+    return v[a.x & 0xffffu] * v[a.x >> 16] / v[a.y];
+  }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_off, int task0,
+                                                     BandProg B, const int4* __restrict__ recs,
+                                                     const uint2* __restrict__ upd,
+                                                     const int4* __restrict__ tasks,
+                                                     const int32_t* __restrict__ slots, T* lu) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* vals = reinterpret_cast<T*>(lds_raw);
+  __shared__ T red[RB];
+  __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
+  const int t = blockIdx.x;
+  const int4 tk = tasks[task0 + t];  // slot offset, slot count, written count
+  const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
+  const int32_t* th = B.tl_heavy + hm_off + (size_t)t * w;
+  const int32_t* tm = B.tl_mid + hm_off + (size_t)t * w;
+  for (int j = threadIdx.x; j <= w; j += blockDim.x) s_ptr[j] = tp[j];
+  for (int j = threadIdx.x; j < w; j += blockDim.x) {
+    s_h[j] = th[j];
+    s_m[j] = tm[j];
+  }
+  for (int i = threadIdx.x; i < tk.y; i += blockDim.x) vals[i] = lu[slots[tk.x + i]];
+  __syncthreads();
+  const LdsTerm<T> term{vals, upd};
+  const int lane = (int)threadIdx.x % kGroupLanes;
+  const int ngrp = (int)blockDim.x / kGroupLanes;
+  for (int j = 0; j < w; ++j) {
+    const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
+    if (lo == hi) continue;  // uniform across the block
+    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x) {
+      const int4 r = recs[g];
+      T acc = (T)0;
+      for (int k = r.y; k < r.y + r.z; ++k) acc += term(k);
+      vals[r.w] -= acc;
+    }
+    for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp) {
+      const int4 r = recs[lo + nh + grp];
+      const T acc = group_sum<kGroupLanes, T>(r.y, r.y + r.z, lane, term);
+      if (lane == 0) vals[r.w] -= acc;
+    }
+    for (int h = 0; h < nh; ++h) {
+      const int4 r = recs[lo + h];
+      const T acc = block_sum<T>(r.y, r.y + r.z, term, red);
+      if (threadIdx.x == 0) vals[r.w] -= acc;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < tk.z; i += blockDim.x) lu[slots[tk.x + i]] = vals[i];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_lu_scale(int n, const int32_t* __restrict__ ls,
                                                   const int32_t* __restrict__ ld, T* lu,
@@ -817,6 +882,83 @@ __global__ void __launch_bounds__(256) k_tri_band(int w, int ptr_off, int hm_off
     for (int h = 0; h < nh; ++h) tri_heavy<T, FWD>(B.items[lo + h], P, lu, y, z, red);
     __syncthreads();
   }
+}
+
+// The same band task with its vector values in LDS: the task's own rows and
+// every row they read are loaded once, the levels run on LDS, and the own
+// rows are written back.  Entry e of an item's row reads the LDS value at
+// lds_col[c0 + e - e0]; the row's values and the sums' order are those of
+// the HBM form, so results are bit-identical.
+template <typename T>
+struct TriLdsTerm {
+  const T* lu;
+  const T* v;
+  const uint16_t* lcol;
+  int shift;  // c0 - e0
+  __device__ __forceinline__ T operator()(int e) const { return lu[e] * v[lcol[e + shift]]; }
+};
+
+template <typename T, bool FWD>
+__global__ void __launch_bounds__(256) k_tri_band_lds(int w, int ptr_off, int hm_off, int task0,
+                                                      BandProg B, const int4* __restrict__ recs,
+                                                      const uint16_t* __restrict__ lcol,
+                                                      const int4* __restrict__ tasks,
+                                                      const int32_t* __restrict__ slots,
+                                                      const T* __restrict__ lu, T* y, T* z) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* v = reinterpret_cast<T*>(lds_raw);
+  __shared__ T red[RB];
+  __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
+  const int t = blockIdx.x;
+  const int4 tk = tasks[task0 + t];  // slot offset, slot count, own rows
+  const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
+  const int32_t* th = B.tl_heavy + hm_off + (size_t)t * w;
+  const int32_t* tm = B.tl_mid + hm_off + (size_t)t * w;
+  for (int j = threadIdx.x; j <= w; j += blockDim.x) s_ptr[j] = tp[j];
+  for (int j = threadIdx.x; j < w; j += blockDim.x) {
+    s_h[j] = th[j];
+    s_m[j] = tm[j];
+  }
+  const T* src = FWD ? y : z;
+  for (int i = threadIdx.x; i < tk.y; i += blockDim.x) v[i] = src[slots[tk.x + i]];
+  __syncthreads();
+  const int lane = (int)threadIdx.x % kGroupLanes;
+  const int ngrp = (int)blockDim.x / kGroupLanes;
+  // row g of the band: its sum, then the forward / backward update in LDS
+  auto finish = [&](int g, const int4& r, T acc) {
+    if (FWD) {
+      v[r.x] = v[r.x] - acc;
+    } else {
+      const int i = B.items[g];
+      v[r.x] = (y[i] - acc) / lu[r.y - 1];  // the diagonal precedes the U part
+    }
+  };
+  for (int j = 0; j < w; ++j) {
+    const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
+    if (lo == hi) continue;  // uniform across the block
+    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x) {
+      const int4 r = recs[g];
+      const TriLdsTerm<T> term{lu, v, lcol, r.w - r.y};
+      T acc = (T)0;
+      for (int e = r.y; e < r.z; ++e) acc += term(e);
+      finish(g, r, acc);
+    }
+    for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp) {
+      const int g = lo + nh + grp;
+      const int4 r = recs[g];
+      const T acc = group_sum<kGroupLanes, T>(r.y, r.z, lane, TriLdsTerm<T>{lu, v, lcol, r.w - r.y});
+      if (lane == 0) finish(g, r, acc);
+    }
+    for (int h = 0; h < nh; ++h) {
+      const int g = lo + h;
+      const int4 r = recs[g];
+      const T acc = block_sum<T>(r.y, r.z, TriLdsTerm<T>{lu, v, lcol, r.w - r.y}, red);
+      if (threadIdx.x == 0) finish(g, r, acc);
+    }
+    __syncthreads();
+  }
+  T* dst = FWD ? y : z;
+  for (int i = threadIdx.x; i < tk.z; i += blockDim.x) dst[slots[tk.x + i]] = v[i];
 }
 
 // Rows longer than kHubChunk: chunk partials over the grid, then one lane per
@@ -1016,8 +1158,15 @@ static void run_fact_bands(DevLU& L, T* lu, hipStream_t s) {
     const int ntask = H.band_task[b + 1] - H.band_task[b];
     if (ntask > 0) {
       ktimer_begin(s);
-      k_lu_band<T><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], B, P,
-                                         lu);
+      const int nv = S.band_lds.empty() ? 0 : S.band_lds[b];
+      if (nv > 0 && L.use_lds)
+        k_lu_band_lds<T><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
+            H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], B,
+            reinterpret_cast<const int4*>(L.lds_item.p), reinterpret_cast<const uint2*>(L.lds_upd.p),
+            reinterpret_cast<const int4*>(L.lds_task.p), L.lds_slots.p, lu);
+      else
+        k_lu_band<T><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], B,
+                                           P, lu);
       ktimer_end(KC_LU_TASKS, s);
     }
     const int lo = S.fact_schur_lo[b], hi = S.fact_schur_hi[b];
@@ -1100,8 +1249,15 @@ static void run_tri_bands(DevLU& L, const BandSchedule& H, DevBand& D, const T* 
     const int ntask = H.band_task[b + 1] - H.band_task[b];
     if (ntask > 0) {
       ktimer_begin(s);
-      k_tri_band<T, FWD><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b],
-                                               B, P, lu, y, z);
+      const int nv = H.band_lds.empty() ? 0 : H.band_lds[b];
+      if (nv > 0 && D.use_lds)
+        k_tri_band_lds<T, FWD><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
+            H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], B,
+            reinterpret_cast<const int4*>(D.lds_rec.p), D.lds_col.p,
+            reinterpret_cast<const int4*>(D.lds_task.p), D.lds_slots.p, lu, y, z);
+      else
+        k_tri_band<T, FWD><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b],
+                                                 H.band_hm_off[b], B, P, lu, y, z);
       ktimer_end(KC_TRI_TASKS, s);
     }
     const int h0 = H.band_hub[b], h1 = H.band_hub[b + 1];

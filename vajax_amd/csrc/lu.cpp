@@ -2,6 +2,7 @@
 // asynchronous refactor / solve sequences used by the C ABI and the
 // transient driver.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "circuit.h"
@@ -141,6 +142,10 @@ static hipError_t upload_band(DevBand& d, const BandSchedule& h) {
   if (e == hipSuccess) e = d.hub_cs.upload(h.hub_cs);
   if (e == hipSuccess) e = d.hub_ce.upload(h.hub_ce);
   if (e == hipSuccess) e = d.part.alloc(h.hub_cs.size());
+  if (e == hipSuccess) e = d.lds_rec.upload(h.lds_rec);
+  if (e == hipSuccess) e = d.lds_col.upload(h.lds_col);
+  if (e == hipSuccess) e = d.lds_task.upload(h.lds_task);
+  if (e == hipSuccess) e = d.lds_slots.upload(h.lds_slots);
   return e;
 }
 
@@ -169,6 +174,16 @@ int pattern_upload(vjx_pattern* pat) {
   VJX_TRY_HIP(upload_band(L.fwd_band, s.fwd_band));
   VJX_TRY_HIP(upload_band(L.bwd_band, s.bwd_band));
   VJX_TRY_HIP(upload_band(L.fact_band, s.fact_band));
+  VJX_TRY_HIP(L.lds_item.upload(s.lds_item));
+  VJX_TRY_HIP(L.lds_upd.upload(s.lds_upd));
+  VJX_TRY_HIP(L.lds_task.upload(s.lds_task));
+  VJX_TRY_HIP(L.lds_slots.upload(s.lds_slots));
+  // VJX_LU_LDS=0 keeps every LU band task in HBM (A/B measurements)
+  const char* lds_env = std::getenv("VJX_LU_LDS");
+  L.use_lds = !(lds_env && lds_env[0] == '0');
+  // VJX_SOLVE_LDS=0 keeps every solve band task in HBM (A/B measurements)
+  const char* tri_env = std::getenv("VJX_SOLVE_LDS");
+  L.fwd_band.use_lds = L.bwd_band.use_lds = !(tri_env && tri_env[0] == '0');
   const size_t nnz = s.col.size();
   const size_t n = (size_t)s.n;
   VJX_TRY_HIP(L.lu64.alloc(nnz));

@@ -3,11 +3,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <set>
 #include <array>
+#include <chrono>
 #include <tuple>
 
 namespace vjx {
@@ -43,6 +45,24 @@ bool csr_validate(int n, int nnz, const int32_t* indptr, const int32_t* indices,
     }
   }
   return true;
+}
+
+static void build_fact_lds(LuSymbolic& s);
+
+// Stable counting sort of v by key(x) in [0, nkeys).
+template <typename Rec, typename Key>
+static void counting_sort(std::vector<Rec>& v, std::vector<Rec>& tmp, int64_t nkeys, Key key) {
+  if (v.size() < 2 || nkeys <= 1) return;
+  std::vector<int64_t> cnt((size_t)nkeys + 1, 0);
+  for (const auto& x : v) cnt[(size_t)key(x) + 1]++;
+  for (int64_t k = 0; k < nkeys; ++k) cnt[(size_t)k + 1] += cnt[(size_t)k];
+  tmp.resize(v.size());
+  for (const auto& x : v) tmp[(size_t)cnt[(size_t)key(x)]++] = x;
+  v.swap(tmp);
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // ---- orderings ----------------------------------------------------------
@@ -468,6 +488,7 @@ static void rows_by_level(const std::vector<int32_t>& level, SegLen seglen, Leve
 static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
                            const double* vals, const AnalyzeOptions& opt, LuSymbolic& s,
                            std::string& err) {
+  const auto t0 = std::chrono::steady_clock::now();
   const int nnz = indptr[n];
   s = LuSymbolic();
   s.n = n;
@@ -494,6 +515,7 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
   } else {
     order_columns(n, indptr, indices, opt.ordering, opt.nd_leaf, s.q, s.n_dense);
   }
+  s.analyze_ms[0] = ms_since(t0);
   s.qinv.assign(n, 0);
   for (int k = 0; k < n; ++k) s.qinv[s.q[k]] = k;
 
@@ -503,6 +525,7 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
     return false;
   s.p.assign(n, 0);
   for (int r = 0; r < n; ++r) s.p[s.pinv[r]] = r;
+  s.analyze_ms[1] = ms_since(t0) - s.analyze_ms[0];
 
   // LU pattern (permuted space) in CSR
   std::vector<int32_t> cnt(n + 1, 0);
@@ -555,6 +578,7 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
       s.src[pos] = eid[d];
     }
 
+  s.analyze_ms[2] = ms_since(t0) - s.analyze_ms[0] - s.analyze_ms[1];
   // CSC view of the LU pattern: column k -> slots, sorted by row
   std::vector<int32_t> ccnt(n + 1, 0);
   for (int64_t e = 0; e < nnz_lu; ++e) ccnt[s.col[e] + 1]++;
@@ -621,7 +645,8 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
       return d0;
     };
     // owned: (task, level in band, target, pivot, L slot, U slot); schur: (target, pivot, L, U)
-    std::vector<std::array<int32_t, 6>> own;
+    std::vector<std::array<int32_t, 6>> own, own_tmp;
+    std::vector<std::array<int32_t, 4>> sch_tmp;
     std::vector<std::array<int32_t, 4>> sch;
     std::vector<int32_t> piv, comps, task_of;
     std::vector<int64_t> cwork;
@@ -629,6 +654,7 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
       const int b1 = std::min(nlev, b0 + W), w = b1 - b0;
       piv.clear();
       for (int l = b0; l < b1; ++l) piv.insert(piv.end(), by_level[l].begin(), by_level[l].end());
+      std::sort(piv.begin(), piv.end());
       for (int k : piv) parent[k] = k;
       for (int k : piv) {
         auto unite = [&](int a) {
@@ -700,8 +726,16 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
         }
       }
       for (int k : piv) comp_id[root(k)] = -1;
-      std::sort(own.begin(), own.end());
-      std::sort(sch.begin(), sch.end());
+      // Pivots were visited in ascending order, so each target's updates are
+      // already ordered by pivot; stable counting sorts group them by target
+      // (owned: then by task and level) without disturbing that order.
+      counting_sort(own, own_tmp, (int64_t)s.col.size(), [](const std::array<int32_t, 6>& a) {
+        return (int64_t)a[2];
+      });
+      counting_sort(own, own_tmp, (int64_t)ntask * w,
+                    [w](const std::array<int32_t, 6>& a) { return (int64_t)a[0] * w + a[1]; });
+      counting_sort(sch, sch_tmp, (int64_t)s.col.size(),
+                    [](const std::array<int32_t, 4>& a) { return (int64_t)a[0]; });
       // emit one target per (task, level, slot) with its contributions by pivot
       auto push_target = [&](int32_t slot) {
         B.items.push_back((int32_t)s.tgt_pos.size());
@@ -840,6 +874,8 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
       s.l_diag.push_back(s.diag[s.col[e]]);
     }
   make_groups(s.fact, opt.wide_threshold);
+  build_fact_lds(s);
+  s.analyze_ms[3] = ms_since(t0) - s.analyze_ms[0] - s.analyze_ms[1] - s.analyze_ms[2];
   return true;
 }
 
@@ -1039,8 +1075,48 @@ static bool check_bands(int n, const BandSchedule& B, Deps deps, std::string& er
   return true;
 }
 
+// LDS form of a solve band: every task fits its band's LDS size, own rows come
+// first, and every local index maps back to the row (column) it stands for.
+template <typename Seg>
+static bool check_tri_lds(const LuSymbolic& s, const BandSchedule& B, Seg seg, std::string& err) {
+  for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
+    if (B.band_lds.empty() || B.band_lds[b] <= 0) continue;
+    const int w = B.band_w[b];
+    for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
+      const int32_t* tk = &B.lds_task[(size_t)t * 4];
+      const int32_t* sl = &B.lds_slots[tk[0]];
+      if (tk[1] > B.band_lds[b] || tk[2] > tk[1]) {
+        err = "solve LDS program: task exceeds its band's LDS size";
+        return false;
+      }
+      const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        const int i = B.items[g];
+        const int32_t* rec = &B.lds_rec[(size_t)g * 4];
+        auto r = seg(i);
+        bool ok = rec[0] < tk[2] && sl[rec[0]] == i && rec[1] == r.first && rec[2] == r.second;
+        for (int e = r.first; ok && e < r.second; ++e) {
+          const int lc = B.lds_col[(size_t)rec[3] + (e - r.first)];
+          ok = lc < tk[1] && sl[lc] == s.col[e];
+        }
+        if (!ok) {
+          err = "solve LDS program: row " + std::to_string(i) + " encoded wrongly";
+          return false;
+        }
+      }
+    }
+  }
+  return true;
+}
+
 bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
   const int n = s.n;
+  if (!check_tri_lds(s, s.fwd_band,
+                     [&](int i) { return std::make_pair((int)s.rowptr[i], (int)s.diag[i]); }, err) ||
+      !check_tri_lds(s, s.bwd_band,
+                     [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); },
+                     err))
+    return false;
   auto fdeps = [&](int i, auto f) {
     for (int e = s.rowptr[i]; e < s.diag[i]; ++e) f(s.col[e]);
   };
@@ -1061,9 +1137,40 @@ bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
   // one task only (tasks of one launch run concurrently).
   const BandSchedule& B = s.fact_band;
   if (B.band_w.empty()) return true;
+  // LDS encoding: every local index maps back to the slot the update reads
+  for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
+    if (s.band_lds[b] <= 0) continue;
+    const int w = B.band_w[b];
+    for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
+      const int32_t* tk = &s.lds_task[(size_t)t * 4];
+      const int32_t* sl = &s.lds_slots[tk[0]];
+      if (tk[1] > s.band_lds[b] || tk[2] > tk[1]) {
+        err = "refactorisation LDS program: task exceeds its band's LDS size";
+        return false;
+      }
+      const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        const int32_t* rec = &s.lds_item[(size_t)g * 4];
+        const int tg = B.items[g];
+        const int u0 = s.tgt_seg[tg];
+        bool ok = rec[0] == tg && rec[2] == s.tgt_seg[tg + 1] - u0 && rec[3] < tk[2] &&
+                  sl[rec[3]] == s.tgt_pos[tg];
+        for (int k = 0; ok && k < rec[2]; ++k) {
+          const uint32_t a = s.lds_upd[(size_t)(rec[1] + k) * 2], d = s.lds_upd[(size_t)(rec[1] + k) * 2 + 1];
+          const int l = (int)(a & 0xffffu), v = (int)(a >> 16);
+          ok = l < tk[1] && v < tk[1] && (int)d < tk[1] && sl[l] == s.upd_l[u0 + k] &&
+               sl[v] == s.upd_u[u0 + k] && sl[d] == s.upd_d[u0 + k];
+        }
+        if (!ok) {
+          err = "refactorisation LDS program: item " + std::to_string(g) + " encoded wrongly";
+          return false;
+        }
+      }
+    }
+  }
   const int64_t nnz = (int64_t)s.col.size();
-  std::vector<int64_t> last_write(nnz, -1);
-  std::vector<int32_t> writer(nnz, -1);
+  std::vector<int64_t> last_write(nnz, -1), task_band_write(nnz, -1);
+  std::vector<int32_t> writer(nnz, -1), task_writer(nnz, -1);
   auto stage = [](int64_t b, int64_t phase, int64_t j) { return (b * 2 + phase) * 128 + j; };
   std::vector<std::pair<int32_t, int64_t>> items;  // (item, stage) in execution order
   std::vector<int32_t> item_task;
@@ -1092,18 +1199,83 @@ bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
     }
     last_write[pos] = std::max(last_write[pos], st);
     writer[pos] = item_task[k];
+    if (item_task[k] >= 0) {
+      task_band_write[pos] = st / 128;
+      task_writer[pos] = item_task[k];
+    }
   }
   for (size_t k = 0; k < items.size(); ++k) {  // reads see final values
     const int t = items[k].first;
     const int64_t st = items[k].second;
     for (int u = s.tgt_seg[t]; u < s.tgt_seg[t + 1]; ++u)
-      for (int32_t slot : {s.upd_l[u], s.upd_u[u], s.upd_d[u]})
+      for (int32_t slot : {s.upd_l[u], s.upd_u[u], s.upd_d[u]}) {
         if (last_write[slot] >= st) {
           err = "refactorisation: an update reads a slot not final at its stage";
           return false;
         }
+        // a slot another task of the same launch writes is not readable
+        // (tasks run concurrently; each holds its own copy in LDS)
+        if (item_task[k] >= 0 && task_band_write[slot] == st / 128 &&
+            task_writer[slot] != item_task[k]) {
+          err = "refactorisation: an update reads a slot another task of its launch writes";
+          return false;
+        }
+      }
   }
   return true;
+}
+
+// LDS form of a solve band program: seg(i) gives row i's off-diagonal entry
+// range; a band runs from LDS only if every one of its tasks fits.
+template <typename Seg>
+static void build_tri_lds(const LuSymbolic& s, Seg seg, BandSchedule& B) {
+  B.lds_rec.assign(B.items.size() * 4, 0);
+  B.lds_col.clear();
+  B.lds_task.clear();
+  B.lds_slots.clear();
+  B.band_lds.clear();
+  std::vector<int32_t> local(s.n, -1), rows;
+  for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
+    const int w = B.band_w[b];
+    int need = 0;
+    for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
+      const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+      rows.clear();
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        local[B.items[g]] = (int32_t)rows.size();
+        rows.push_back(B.items[g]);
+      }
+      const int nown = (int)rows.size();
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        auto r = seg(B.items[g]);
+        for (int e = r.first; e < r.second; ++e) {
+          const int j = s.col[e];
+          if (local[j] < 0) {
+            local[j] = (int32_t)rows.size();
+            rows.push_back(j);
+          }
+        }
+      }
+      const int cnt = (int)rows.size();
+      const bool fits = cnt <= kLdsSlots;
+      need = (need < 0 || !fits) ? -1 : std::max(need, cnt);
+      B.lds_task.insert(B.lds_task.end(), {(int32_t)B.lds_slots.size(), cnt, nown, 0});
+      B.lds_slots.insert(B.lds_slots.end(), rows.begin(), rows.end());
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        const int i = B.items[g];
+        auto r = seg(i);
+        int32_t* rec = &B.lds_rec[(size_t)g * 4];
+        rec[0] = fits ? local[i] : 0;
+        rec[1] = r.first;
+        rec[2] = r.second;
+        rec[3] = (int32_t)B.lds_col.size();
+        for (int e = r.first; e < r.second; ++e)
+          B.lds_col.push_back(fits ? (uint16_t)local[s.col[e]] : (uint16_t)0);
+      }
+      for (int j : rows) local[j] = -1;
+    }
+    B.band_lds.push_back(need > 0 ? need : 0);
+  }
 }
 
 // Level sets and band programs of both solves from the LU pattern.
@@ -1134,6 +1306,10 @@ static void solve_schedules(LuSymbolic& s, const AnalyzeOptions& opt) {
         for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) f(s.col[e]);
       },
       opt.band_width, opt.max_band_tasks, s.bwd_band);
+  build_tri_lds(s, [&](int i) { return std::make_pair((int)s.rowptr[i], (int)s.diag[i]); },
+                s.fwd_band);
+  build_tri_lds(s, [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); },
+                s.bwd_band);
 }
 
 // ---- block-diagonal systems --------------------------------------------------
@@ -1233,6 +1409,67 @@ static void merge_schedule(std::vector<std::vector<int32_t>> (&bk)[3], LevelSche
   }
 }
 
+// LDS form of the refactorisation band tasks (see LuSymbolic::lds_*).  A band
+// runs from LDS only if every one of its tasks fits in kLdsSlots values.
+static void build_fact_lds(LuSymbolic& s) {
+  const BandSchedule& B = s.fact_band;
+  s.lds_item.assign(B.items.size() * 4, 0);
+  s.lds_upd.clear();
+  s.lds_task.clear();
+  s.lds_slots.clear();
+  s.band_lds.clear();
+  std::vector<int32_t> local(s.col.size(), -1), slots;
+  for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
+    const int w = B.band_w[b];
+    int need = 0;
+    for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
+      const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+      // written slots first, then the operands only read
+      slots.clear();
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        const int pos = s.tgt_pos[B.items[g]];
+        if (local[pos] < 0) {
+          local[pos] = (int32_t)slots.size();
+          slots.push_back(pos);
+        }
+      }
+      const int nwrite = (int)slots.size();
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        const int tg = B.items[g];
+        for (int u = s.tgt_seg[tg]; u < s.tgt_seg[tg + 1]; ++u)
+          for (int32_t sl : {s.upd_l[u], s.upd_u[u], s.upd_d[u]})
+            if (local[sl] < 0) {
+              local[sl] = (int32_t)slots.size();
+              slots.push_back(sl);
+            }
+      }
+      const int cnt = (int)slots.size();
+      const bool fits = cnt <= kLdsSlots;
+      need = (need < 0 || !fits) ? -1 : std::max(need, cnt);
+      s.lds_task.insert(s.lds_task.end(), {(int32_t)s.lds_slots.size(), cnt, nwrite, 0});
+      s.lds_slots.insert(s.lds_slots.end(), slots.begin(), slots.end());
+      for (int g = tp[0]; g < tp[w]; ++g) {
+        const int tg = B.items[g];
+        const int u0 = s.tgt_seg[tg], u1 = s.tgt_seg[tg + 1];
+        int32_t* rec = &s.lds_item[(size_t)g * 4];
+        rec[0] = tg;
+        rec[1] = (int32_t)(s.lds_upd.size() / 2);
+        rec[2] = u1 - u0;
+        rec[3] = fits ? local[s.tgt_pos[tg]] : 0;
+        for (int u = u0; u < u1; ++u) {
+          const uint32_t l = fits ? (uint32_t)local[s.upd_l[u]] : 0u;
+          const uint32_t v = fits ? (uint32_t)local[s.upd_u[u]] : 0u;
+          const uint32_t d = fits ? (uint32_t)local[s.upd_d[u]] : 0u;
+          s.lds_upd.push_back(l | (v << 16));
+          s.lds_upd.push_back(d);
+        }
+      }
+      for (int sl : slots) local[sl] = -1;
+    }
+    s.band_lds.push_back(need > 0 ? need : 0);
+  }
+}
+
 // Merge the refactorisation band programs of the diagonal blocks (target ids
 // offset by each block's first target): band b runs every block's band-b
 // tasks in one launch (a narrower block band is padded with empty levels),
@@ -1314,7 +1551,9 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     if (!analyze_single(n, indptr, indices, vals, opt, s, err)) return false;
     s.n_blocks = 1;
     s.n_block_patterns = 1;
+    const auto t1 = std::chrono::steady_clock::now();
     solve_schedules(s, opt);
+    s.analyze_ms[4] = ms_since(t1);
     make_groups(s.fwd, opt.wide_threshold);
     make_groups(s.bwd, opt.wide_threshold);
     return true;
@@ -1445,7 +1684,12 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
   }
   merge_schedule(bf, s.fact);
   merge_fact_bands(fact_parts, s);
+  build_fact_lds(s);
+  for (const auto& b : sym)
+    for (int k = 0; k < 4; ++k) s.analyze_ms[k] += b.analyze_ms[k];
+  const auto t1 = std::chrono::steady_clock::now();
   solve_schedules(s, opt);
+  s.analyze_ms[4] = ms_since(t1);
   make_groups(s.fact, opt.wide_threshold);
   make_groups(s.fwd, opt.wide_threshold);
   make_groups(s.bwd, opt.wide_threshold);

@@ -67,6 +67,9 @@ inline int reduction_class(int len) {
 // and the chunk partials are then added in chunk order.
 constexpr int kHubChunk = 4096;
 constexpr int kMaxBandWidth = 64;  // levels per band (LDS table of one task)
+// LU values a band task may hold in LDS: 60 KiB in FP64, so that with the
+// kernel's own reduction and level tables one workgroup stays within 64 KiB
+constexpr int kLdsSlots = 7680;
 
 // Band program of a triangular solve.  The levels are cut into bands of at
 // most `width` levels; inside a band the rows form connected components
@@ -89,6 +92,15 @@ struct BandSchedule {
   std::vector<int32_t> hub_cptr;             // hub h: chunks [hub_cptr[h], hub_cptr[h+1])
   std::vector<int32_t> hub_cs, hub_ce;       // chunk c covers LU entries [hub_cs[c], hub_ce[c])
   int max_tasks = 0;
+  // LDS form of a solve band (triangular solves only): a task keeps the
+  // vector values of its own rows and of the rows they read in LDS.
+  // Per item: {local row, first entry, end entry, first local column};
+  // lds_col holds the local index of every entry's column.
+  std::vector<int32_t> lds_rec;       // 4 ints per item
+  std::vector<uint16_t> lds_col;
+  std::vector<int32_t> lds_task;      // 4 ints per task: slot offset, count, own rows, 0
+  std::vector<int32_t> lds_slots;     // per task: rows, own rows first
+  std::vector<int32_t> band_lds;      // per band: LDS values needed (0: band runs from HBM)
 };
 
 struct LuSymbolic {
@@ -113,6 +125,16 @@ struct LuSymbolic {
   // fact_band.items[fact_schur_lo[b] .. fact_schur_hi[b]) ordered by class.
   BandSchedule fact_band;
   std::vector<int32_t> fact_schur_lo, fact_schur_hi, fact_schur_heavy, fact_schur_mid;
+  // LDS form of the band tasks: a task whose slots (its targets and every
+  // operand its updates read) fit in kLdsSlots values loads them into LDS,
+  // runs its levels there and writes its targets back.  Per task item
+  // (parallel to fact_band.items over task items; unused for Schur items):
+  // {target id, first packed update, update count, local target index}.
+  std::vector<int32_t> lds_item;      // 4 ints per fact_band item
+  std::vector<uint32_t> lds_upd;      // 2 words per update: (l | u << 16), d
+  std::vector<int32_t> lds_task;      // 4 ints per task: slot offset, slot count, written count, 0
+  std::vector<int32_t> lds_slots;     // per task: global slots, written ones first
+  std::vector<int32_t> band_lds;      // per band: LDS values needed (0: band runs from HBM)
   std::vector<int32_t> l_slot;   // every L slot (i,k) ...
   std::vector<int32_t> l_diag;   // ... and the slot of its pivot U(k,k)
   LevelSchedule fwd;             // items = rows, L solve
@@ -124,6 +146,9 @@ struct LuSymbolic {
   double flops = 0.0;
   int n_dense = 0;
   int n_fact_levels = 0;     // pivot levels of the refactorisation
+  // analysis time (ms): ordering, pivoting, LU pattern, refactorisation
+  // program, solve programs (single-block systems)
+  double analyze_ms[5] = {0, 0, 0, 0, 0};
   int n_blocks = 1;          // diagonal blocks (connected components)
   int n_block_patterns = 1;  // distinct block patterns analysed
   int n_offdiag_pivots = 0;
