@@ -291,9 +291,14 @@ __global__ void k_isrc(int k, const double* __restrict__ isrc, const int32_t* __
 
 constexpr int RB = kReduceLanes;
 
-// All threads of the block call; threads >= RB contribute nothing.
+// All threads of the block call; threads >= RB contribute nothing.  The tree
+// is the halving tree r[t] += r[t + w] for w = 128 .. 1: its first two steps
+// run in LDS in one pass ((r[t] + r[t+128]) + (r[t+64] + r[t+192]) for
+// t < 64), the last six as wavefront shuffles, so the block waits on two
+// barriers instead of nine; the additions and their order are unchanged.
 template <typename T, typename F>
 __device__ __forceinline__ T block_sum(int start, int end, F term, T* red) {
+  static_assert(RB == 256, "tree layout assumes 256 lanes");
   const int t = threadIdx.x;
   if (t < RB) {
     T acc = (T)0;
@@ -301,11 +306,14 @@ __device__ __forceinline__ T block_sum(int start, int end, F term, T* red) {
     red[t] = acc;
   }
   __syncthreads();
-  for (int w = RB / 2; w > 0; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
+  if (t < 64) {
+    T a = (red[t] + red[t + 128]) + (red[t + 64] + red[t + 192]);
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) a += __shfl_down(a, w, 64);
+    if (t == 0) red[RB] = a;
   }
-  const T total = red[0];
+  __syncthreads();
+  const T total = red[RB];
   __syncthreads();
   return total;
 }
@@ -355,7 +363,7 @@ __device__ __forceinline__ double finish_slot(double acc, int s, const int32_t* 
 __global__ void __launch_bounds__(256) k_chunk_sum(const int32_t* __restrict__ cs,
                                                    const int32_t* __restrict__ ce,
                                                    const double* __restrict__ v, double* part) {
-  __shared__ double red[RB];
+  __shared__ double red[RB + 1];
   const int c = blockIdx.x;
   const double s = block_sum<double>(cs[c], ce[c], [&](int u) { return v[u]; }, red);
   if (threadIdx.x == 0) part[c] = s;
@@ -367,7 +375,7 @@ __global__ void __launch_bounds__(256) k_chunk_res(const int32_t* __restrict__ c
                                                    const double* __restrict__ rf,
                                                    const double* __restrict__ rq, double* pf,
                                                    double* pq, double* pm) {
-  __shared__ double red[RB];
+  __shared__ double red[RB + 1];
   const int c = blockIdx.x;
   const double f = block_sum<double>(cs[c], ce[c], [&](int u) { return rf[u]; }, red);
   const double q = block_sum<double>(cs[c], ce[c], [&](int u) { return rq[u]; }, red);
@@ -385,7 +393,7 @@ __global__ void __launch_bounds__(256) k_chunk_spmv(const int32_t* __restrict__ 
                                                     const double* __restrict__ a,
                                                     const int32_t* __restrict__ ci,
                                                     const double* __restrict__ x, double* part) {
-  __shared__ double red[RB];
+  __shared__ double red[RB + 1];
   const int c = blockIdx.x;
   const double s = block_sum<double>(cs[c], ce[c], [&](int e) { return a[e] * x[ci[e]]; }, red);
   if (threadIdx.x == 0) part[c] = s;
@@ -571,7 +579,7 @@ constexpr int kGroupsPerBlock = 256 / kGroupLanes;
 // (a workgroup each), nm medium (kGroupLanes lanes each), then short ones.
 template <typename T>
 __global__ void __launch_bounds__(256) k_lu_wide(int lo, int hi, int nh, int nm, LuProg P, T* lu) {
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   const int b = blockIdx.x;
   const int nmb = (nm + kGroupsPerBlock - 1) / kGroupsPerBlock;
   if (b < nh) {
@@ -593,7 +601,7 @@ __global__ void __launch_bounds__(512) k_lu_narrow(int l0, int l1, const int32_t
                                                    const int32_t* __restrict__ lheavy,
                                                    const int32_t* __restrict__ lmid, LuProg P,
                                                    T* lu) {
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   extern __shared__ int32_t s_lv[];
   const int nl = l1 - l0;
   int32_t* s_hv = s_lv + nl + 1;
@@ -624,7 +632,7 @@ __global__ void __launch_bounds__(512) k_lu_narrow(int l0, int l1, const int32_t
 template <typename T>
 __global__ void __launch_bounds__(256) k_lu_band(int w, int ptr_off, int hm_off, BandProg B,
                                                  LuProg P, T* lu) {
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
   const int t = blockIdx.x;
   const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
@@ -674,7 +682,7 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
                                                      const int32_t* __restrict__ slots, T* lu) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* vals = reinterpret_cast<T*>(lds_raw);
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
   const int t = blockIdx.x;
   const int4 tk = tasks[task0 + t];  // slot offset, slot count, written count
@@ -691,10 +699,49 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
   const LdsTerm<T> term{vals, upd};
   const int lane = (int)threadIdx.x % kGroupLanes;
   const int ngrp = (int)blockDim.x / kGroupLanes;
+  // Software pipeline for each lane's first short item of a level: record two
+  // levels ahead, its (at most kShortSegment = 3) packed updates one level
+  // ahead, so a level waits only on LDS and the barrier.
+  auto first_short = [&](int jj) -> int {
+    if (jj >= w) return -1;
+    const int g = s_ptr[jj] + s_h[jj] + s_m[jj] + (int)threadIdx.x;
+    return g < s_ptr[jj + 1] ? g : -1;
+  };
+  struct Pre {
+    int g;
+    int4 r;
+    uint2 u[3];
+  };
+  auto fetch_rec = [&](Pre& p, int g) {
+    p.g = g;
+    if (g >= 0) p.r = recs[g];
+  };
+  auto fetch_upd = [&](Pre& p) {
+    if (p.g < 0) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k < p.r.z) p.u[k] = upd[p.r.y + k];
+  };
+  Pre cur, nxt, far;
+  fetch_rec(cur, first_short(0));
+  fetch_rec(nxt, first_short(1));
+  fetch_upd(cur);
   for (int j = 0; j < w; ++j) {
     const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
-    if (lo == hi) continue;  // uniform across the block
-    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x) {
+    fetch_rec(far, first_short(j + 2));
+    fetch_upd(nxt);
+    if (cur.g >= 0) {
+      T acc = (T)0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (k < cur.r.z) {
+          const uint2 a = cur.u[k];
+          acc += vals[a.x & 0xffffu] * vals[a.x >> 16] / vals[a.y];
+        }
+      vals[cur.r.w] -= acc;
+    }
+    // further short items of this lane (levels wider than the workgroup)
+    for (int g = lo + nh + nm + (int)threadIdx.x + (int)blockDim.x; g < hi; g += blockDim.x) {
       const int4 r = recs[g];
       T acc = (T)0;
       for (int k = r.y; k < r.y + r.z; ++k) acc += term(k);
@@ -711,6 +758,8 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
       if (threadIdx.x == 0) vals[r.w] -= acc;
     }
     __syncthreads();
+    cur = nxt;
+    nxt = far;
   }
   for (int i = threadIdx.x; i < tk.z; i += blockDim.x) lu[slots[tk.x + i]] = vals[i];
 }
@@ -806,7 +855,7 @@ __device__ __forceinline__ void tri_heavy(int i, const TriProg& P, const T* __re
 template <typename T, bool FWD>
 __global__ void __launch_bounds__(256) k_tri_wide(int lo, int hi, int nh, int nm, TriProg P,
                                                   const T* __restrict__ lu, T* y, T* z) {
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   const int b = blockIdx.x;
   const int nmb = (nm + kGroupsPerBlock - 1) / kGroupsPerBlock;
   if (b < nh) {
@@ -828,7 +877,7 @@ __global__ void __launch_bounds__(512) k_tri_narrow(int l0, int l1, const int32_
                                                     const int32_t* __restrict__ lheavy,
                                                     const int32_t* __restrict__ lmid, TriProg P,
                                                     const T* __restrict__ lu, T* y, T* z) {
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   extern __shared__ int32_t s_lv[];
   const int nl = l1 - l0;
   int32_t* s_hv = s_lv + nl + 1;
@@ -858,7 +907,7 @@ __global__ void __launch_bounds__(512) k_tri_narrow(int l0, int l1, const int32_
 template <typename T, bool FWD>
 __global__ void __launch_bounds__(256) k_tri_band(int w, int ptr_off, int hm_off, BandProg B,
                                                   TriProg P, const T* __restrict__ lu, T* y, T* z) {
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
   const int t = blockIdx.x;
   const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
@@ -907,7 +956,7 @@ __global__ void __launch_bounds__(256) k_tri_band_lds(int w, int ptr_off, int hm
                                                       const T* __restrict__ lu, T* y, T* z) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* v = reinterpret_cast<T*>(lds_raw);
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
   const int t = blockIdx.x;
   const int4 tk = tasks[task0 + t];  // slot offset, slot count, own rows
@@ -933,10 +982,66 @@ __global__ void __launch_bounds__(256) k_tri_band_lds(int w, int ptr_off, int hm
       v[r.x] = (y[i] - acc) / lu[r.y - 1];  // the diagonal precedes the U part
     }
   };
+  // Software pipeline for each lane's first short row of a level: its record
+  // is fetched two levels ahead and its entries (values, local columns, and
+  // for the backward solve the right-hand side and pivot) one level ahead, so
+  // a level waits only on LDS and the barrier.  Short rows have at most
+  // kShortSegment = 3 entries.
+  auto first_short = [&](int jj) -> int {
+    if (jj >= w) return -1;
+    const int g = s_ptr[jj] + s_h[jj] + s_m[jj] + (int)threadIdx.x;
+    return g < s_ptr[jj + 1] ? g : -1;
+  };
+  struct Pre {
+    int g;
+    int4 r;
+    int i;
+    T a[3];
+    int c[3];
+    T rhs, piv;
+  };
+  auto fetch_rec = [&](Pre& p, int g) {
+    p.g = g;
+    if (g >= 0) {
+      p.r = recs[g];
+      if (!FWD) p.i = B.items[g];
+    }
+  };
+  auto fetch_entries = [&](Pre& p) {
+    if (p.g < 0) return;
+    const int len = p.r.z - p.r.y;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k < len) {
+        p.a[k] = lu[p.r.y + k];
+        p.c[k] = lcol[p.r.w + k];
+      }
+    if (!FWD) {
+      p.rhs = y[p.i];
+      p.piv = lu[p.r.y - 1];
+    }
+  };
+  Pre cur, nxt, far;
+  fetch_rec(cur, first_short(0));
+  fetch_rec(nxt, first_short(1));
+  fetch_entries(cur);
   for (int j = 0; j < w; ++j) {
     const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
-    if (lo == hi) continue;  // uniform across the block
-    for (int g = lo + nh + nm + (int)threadIdx.x; g < hi; g += blockDim.x) {
+    fetch_rec(far, first_short(j + 2));
+    fetch_entries(nxt);
+    if (cur.g >= 0) {
+      const int len = cur.r.z - cur.r.y;
+      T acc = (T)0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (k < len) acc += cur.a[k] * v[cur.c[k]];
+      if (FWD)
+        v[cur.r.x] = v[cur.r.x] - acc;
+      else
+        v[cur.r.x] = (cur.rhs - acc) / cur.piv;
+    }
+    // further short rows of this lane (levels wider than the workgroup)
+    for (int g = lo + nh + nm + (int)threadIdx.x + (int)blockDim.x; g < hi; g += blockDim.x) {
       const int4 r = recs[g];
       const TriLdsTerm<T> term{lu, v, lcol, r.w - r.y};
       T acc = (T)0;
@@ -956,6 +1061,8 @@ __global__ void __launch_bounds__(256) k_tri_band_lds(int w, int ptr_off, int hm
       if (threadIdx.x == 0) finish(g, r, acc);
     }
     __syncthreads();
+    cur = nxt;
+    nxt = far;
   }
   T* dst = FWD ? y : z;
   for (int i = threadIdx.x; i < tk.z; i += blockDim.x) dst[slots[tk.x + i]] = v[i];
@@ -968,7 +1075,7 @@ __global__ void __launch_bounds__(256) k_tri_hub_part(const int32_t* __restrict_
                                                       const int32_t* __restrict__ ce, TriProg P,
                                                       const T* __restrict__ lu, const T* y,
                                                       const T* z, T* part) {
-  __shared__ T red[RB];
+  __shared__ T red[RB + 1];
   const int c = blockIdx.x;
   const T s = block_sum<T>(cs[c], ce[c], TriTerm<T>{lu, FWD ? y : z, P.col}, red);
   if (threadIdx.x == 0) part[c] = s;
