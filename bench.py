@@ -42,12 +42,12 @@ def build_circuit(gates: int, seed: int):
     return G.gate_dag(gates, seed=seed)
 
 
-def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None):
+def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None, autotune=True):
     from vajax_amd.engine import AdaptiveConfig
 
     eng.prepare(t_stop=1e-9, dt=1e-12, adaptive_config=AdaptiveConfig(integration_method="gear2"),
                 precision=precision, max_steps=steps + 1, use_graph=use_graph,
-                band_width=band_width)
+                band_width=band_width, autotune=autotune)
     return eng.run_transient()
 
 
@@ -345,6 +345,8 @@ def main():
             "lu_nnz": lu_info["nnz_l"] + lu_info["nnz_u"],
             "lu_levels": lu_info["fact_levels"],
             "solve_bands": [lu_info["fwd_bands"], lu_info["bwd_bands"]],
+            "band_width": lu_info["band_width"],
+            "solve_band_width": lu_info["solve_band_width"],
             "integration": "gear2 adaptive",
             "precision": args.precision,
             "hip_graph": use_graph,

@@ -107,6 +107,8 @@ typedef struct {
   /* analysis time in ms: ordering, pivoting, LU pattern, refactorisation
    * program, solve programs (single-block systems; zero for packed batches) */
   double analyze_ms[5];
+  int32_t band_width, solve_band_width; /* launch plan in use (solve 0: band_width) */
+  double tuned_solve_ms;                /* solve time the autotuner measured (0: not run) */
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
@@ -122,6 +124,11 @@ int vjx_lu_check_schedules(const vjx_pattern* pat);
  * the analysis; the next refactorisation analyses again.  No counterpart in
  * the reference (a launch-plan knob). */
 int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width);
+/* Autotuner of the solves' launch plan: with the matrix data_dev (device),
+ * factorise, then time the triangular solves for band widths 8..48 on the
+ * device and keep the fastest (results do not depend on the width).  No
+ * counterpart in the reference (a launch-plan choice). */
+int vjx_lu_autotune(vjx_pattern* pat, const double* data_dev);
 /* Factorise / solve on the CPU with the same numeric program as the GPU path. */
 int vjx_lu_refactor_cpu(vjx_pattern* pat, const double* data_host, int32_t precision,
                         double* lu_host);
@@ -253,6 +260,7 @@ typedef struct {
   double tran_ft;     /* 0.25 */
   int32_t max_consecutive_rejects; /* 5 */
   const double* v_init_host; /* [n_total] initial node voltages for the DC solve */
+  int32_t autotune;          /* 1: after the symbolic analysis, vjx_lu_autotune */
 } vjx_tran_config;
 
 typedef struct {

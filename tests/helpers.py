@@ -32,7 +32,7 @@ def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, **over):
 
 
 def run_gpu(net, t_stop, dt, fixed=False, tran_fs=None, precision="f64", refine_steps=None,
-            use_graph=False, band_width=None, **over):
+            use_graph=False, band_width=None, autotune=False, **over):
     from vajax_amd.engine import AdaptiveConfig, CircuitEngine
 
     eng = CircuitEngine(net)
@@ -44,7 +44,8 @@ def run_gpu(net, t_stop, dt, fixed=False, tran_fs=None, precision="f64", refine_
     for k, v in over.items():
         setattr(cfg, k, v)
     eng.prepare(t_stop=t_stop, dt=dt, adaptive_config=cfg, precision=precision,
-                refine_steps=refine_steps, use_graph=use_graph, band_width=band_width)
+                refine_steps=refine_steps, use_graph=use_graph, band_width=band_width,
+                autotune=autotune)
     return eng, eng.run_transient()
 
 

@@ -297,3 +297,15 @@ def test_band_programs_match_level_programs(precision):
         assert np.array_equal(r.times, ref.times)
         err = float(np.max(np.abs(r.stats["V_out"] - ref.V)))
         assert err <= tol, err
+
+
+def test_autotuned_solves_give_the_same_waveforms():
+    """The autotuner only changes the solves' band width (launch plan); every
+    row's sum is the same, so the waveforms are bit-identical."""
+    net = G.gate_dag(400, seed=17)
+    eng_a, a = run_gpu(net, 0.05e-9, 1e-12, autotune=True, use_graph=True)
+    eng_b, b = run_gpu(net, 0.05e-9, 1e-12, autotune=False, use_graph=True)
+    ia = eng_a.device.lu.info()
+    assert ia["tuned_solve_ms"] > 0 and ia["solve_band_width"] in (8, 12, 16, 24, 32, 48)
+    assert np.array_equal(a.times, b.times)
+    assert np.array_equal(a.stats["V_out"], b.stats["V_out"])

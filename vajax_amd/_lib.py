@@ -42,6 +42,8 @@ class LuInfo(C.Structure):
         ("fact_task_updates", C.c_int64), ("fact_schur_updates", C.c_int64),
         ("fact_task_targets", C.c_int64), ("fact_schur_targets", C.c_int64),
         ("analyze_ms", C.c_double * 5),
+        ("band_width", C.c_int32), ("solve_band_width", C.c_int32),
+        ("tuned_solve_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -88,6 +90,7 @@ class TranConfig(C.Structure):
         ("tran_fs", C.c_double), ("tran_ft", C.c_double),
         ("max_consecutive_rejects", C.c_int32),
         ("v_init_host", f64p),
+        ("autotune", C.c_int32),
     ]
 
 
@@ -135,6 +138,7 @@ _PROTOS = {
     "vjx_lu_get_pattern": (C.c_int, [vp, i32p, i32p]),
     "vjx_lu_check_schedules": (C.c_int, [vp]),
     "vjx_pattern_set_band_width": (C.c_int, [vp, C.c_int32]),
+    "vjx_lu_autotune": (C.c_int, [vp, vp]),
     "vjx_lu_refactor_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p]),
     "vjx_lu_solve_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p, f64p]),
     "vjx_lu_download": (C.c_int, [vp, f64p]),

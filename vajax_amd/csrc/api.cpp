@@ -124,6 +124,9 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->fwd_hub_rows = (int32_t)s.fwd_band.hub_rows.size();
   o->bwd_hub_rows = (int32_t)s.bwd_band.hub_rows.size();
   for (int k = 0; k < 5; ++k) o->analyze_ms[k] = s.analyze_ms[k];
+  o->band_width = pat->opts.band_width;
+  o->solve_band_width = pat->opts.solve_band_width;
+  o->tuned_solve_ms = pat->tuned_solve_ms;
   o->fact_task_updates = o->fact_schur_updates = o->fact_task_targets = o->fact_schur_targets = 0;
   if (!s.fact_band.band_w.empty()) {
     std::vector<char> schur(s.fact_band.items.size(), 0);
@@ -167,6 +170,19 @@ int vjx_lu_check_schedules(const vjx_pattern* pat) {
     set_error("vjx_lu_check_schedules: " + err);
     return VJX_ERR_STATE;
   }
+  return VJX_OK;
+}
+
+int vjx_lu_autotune(vjx_pattern* pat, const double* data_dev) {
+  if (!pat || (!data_dev && pat->nnz)) {
+    set_error("vjx_lu_autotune: null argument");
+    return VJX_ERR_ARG;
+  }
+  int rc = pattern_ensure_analyzed(pat, data_dev);
+  if (rc < 0) return rc;
+  rc = pattern_autotune_solves(pat, data_dev);
+  if (rc < 0) return rc;
+  VJX_TRY_HIP(hipStreamSynchronize(pat->stream));
   return VJX_OK;
 }
 

@@ -21,6 +21,7 @@ struct vjx_pattern {
   vjx::DevStatus* h_status = nullptr;  // pinned
   double analyze_seconds = 0.0;
   int n_analyses = 0;
+  double tuned_solve_ms = 0.0;  // solve time at the width the autotuner kept
   ~vjx_pattern() {
     if (h_status) (void)hipHostFree(h_status);
   }
@@ -52,6 +53,8 @@ int pattern_ensure_analyzed(vjx_pattern* pat, const double* data_dev);
 int pattern_upload(vjx_pattern* pat);
 int pattern_refactor_async(vjx_pattern* pat, const double* data_dev, int prec, bool reset_status);
 int pattern_check_pivots(vjx_pattern* pat);
+// Pick the solves' band width by timing candidates on the device.
+int pattern_autotune_solves(vjx_pattern* pat, const double* data_dev);
 // Solve A x = scale*rhs (scale = +1 or -1) with the current factors; for FP32
 // factors run refine_steps FP64 residual corrections.
 int pattern_solve_async(vjx_pattern* pat, const double* data_dev, const double* rhs, double scale,

@@ -223,6 +223,72 @@ int pattern_refactor_async(vjx_pattern* pat, const double* data_dev, int prec, b
   return VJX_OK;
 }
 
+// Autotuner of the solve launch plan: rebuilds the two solves' band programs
+// for each candidate band width, times the solves on the device with HIP
+// events and keeps the fastest.  Results do not depend on the width (each
+// row's sum is fixed), only the number of launches and the work per task.
+int pattern_autotune_solves(vjx_pattern* pat, const double* data_dev) {
+  DevLU& L = pat->lu;
+  if (!L.analyzed || pat->opts.band_width <= 0) return VJX_OK;
+  int rc = pattern_ensure_analyzed(pat, data_dev);
+  if (rc < 0) return rc;
+  hipStream_t s = pat->stream;
+  const int prec = L.factored_prec < 0 ? VJX_PREC_F64 : L.factored_prec;
+  rc = pattern_refactor_async(pat, data_dev, prec, true);
+  if (rc < 0) return rc;
+  const int n = pat->n;
+  VJX_TRY_HIP(hipMemsetAsync(L.r64.p, 0, sizeof(double) * n, s));
+  hipEvent_t ev[2];
+  VJX_TRY_HIP(hipEventCreate(&ev[0]));
+  VJX_TRY_HIP(hipEventCreate(&ev[1]));
+  auto solve = [&]() {
+    if (prec == VJX_PREC_F32)
+      launch_lu_solve<float>(L, L.lu32.p, L.r64.p, 1.0, L.x64.p, L.y32.p, L.z32.p, s);
+    else
+      launch_lu_solve<double>(L, L.lu64.p, L.r64.p, 1.0, L.x64.p, L.y64.p, L.z64.p, s);
+  };
+  const int candidates[] = {8, 12, 16, 24, 32, 48};
+  int best = 0;
+  float best_ms = 0.f;
+  rc = VJX_OK;
+  for (int w : candidates) {
+    pat->opts.solve_band_width = w;
+    lu_rebuild_solve_programs(L.sym, pat->opts);
+    hipError_t e = upload_band(L.fwd_band, L.sym.fwd_band);
+    if (e == hipSuccess) e = upload_band(L.bwd_band, L.sym.bwd_band);
+    if (e != hipSuccess) {
+      set_error(std::string("vjx_lu_autotune: upload failed: ") + hipGetErrorString(e));
+      rc = VJX_ERR_HIP;
+      break;
+    }
+    solve();  // warm-up
+    (void)hipEventRecord(ev[0], s);
+    for (int r = 0; r < 5; ++r) solve();
+    (void)hipEventRecord(ev[1], s);
+    e = hipEventSynchronize(ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (e != hipSuccess) {
+      set_error(std::string("vjx_lu_autotune: timing failed: ") + hipGetErrorString(e));
+      rc = VJX_ERR_HIP;
+      break;
+    }
+    if (best == 0 || ms < best_ms) {
+      best = w;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc < 0) return rc;
+  pat->opts.solve_band_width = best;
+  lu_rebuild_solve_programs(L.sym, pat->opts);
+  VJX_TRY_HIP(upload_band(L.fwd_band, L.sym.fwd_band));
+  VJX_TRY_HIP(upload_band(L.bwd_band, L.sym.bwd_band));
+  pat->tuned_solve_ms = best_ms / 5.0;
+  return VJX_OK;
+}
+
 int pattern_check_pivots(vjx_pattern* pat) {
   VJX_TRY_HIP(hipMemcpy(pat->h_status, pat->d_status.p, sizeof(DevStatus), hipMemcpyDeviceToHost));
   double maxl;

@@ -1281,6 +1281,9 @@ static void build_tri_lds(const LuSymbolic& s, Seg seg, BandSchedule& B) {
 // Level sets and band programs of both solves from the LU pattern.
 static void solve_schedules(LuSymbolic& s, const AnalyzeOptions& opt) {
   const int n = s.n;
+  // the solves follow the factors' band width unless given their own
+  const int sw = opt.band_width > 0 && opt.solve_band_width > 0 ? opt.solve_band_width
+                                                                 : opt.band_width;
   std::vector<int32_t> lf(n, 0), lb(n, 0);
   for (int i = 0; i < n; ++i) {
     int lv = 0;
@@ -1299,17 +1302,23 @@ static void solve_schedules(LuSymbolic& s, const AnalyzeOptions& opt) {
       [&](int i, auto f) {
         for (int e = s.rowptr[i]; e < s.diag[i]; ++e) f(s.col[e]);
       },
-      opt.band_width, opt.max_band_tasks, s.fwd_band);
+      sw, opt.max_band_tasks, s.fwd_band);
   build_bands(
       lb, [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); },
       [&](int i, auto f) {
         for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) f(s.col[e]);
       },
-      opt.band_width, opt.max_band_tasks, s.bwd_band);
+      sw, opt.max_band_tasks, s.bwd_band);
   build_tri_lds(s, [&](int i) { return std::make_pair((int)s.rowptr[i], (int)s.diag[i]); },
                 s.fwd_band);
   build_tri_lds(s, [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); },
                 s.bwd_band);
+}
+
+void lu_rebuild_solve_programs(LuSymbolic& s, const AnalyzeOptions& opt) {
+  solve_schedules(s, opt);
+  make_groups(s.fwd, opt.wide_threshold);
+  make_groups(s.bwd, opt.wide_threshold);
 }
 
 // ---- block-diagonal systems --------------------------------------------------

@@ -160,9 +160,14 @@ struct AnalyzeOptions {
   int wide_threshold = 2048;  // levels with at least this many items get a grid
   int ordering = 2;           // 0 = minimum degree, 1 = natural, 2 = nested dissection
   int nd_leaf = 48;           // nested-dissection leaf size (min-degree inside)
-  int band_width = 16;        // solve bands of this many levels (0: one launch per level)
+  int band_width = 16;        // bands of this many levels (0: one launch per level)
+  int solve_band_width = 0;   // solve bands (0: band_width)
   int max_band_tasks = 2048;  // workgroups per band launch
 };
+
+// Rebuild the level sets and band programs of the two solves (e.g. for a
+// new opt.solve_band_width); the factors' program is untouched.
+void lu_rebuild_solve_programs(LuSymbolic& s, const AnalyzeOptions& opt);
 
 // Full analysis from a CSR matrix (host arrays).  Returns false and sets err on
 // malformed input.  Values are used only to choose pivots.

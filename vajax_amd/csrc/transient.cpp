@@ -142,7 +142,9 @@ struct Driver {
                         w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, s);
     VJX_TRY_HIP(hipGetLastError());
     VJX_TRY_HIP(hipStreamSynchronize(s));
-    return pattern_ensure_analyzed(pat, w.csr.p);
+    int rc = pattern_ensure_analyzed(pat, w.csr.p);
+    if (rc < 0 || !cfg->autotune) return rc;
+    return pattern_autotune_solves(pat, w.csr.p);
   }
 
   // One Newton iteration (solver_factories.py:230-325) on Xw, kernels only

@@ -297,7 +297,7 @@ class CircuitEngine:
                 adaptive_config: Optional[AdaptiveConfig] = None, precision: str = "f64",
                 refine_steps: Optional[int] = None, max_steps: Optional[int] = None,
                 use_graph: bool = False, band_width: Optional[int] = None,
-                **_ignored) -> None:
+                autotune: bool = False, **_ignored) -> None:
         if self.topology is None:
             self.parse()
         if band_width is not None:
@@ -330,7 +330,8 @@ class CircuitEngine:
             raise ValueError("precision must be 'f64' or 'f32'")
         self._prep = dict(t_stop=float(t_stop), dt=float(dt), cfg=cfg, precision=precision,
                           refine_steps=(1 if precision == "f32" else 0) if refine_steps is None
-                          else int(refine_steps), max_steps=int(max_steps), use_graph=use_graph)
+                          else int(refine_steps), max_steps=int(max_steps), use_graph=use_graph,
+                          autotune=bool(autotune))
         self._prepared = True
 
     def _tran_config(self) -> L.TranConfig:
@@ -348,6 +349,7 @@ class CircuitEngine:
         tc.refine_steps = p["refine_steps"]
         tc.tran_itl = o.tran_itl
         tc.use_graph = 1 if p["use_graph"] else 0
+        tc.autotune = 1 if p.get("autotune") else 0
         tc.reltol = o.reltol
         tc.abstol = o.abstol
         tc.vntol = o.vntol
