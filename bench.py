@@ -106,23 +106,30 @@ KERNEL_NAMES = {"eval": "k_eval<MOSFET> (+R/C)", "stamp": "k_chunk_sum/k_reduce_
                 "solve_hub": "k_tri_hub_part/k_tri_hub_finish"}
 
 
-# kernel class -> kernel name in the rocprofv3 traffic summaries
-TRAFFIC_KERNEL = {"eval": "k_eval<2>", "stamp": "k_reduce_jac", "lu_gather": "k_lu_gather<double>",
-                  "lu_tasks": "k_lu_band<double>", "lu_schur": "k_lu_wide<double>",
-                  "lu_scale": "k_lu_scale<double>", "solve_tasks": "k_tri_band<double, true>"}
+# kernel class -> kernel names in the rocprofv3 traffic summaries
+TRAFFIC_KERNELS = {
+    "eval": ["k_eval<2>"], "stamp": ["k_reduce_jac", "k_residual", "k_chunk_sum", "k_chunk_res"],
+    "lu_gather": ["k_lu_gather<double>"], "lu_tasks": ["k_lu_band_lds<double>", "k_lu_band<double>"],
+    "lu_schur": ["k_lu_wide<double>"], "lu_scale": ["k_lu_scale<double>"],
+    "solve_tasks": ["k_tri_band_lds<double, true>", "k_tri_band_lds<double, false>",
+                    "k_tri_band<double, true>", "k_tri_band<double, false>"]}
 
 
 def measured_traffic(args, cls):
-    """HBM bytes per dispatch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the
-    copy-kernel calibration), or None when no summary covers it."""
+    """HBM bytes per dispatch of a kernel class (dispatch-weighted mean over
+    its kernels) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload, scaled by the copy-kernel calibration; None when no
+    summary covers it."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_gate_dag_{args.gates}.json")
     if args.precision != "f64" or args.seed != 0 or not os.path.exists(path):
         return None
     with open(path) as fh:
-        data = json.load(fh)
-    k = data["kernels"].get(TRAFFIC_KERNEL.get(cls, ""))
-    return int(k["traffic_bytes"]) if k else None
+        data = json.load(fh)["kernels"]
+    ks = [data[k] for k in TRAFFIC_KERNELS.get(cls, []) if k in data]
+    n = sum(k["dispatches"] for k in ks)
+    if n == 0:
+        return None
+    return int(sum(k["traffic_bytes"] * k["dispatches"] for k in ks) / n)
 
 
 def cpu_baseline(net, steps: int):
