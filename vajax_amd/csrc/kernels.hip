@@ -296,26 +296,44 @@ constexpr int RB = kReduceLanes;
 // run in LDS in one pass ((r[t] + r[t+128]) + (r[t+64] + r[t+192]) for
 // t < 64), the last six as wavefront shuffles, so the block waits on two
 // barriers instead of nine; the additions and their order are unchanged.
-template <typename T, typename F>
-__device__ __forceinline__ T block_sum(int start, int end, F term, T* red) {
+// Workgroup barrier for data exchanged through LDS only: waits for this
+// wave's LDS operations (lgkmcnt(0); s_waitcnt immediate 0xC07F leaves
+// vmcnt and expcnt at their maxima in the gfx9 encoding), not for its
+// outstanding global loads, so prefetches stay in flight across it.  The
+// empty asm statements keep the compiler from moving memory accesses
+// across the barrier.  Not for data exchanged through global memory.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// The tree part of block_sum: lane t (< RB) holds its strided partial.
+template <typename T>
+__device__ __forceinline__ T block_tree(T acc, T* red) {
   static_assert(RB == 256, "tree layout assumes 256 lanes");
   const int t = threadIdx.x;
-  if (t < RB) {
-    T acc = (T)0;
-    for (int u = start + t; u < end; u += RB) acc += term(u);
-    red[t] = acc;
-  }
-  __syncthreads();
+  if (t < RB) red[t] = acc;
+  lds_barrier();
   if (t < 64) {
     T a = (red[t] + red[t + 128]) + (red[t + 64] + red[t + 192]);
 #pragma unroll
     for (int w = 32; w > 0; w >>= 1) a += __shfl_down(a, w, 64);
     if (t == 0) red[RB] = a;
   }
-  __syncthreads();
+  lds_barrier();
   const T total = red[RB];
-  __syncthreads();
+  lds_barrier();
   return total;
+}
+
+template <typename T, typename F>
+__device__ __forceinline__ T block_sum(int start, int end, F term, T* red) {
+  T acc = (T)0;
+  if ((int)threadIdx.x < RB)
+    for (int u = start + (int)threadIdx.x; u < end; u += RB) acc += term(u);
+  return block_tree<T>(acc, red);
 }
 
 // G consecutive lanes (G divides the wavefront): strided partials, then the
@@ -757,10 +775,11 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
       const T acc = block_sum<T>(r.y, r.y + r.z, term, red);
       if (threadIdx.x == 0) vals[r.w] -= acc;
     }
-    __syncthreads();
+    lds_barrier();  // the levels exchange values through LDS only
     cur = nxt;
     nxt = far;
   }
+  __syncthreads();
   for (int i = threadIdx.x; i < tk.z; i += blockDim.x) lu[slots[tk.x + i]] = vals[i];
 }
 
@@ -982,87 +1001,134 @@ __global__ void __launch_bounds__(256) k_tri_band_lds(int w, int ptr_off, int hm
       v[r.x] = (y[i] - acc) / lu[r.y - 1];  // the diagonal precedes the U part
     }
   };
-  // Software pipeline for each lane's first short row of a level: its record
-  // is fetched two levels ahead and its entries (values, local columns, and
-  // for the backward solve the right-hand side and pivot) one level ahead, so
-  // a level waits only on LDS and the barrier.  Short rows have at most
-  // kShortSegment = 3 entries.
-  auto first_short = [&](int jj) -> int {
+  // Software pipeline: for each lane's first short row, its group's first
+  // medium row and the level's first long row, the record is fetched two
+  // levels ahead and the lane's entries (values and local columns; for the
+  // backward solve also the right-hand side and pivot) one level ahead, so a
+  // level mostly waits on LDS and barriers.  A lane's entries of a row are
+  // those its fixed-order sum visits: all of a short row (<= 3), every 16th
+  // of a medium row (<= 128 entries, so <= 8), every 256th of a long row
+  // (the first two; the rest are loaded in the level).
+  const int grp0 = (int)threadIdx.x / kGroupLanes;
+  auto row_of = [&](int kind, int jj) -> int {
     if (jj >= w) return -1;
-    const int g = s_ptr[jj] + s_h[jj] + s_m[jj] + (int)threadIdx.x;
-    return g < s_ptr[jj + 1] ? g : -1;
+    const int lo = s_ptr[jj], nh = s_h[jj], nm = s_m[jj];
+    if (kind == 0) {
+      const int g = lo + nh + nm + (int)threadIdx.x;
+      return g < s_ptr[jj + 1] ? g : -1;
+    }
+    if (kind == 1) return grp0 < nm ? lo + nh + grp0 : -1;
+    return nh > 0 ? lo : -1;
   };
   struct Pre {
     int g;
     int4 r;
     int i;
-    T a[3];
-    int c[3];
+    T a[8];
+    int c[8];
     T rhs, piv;
   };
-  auto fetch_rec = [&](Pre& p, int g) {
-    p.g = g;
-    if (g >= 0) {
-      p.r = recs[g];
-      if (!FWD) p.i = B.items[g];
+  auto fetch_rec = [&](Pre& p, int kind, int jj) {
+    p.g = row_of(kind, jj);
+    if (p.g >= 0) {
+      p.r = recs[p.g];
+      if (!FWD) p.i = B.items[p.g];
     }
   };
-  auto fetch_entries = [&](Pre& p) {
+  // lane offset / stride / count of the lane's entries for each kind
+  auto fetch_entries = [&](Pre& p, int off, int stride, int cnt) {
     if (p.g < 0) return;
-    const int len = p.r.z - p.r.y;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (k < len) {
-        p.a[k] = lu[p.r.y + k];
-        p.c[k] = lcol[p.r.w + k];
+    for (int k = 0; k < 8; ++k) {
+      const int e = p.r.y + off + stride * k;
+      if (k < cnt && e < p.r.z) {
+        p.a[k] = lu[e];
+        p.c[k] = lcol[p.r.w + (e - p.r.y)];
       }
+    }
     if (!FWD) {
       p.rhs = y[p.i];
       p.piv = lu[p.r.y - 1];
     }
   };
-  Pre cur, nxt, far;
-  fetch_rec(cur, first_short(0));
-  fetch_rec(nxt, first_short(1));
-  fetch_entries(cur);
+  auto part = [&](const Pre& p, int off, int stride, int cnt) {
+    T acc = (T)0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int e = p.r.y + off + stride * k;
+      if (k < cnt && e < p.r.z) acc += p.a[k] * v[p.c[k]];
+    }
+    return acc;
+  };
+  auto finish_pre = [&](const Pre& p, T acc) {
+    if (FWD)
+      v[p.r.x] = v[p.r.x] - acc;
+    else
+      v[p.r.x] = (p.rhs - acc) / p.piv;
+  };
+  const int tid = (int)threadIdx.x;
+  Pre s_cur, s_nxt, s_far, m_cur, m_nxt, m_far, h_cur, h_nxt, h_far;
+  fetch_rec(s_cur, 0, 0);
+  fetch_rec(m_cur, 1, 0);
+  fetch_rec(h_cur, 2, 0);
+  fetch_rec(s_nxt, 0, 1);
+  fetch_rec(m_nxt, 1, 1);
+  fetch_rec(h_nxt, 2, 1);
+  fetch_entries(s_cur, 0, 1, 3);
+  fetch_entries(m_cur, lane, kGroupLanes, 8);
+  fetch_entries(h_cur, tid, RB, 2);
   for (int j = 0; j < w; ++j) {
     const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
-    fetch_rec(far, first_short(j + 2));
-    fetch_entries(nxt);
-    if (cur.g >= 0) {
-      const int len = cur.r.z - cur.r.y;
-      T acc = (T)0;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (k < len) acc += cur.a[k] * v[cur.c[k]];
-      if (FWD)
-        v[cur.r.x] = v[cur.r.x] - acc;
-      else
-        v[cur.r.x] = (cur.rhs - acc) / cur.piv;
-    }
+    fetch_rec(s_far, 0, j + 2);
+    fetch_rec(m_far, 1, j + 2);
+    fetch_rec(h_far, 2, j + 2);
+    fetch_entries(s_nxt, 0, 1, 3);
+    fetch_entries(m_nxt, lane, kGroupLanes, 8);
+    fetch_entries(h_nxt, tid, RB, 2);
+    if (s_cur.g >= 0) finish_pre(s_cur, part(s_cur, 0, 1, 3));
     // further short rows of this lane (levels wider than the workgroup)
-    for (int g = lo + nh + nm + (int)threadIdx.x + (int)blockDim.x; g < hi; g += blockDim.x) {
+    for (int g = lo + nh + nm + tid + (int)blockDim.x; g < hi; g += blockDim.x) {
       const int4 r = recs[g];
       const TriLdsTerm<T> term{lu, v, lcol, r.w - r.y};
       T acc = (T)0;
       for (int e = r.y; e < r.z; ++e) acc += term(e);
       finish(g, r, acc);
     }
-    for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp) {
+    // medium rows: the group's first from the pipeline, the rest direct
+    if (grp0 < nm) {
+      T acc = part(m_cur, lane, kGroupLanes, 8);
+#pragma unroll
+      for (int s = kGroupLanes / 2; s > 0; s >>= 1) acc += __shfl_down(acc, s, kGroupLanes);
+      if (lane == 0) finish_pre(m_cur, acc);
+    }
+    for (int grp = grp0 + ngrp; grp < nm; grp += ngrp) {
       const int g = lo + nh + grp;
       const int4 r = recs[g];
       const T acc = group_sum<kGroupLanes, T>(r.y, r.z, lane, TriLdsTerm<T>{lu, v, lcol, r.w - r.y});
       if (lane == 0) finish(g, r, acc);
     }
-    for (int h = 0; h < nh; ++h) {
+    // long rows: the first from the pipeline (two entries per lane, the rest
+    // of its strided sum loaded here), the others direct
+    if (nh > 0) {
+      T acc = part(h_cur, tid, RB, 2);
+      const TriLdsTerm<T> term{lu, v, lcol, h_cur.r.w - h_cur.r.y};
+      for (int e = h_cur.r.y + tid + 2 * RB; e < h_cur.r.z; e += RB) acc += term(e);
+      acc = block_tree<T>(acc, red);
+      if (tid == 0) finish_pre(h_cur, acc);
+    }
+    for (int h = 1; h < nh; ++h) {
       const int g = lo + h;
       const int4 r = recs[g];
       const T acc = block_sum<T>(r.y, r.z, TriLdsTerm<T>{lu, v, lcol, r.w - r.y}, red);
-      if (threadIdx.x == 0) finish(g, r, acc);
+      if (tid == 0) finish(g, r, acc);
     }
-    __syncthreads();
-    cur = nxt;
-    nxt = far;
+    lds_barrier();  // the levels exchange values through LDS only
+    s_cur = s_nxt;
+    s_nxt = s_far;
+    m_cur = m_nxt;
+    m_nxt = m_far;
+    h_cur = h_nxt;
+    h_nxt = h_far;
   }
   T* dst = FWD ? y : z;
   for (int i = threadIdx.x; i < tk.z; i += blockDim.x) dst[slots[tk.x + i]] = v[i];
