@@ -18,7 +18,7 @@ def oracle_configs(net, **over):
     return cfg, opts
 
 
-def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, **over):
+def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, max_steps=None, **over):
     cir = omna.setup(net)
     cfg, opts = oracle_configs(net, **over)
     if tran_fs is not None:
@@ -27,12 +27,12 @@ def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, **over):
         out = otr.run_fixed(cir, t_stop, dt, cfg.integration_method, opts, tran_fs=cfg.tran_fs,
                             dense=dense)
     else:
-        out = otr.run_adaptive(cir, t_stop, dt, cfg, opts, dense=dense)
+        out = otr.run_adaptive(cir, t_stop, dt, cfg, opts, dense=dense, max_steps=max_steps)
     return cir, out
 
 
 def run_gpu(net, t_stop, dt, fixed=False, tran_fs=None, precision="f64", refine_steps=None,
-            use_graph=False, band_width=None, autotune=False, **over):
+            use_graph=False, band_width=None, autotune=False, max_steps=None, **over):
     from vajax_amd.engine import AdaptiveConfig, CircuitEngine
 
     eng = CircuitEngine(net)
@@ -45,7 +45,7 @@ def run_gpu(net, t_stop, dt, fixed=False, tran_fs=None, precision="f64", refine_
         setattr(cfg, k, v)
     eng.prepare(t_stop=t_stop, dt=dt, adaptive_config=cfg, precision=precision,
                 refine_steps=refine_steps, use_graph=use_graph, band_width=band_width,
-                autotune=autotune)
+                autotune=autotune, max_steps=max_steps)
     return eng, eng.run_transient()
 
 
