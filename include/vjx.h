@@ -261,6 +261,7 @@ typedef struct {
   int32_t max_consecutive_rejects; /* 5 */
   const double* v_init_host; /* [n_total] initial node voltages for the DC solve */
   int32_t autotune;          /* 1: after the symbolic analysis, vjx_lu_autotune */
+  double repivot_growth;     /* re-analyse when max |L| exceeds this (0: 1e12) */
 } vjx_tran_config;
 
 typedef struct {

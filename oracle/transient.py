@@ -299,6 +299,27 @@ def run_adaptive(cir, t_stop: float, dt: float, cfg: AdaptiveConfig, opts: Optio
             break
         iters += 1
         dt_cur = dt
+        if dt_cur <= 0.0:
+            # full_mna.py:2076 clamps the next dt to t_stop - t_next also after a
+            # rejected step, so rejecting a step that ends on t_stop leaves dt = 0.
+            # The reference then builds with 1/dt = inf: every Newton iteration is
+            # non-finite, the step is a Newton failure at min_dt, and it is
+            # force-accepted with the previous solution and zero source currents
+            # (full_mna.py:2059-2085).  Restated without the non-finite arithmetic.
+            total_nr += opts.tran_itl
+            consec = 0
+            last_good = dt_hist[0] if hist_count > 0 else cfg.min_dt * 16.0
+            new_dt = max(last_good / 8.0, cfg.min_dt * 2.0)
+            new_dt = min(max(new_dt, cfg.min_dt), cfg.max_dt)
+            new_dt = min(new_dt, t_stop - t)
+            times.append(t)
+            Vout.append(X[:n_total].copy())
+            Iout.append(np.zeros(max(m, 1)))
+            step_idx += 1
+            min_dt_used = min(min_dt_used, dt_cur)
+            max_dt_used = max(max_dt_used, dt_cur)
+            dt = new_dt
+            continue
         c0, c1, d1, c2, e_int = integration_coeffs(method, dt_cur, dt_hist[0])
         t_next = t + dt_cur
         vs = source_values(cir.net.vsources, t_next)

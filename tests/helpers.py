@@ -18,9 +18,12 @@ def oracle_configs(net, **over):
     return cfg, opts
 
 
-def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, max_steps=None, **over):
+def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, max_steps=None,
+               tran_itl=None, **over):
     cir = omna.setup(net)
     cfg, opts = oracle_configs(net, **over)
+    if tran_itl is not None:
+        opts.tran_itl = int(tran_itl)
     if tran_fs is not None:
         cfg.tran_fs = tran_fs
     if fixed:
@@ -32,12 +35,17 @@ def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, max_step
 
 
 def run_gpu(net, t_stop, dt, fixed=False, tran_fs=None, precision="f64", refine_steps=None,
-            use_graph=False, band_width=None, autotune=False, max_steps=None, **over):
+            use_graph=False, band_width=None, autotune=False, max_steps=None, tran_itl=None,
+            repivot_growth=None, **over):
     from vajax_amd.engine import AdaptiveConfig, CircuitEngine
 
     eng = CircuitEngine(net)
     eng.parse()
     eng.options.fixed_step_transient = fixed
+    if tran_itl is not None:
+        eng.options.tran_itl = int(tran_itl)
+    if repivot_growth is not None:
+        eng.options.repivot_growth = float(repivot_growth)
     cfg = AdaptiveConfig(integration_method=str(net.options.get("tran_method", "trap")))
     if tran_fs is not None:
         cfg.tran_fs = tran_fs

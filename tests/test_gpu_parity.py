@@ -309,3 +309,38 @@ def test_autotuned_solves_give_the_same_waveforms():
     assert ia["tuned_solve_ms"] > 0 and ia["solve_band_width"] in (8, 12, 16, 24, 32, 48)
     assert np.array_equal(a.times, b.times)
     assert np.array_equal(a.stats["V_out"], b.stats["V_out"])
+
+
+def test_repivot_path_keeps_parity():
+    """Force the pivot-growth monitor from the host (growth limit 1e-300):
+    every Newton iteration re-runs the symbolic analysis on its matrix (up to
+    the library's cap of 64), drops the captured graph and redoes the
+    iteration from the saved iterate; the waveform must not change."""
+    net = G.gate_dag(150, seed=23)
+    _, ref = run_oracle(net, 40e-12, 1e-12, fixed=True, tran_fs=1.0)
+    _, res = run_gpu(net, 40e-12, 1e-12, fixed=True, tran_fs=1.0, use_graph=True,
+                     repivot_growth=1e-300)
+    assert res.stats["repivots"] == 64
+    assert np.array_equal(res.times, ref.times)
+    err = float(np.max(np.abs(res.stats["V_out"] - ref.V)))
+    assert err <= 1e-9, err
+
+
+def test_newton_failure_path_matches_oracle():
+    """Newton capped at 2 iterations: failed steps are retried at dt/4 and
+    the step control must make the same decisions as the oracle.  This case
+    also rejects a step that ends on t_stop, which leaves dt = 0 and a
+    force-accepted repeat of the previous time point (full_mna.py:2076)."""
+    net = G.gate_dag(150, seed=29)
+    _, ref = run_oracle(net, 0.1e-9, 1e-12, tran_itl=2)
+    _, res = run_gpu(net, 0.1e-9, 1e-12, tran_itl=2)
+    assert ref.stats["rejected_steps"] > 0
+    assert ref.stats["min_dt_used"] == 0.0
+    assert res.stats["rejected_steps"] == ref.stats["rejected_steps"]
+    assert res.stats["min_dt_used"] == 0.0
+    n = min(len(ref.times), len(res.times))
+    same = np.isclose(res.times[:n], ref.times[:n], rtol=1e-9, atol=0)
+    k = n if same.all() else int(np.argmin(same))
+    assert k >= 0.3 * len(ref.times), (k, len(ref.times))
+    err = float(np.max(np.abs(res.stats["V_out"][:k] - ref.V[:k])))
+    assert err <= 1e-6, err

@@ -91,6 +91,7 @@ class TranConfig(C.Structure):
         ("max_consecutive_rejects", C.c_int32),
         ("v_init_host", f64p),
         ("autotune", C.c_int32),
+        ("repivot_growth", C.c_double),
     ]
 
 

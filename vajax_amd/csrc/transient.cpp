@@ -236,7 +236,8 @@ struct Driver {
       // pivot monitor: re-pivot on this matrix and redo the iteration from
       // the saved iterate
       const double maxl = bits_to_double(pat->h_status->maxl_bits);
-      if ((pat->h_status->nonfinite > 0 || maxl > 1e12) && repivots < 64) {
+      const double growth = cfg->repivot_growth > 0.0 ? cfg->repivot_growth : 1e12;
+      if ((pat->h_status->nonfinite > 0 || maxl > growth) && repivots < 64) {
         std::vector<double> h(c->nse);
         VJX_TRY_HIP(hipMemcpy(h.data(), w.csr.p, sizeof(double) * c->nse, hipMemcpyDeviceToHost));
         rc = vjx_lu_analyze(pat, h.data());
@@ -446,6 +447,29 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     auto hist_ptr = [&](int j) { return w.hist.p + (size_t)slot[j] * n_total; };
     while (t < cfg->t_stop && step_idx < cfg->max_steps) {
       const double dt_cur = dt;
+      if (!(dt_cur > 0.0)) {
+        // full_mna.py:2076 clamps the next dt to t_stop - t_next also after a
+        // rejected step, so rejecting a step that ends on t_stop leaves dt = 0.
+        // The reference then builds with 1/dt = inf: every Newton iteration is
+        // non-finite, the step is a Newton failure at min_dt, and it is
+        // force-accepted with the previous solution and zero source currents
+        // (full_mna.py:2059-2085).  Restated without the non-finite arithmetic,
+        // which would otherwise trip the re-pivot monitor.
+        total_nr += cfg->tran_itl;
+        consec = 0;
+        const double last_good = hist_count > 0 ? dt_hist[0] : cfg->min_dt * 16.0;
+        double new_dt = std::max(last_good / 8.0, cfg->min_dt * 2.0);
+        new_dt = std::min(std::max(new_dt, cfg->min_dt), max_dt);
+        new_dt = std::min(new_dt, cfg->t_stop - t);
+        times[step_idx] = t;
+        launch_output(n_total, m, w.Xa.p, w.Ivs.p, 1, w.vout.p + (size_t)step_idx * n_total,
+                      w.iout.p + (size_t)step_idx * mo, s);
+        ++step_idx;
+        min_dt_used = std::min(min_dt_used, dt_cur);
+        max_dt_used = std::max(max_dt_used, dt_cur);
+        dt = new_dt;
+        continue;
+      }
       const double inv = 1.0 / dt_cur;
       double c0, c1, d1, c2, e_int;
       if (cfg->method == VJX_METHOD_BE) {

@@ -68,6 +68,9 @@ class SimulationOptions:
     gmin: float = 1e-12
     icmode: str = "op"
     fixed_step_transient: bool = False
+    # re-run the symbolic analysis when the factors' max |L| exceeds this
+    # (pivot growth; 0 keeps the library default 1e12)
+    repivot_growth: float = 0.0
 
 
 @dataclass
@@ -350,6 +353,7 @@ class CircuitEngine:
         tc.tran_itl = o.tran_itl
         tc.use_graph = 1 if p["use_graph"] else 0
         tc.autotune = 1 if p.get("autotune") else 0
+        tc.repivot_growth = float(o.repivot_growth)
         tc.reltol = o.reltol
         tc.abstol = o.abstol
         tc.vntol = o.vntol
