@@ -11,8 +11,9 @@ reference excludes JIT and setup).
 Prints one JSON line with ms/step and Newton iterations/s (``value``), a
 roofline estimate for the dominant phase from HIP events recorded on the
 library's own stream in a second, eager (non-graph) pass over the same steps,
-and a CPU baseline: the numpy/SciPy oracle (a restatement of the reference
-algorithm) timed on a bounded sample of the same workload on one host thread.
+and a CPU baseline: the C++ restatement of the reference algorithm
+(oracle/ctran.cpp) timed on the same circuit's first steps on all host threads
+and on one.
 """
 
 from __future__ import annotations
@@ -42,12 +43,13 @@ def build_circuit(gates: int, seed: int):
     return G.gate_dag(gates, seed=seed)
 
 
-def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None, autotune=True):
+def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None, autotune=True,
+            nd_leaf=None):
     from vajax_amd.engine import AdaptiveConfig
 
     eng.prepare(t_stop=1e-9, dt=1e-12, adaptive_config=AdaptiveConfig(integration_method="gear2"),
                 precision=precision, max_steps=steps + 1, use_graph=use_graph,
-                band_width=band_width, autotune=autotune)
+                band_width=band_width, autotune=autotune, nd_leaf=nd_leaf)
     return eng.run_transient()
 
 
@@ -132,28 +134,67 @@ def measured_traffic(args, cls):
     return int(sum(k["traffic_bytes"] * k["dispatches"] for k in ks) / n)
 
 
-def cpu_baseline(net, steps: int):
+def host_threads() -> int:
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() and int(env) > 0 else n
+
+
+def cpu_baseline(args):
+    """The C++ CPU restatement of the same algorithm (OpenMP over device
+    instances, serial KLU-style sparse LU) on the same circuit and the same
+    first steps, on all host threads and on one, in a child process so its
+    OpenMP runtime stays apart from this one's."""
+    import subprocess
+
+    threads = host_threads()
+    cmd = [sys.executable, "-m", "oracle.cpu_bench", "--gates", str(args.gates), "--seed",
+           str(args.seed), "--steps", str(args.cpu_steps), "--threads", str(threads), "1"]
+    r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline failed ({r.returncode}): {r.stderr[-2000:]}")
+    data = json.loads(r.stdout.strip().splitlines()[-1])
+    full, one = data["runs"][0], data["runs"][1]
+    out = {
+        "value": round(full["newton_iters_per_s"], 3),
+        "unit": "Newton iters/s",
+        "ms_per_step": round(full["ms_per_step"], 3),
+        "cores": full["threads"],
+        "kind": "port",
+        "sample": (f"C++ restatement of the reference algorithm (oracle/ctran.cpp: OpenMP device "
+                   f"eval and stamping, serial Gilbert-Peierls LU with threshold pivoting 0.1 "
+                   f"then refactorisation on the pivot order), same {data['workload']} circuit, "
+                   f"first {full['accepted_steps']} accepted steps ({full['newton_iterations']} "
+                   f"Newton iterations) in {full['wall_s']:.2f} s on {full['threads']} host "
+                   f"threads; setup and ordering excluded, first factorisation included"),
+        "phase_s": {k: round(v, 4) for k, v in full["phase_s"].items()},
+        "single_thread": {"value": round(one["newton_iters_per_s"], 3),
+                          "ms_per_step": round(one["ms_per_step"], 3),
+                          "phase_s": {k: round(v, 4) for k, v in one["phase_s"].items()}},
+    }
+    if args.cpu_numpy:
+        out["python_oracle"] = numpy_oracle_baseline(args)
+    return out
+
+
+def numpy_oracle_baseline(args):
+    """Secondary figure: the numpy/SciPy-SuperLU oracle on one thread."""
     from threadpoolctl import threadpool_limits
 
     from oracle import mna
     from oracle import transient as T
 
+    net = build_circuit(args.gates, args.seed)
+    steps = min(args.cpu_steps, 3)
     with threadpool_limits(limits=1):
         cir = mna.setup(net)
         cfg = T.AdaptiveConfig(integration_method="gear2")
         out = T.run_adaptive(cir, 1e-9, 1e-12, cfg, T.Options(icmode="uic"), max_steps=steps + 1)
     st = out.stats
     acc = max(st["n_steps"] - 1, 1)
-    return {
-        "value": st["total_nr_iterations"] / st["wall_time"],
-        "unit": "Newton iters/s",
-        "ms_per_step": st["wall_time"] / acc * 1e3,
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"numpy/SciPy-SuperLU oracle, same {net.title} circuit, first {acc} accepted "
-                   f"steps ({st['total_nr_iterations']} Newton iterations) in "
-                   f"{st['wall_time']:.1f} s, one host thread"),
-    }
+    return {"value": round(st["total_nr_iterations"] / st["wall_time"], 3),
+            "ms_per_step": round(st["wall_time"] / acc * 1e3, 3), "cores": 1,
+            "sample": f"first {acc} accepted steps, numpy/SciPy oracle, one thread"}
 
 
 def run_batched(args):
@@ -226,7 +267,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--band-width", type=int, default=None,
                     help="levels per triangular-solve band (0: one launch per level)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--nd-leaf", type=int, default=None,
+                    help="nested-dissection leaf size of the LU ordering")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-numpy", action="store_true",
+                    help="also time the numpy oracle (slow) as a secondary CPU figure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=256, help="Monte Carlo instances (0: skip)")
     ap.add_argument("--batch-gates", type=int, default=2000)
@@ -265,13 +310,14 @@ def main():
 
     use_graph = not args.no_graph
     t0 = time.time()
-    run_gpu(eng, args.warmup, args.precision, use_graph, args.band_width)
+    run_gpu(eng, args.warmup, args.precision, use_graph, args.band_width, nd_leaf=args.nd_leaf)
     log(f"warm-up ({args.warmup} steps incl. symbolic analysis) {time.time() - t0:.1f}s")
     lu_info = eng.device.lu.info()
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = run_gpu(eng, args.steps, args.precision, use_graph, args.band_width)
+    res = run_gpu(eng, args.steps, args.precision, use_graph, args.band_width,
+                  nd_leaf=args.nd_leaf)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = res.stats
@@ -283,7 +329,7 @@ def main():
     # eager pass with phase events on the library stream
     L.lib.vjx_profile_enable(1)
     L.lib.vjx_profile_reset()
-    run_gpu(eng, args.steps, args.precision, False, args.band_width)
+    run_gpu(eng, args.steps, args.precision, False, args.band_width, nd_leaf=args.nd_leaf)
     pt = L.PhaseTimes()
     L.lib.vjx_profile_read(C.byref(pt))
     L.lib.vjx_profile_enable(0)
@@ -323,8 +369,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
-        cpu = cpu_baseline(net, args.cpu_steps)
-        log(f"cpu baseline {time.time() - t0:.1f}s: {cpu['ms_per_step']:.1f} ms/step")
+        cpu = cpu_baseline(args)
+        log(f"cpu baseline {time.time() - t0:.1f}s: {cpu['ms_per_step']:.1f} ms/step on "
+            f"{cpu['cores']} threads, {cpu['single_thread']['ms_per_step']:.1f} on one")
 
     batched = None
     if args.batch > 0:

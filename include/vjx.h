@@ -124,6 +124,13 @@ int vjx_lu_check_schedules(const vjx_pattern* pat);
  * the analysis; the next refactorisation analyses again.  No counterpart in
  * the reference (a launch-plan knob). */
 int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width);
+/* Fill-reducing column order of the analysis: 0 = minimum degree on A+A^T,
+ * 1 = natural, 2 = nested dissection with minimum-degree leaves of at most
+ * nd_leaf rows (0: max(48, n / 128) per diagonal block; default 2, 0).  The
+ * role of UMFPACK's ordering choice (vajax/sparse/umfpack_ffi.cpp:140-179
+ * caches one symbolic analysis per pattern); a new value drops the analysis
+ * like a new band width. */
+int vjx_pattern_set_ordering(vjx_pattern* pat, int32_t ordering, int32_t nd_leaf);
 /* Autotuner of the solves' launch plan: with the matrix data_dev (device),
  * factorise, then time the triangular solves for band widths 8..48 on the
  * device and keep the fastest (results do not depend on the width).  No

@@ -159,7 +159,8 @@ struct AnalyzeOptions {
   double pivot_tol = 0.1;
   int wide_threshold = 2048;  // levels with at least this many items get a grid
   int ordering = 2;           // 0 = minimum degree, 1 = natural, 2 = nested dissection
-  int nd_leaf = 48;           // nested-dissection leaf size (min-degree inside)
+  int nd_leaf = 0;            // nested-dissection leaf size (min-degree inside);
+                              // 0: max(48, n / 128) of the block being ordered
   int band_width = 16;        // bands of this many levels (0: one launch per level)
   int solve_band_width = 0;   // solve bands (0: band_width)
   int max_band_tasks = 2048;  // workgroups per band launch

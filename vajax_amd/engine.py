@@ -300,9 +300,14 @@ class CircuitEngine:
                 adaptive_config: Optional[AdaptiveConfig] = None, precision: str = "f64",
                 refine_steps: Optional[int] = None, max_steps: Optional[int] = None,
                 use_graph: bool = False, band_width: Optional[int] = None,
-                autotune: bool = False, **_ignored) -> None:
+                autotune: bool = False, nd_leaf: Optional[int] = None,
+                **_ignored) -> None:
         if self.topology is None:
             self.parse()
+        if nd_leaf is not None:
+            # nested-dissection leaf size of the LU ordering (fill versus
+            # depth of the elimination); a change re-runs the analysis
+            self.device.lu.set_ordering("nd", nd_leaf)
         if band_width is not None:
             # launch plan of the triangular solves (levels per band; 0 = one
             # launch per level); a change re-runs the symbolic analysis

@@ -62,6 +62,14 @@ class SparseLU:
         from the next analysis."""
         L.check(L.lib.vjx_pattern_set_band_width(self.h, int(width)), "vjx_pattern_set_band_width")
 
+    def set_ordering(self, ordering: str = "nd", nd_leaf: int = 0) -> None:
+        """Fill-reducing order of the next analysis: "md" (minimum degree),
+        "natural" or "nd" (nested dissection, minimum-degree leaves of at most
+        ``nd_leaf`` rows; 0 = max(48, n / 128) per diagonal block)."""
+        code = {"md": 0, "natural": 1, "nd": 2}[ordering]
+        L.check(L.lib.vjx_pattern_set_ordering(self.h, code, int(nd_leaf)),
+                "vjx_pattern_set_ordering")
+
     def check_schedules(self) -> None:
         L.check(L.lib.vjx_lu_check_schedules(self.h), "vjx_lu_check_schedules")
 
