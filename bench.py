@@ -113,7 +113,7 @@ TRAFFIC_KERNELS = {
     "eval": ["k_eval<2>"], "stamp": ["k_reduce_jac", "k_residual", "k_chunk_sum", "k_chunk_res"],
     "lu_gather": ["k_lu_gather<double>"], "lu_tasks": ["k_lu_band_lds<double>", "k_lu_band<double>"],
     "lu_schur": ["k_lu_wide<double>"], "lu_scale": ["k_lu_scale<double>"],
-    "solve_tasks": ["k_tri_band_lds<double, true>", "k_tri_band_lds<double, false>",
+    "solve_tasks": ["k_tri_band_lde<double, true>", "k_tri_band_lde<double, false>",
                     "k_tri_band<double, true>", "k_tri_band<double, false>"]}
 
 

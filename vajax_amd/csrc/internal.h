@@ -99,8 +99,10 @@ struct DevBand {
   DBuf<double> part;  // hub chunk partials (FP32 solves use it as float storage)
   DBuf<int32_t> lds_rec, lds_task, lds_slots;  // LDS form (solves)
   DBuf<uint16_t> lds_col;
+  DBuf<int32_t> lds_src;  // entries-in-LDS form: LU slot of each lds_col entry
   bool on = false;
   bool use_lds = true;
+  int lde_bytes = 0;  // dynamic LDS a solve band task may use
 };
 
 struct DevLU {

@@ -845,6 +845,20 @@ struct TriTerm {
   __device__ __forceinline__ T operator()(int e) const { return lu[e] * v[col[e]]; }
 };
 
+// A long row of a solve band: one wavefront, lane = lane in the wavefront.
+template <typename T, bool FWD>
+__device__ __forceinline__ void tri_wave(int i, int lane, const TriProg& P,
+                                         const T* __restrict__ lu, T* y, T* z) {
+  if (FWD) {
+    const T acc = group_sum<kWaveLanes, T>(P.rp[i], P.dg[i], lane, TriTerm<T>{lu, y, P.col});
+    if (lane == 0) y[i] = y[i] - acc;
+  } else {
+    const int d = P.dg[i];
+    const T acc = group_sum<kWaveLanes, T>(d + 1, P.rp[i + 1], lane, TriTerm<T>{lu, z, P.col});
+    if (lane == 0) z[i] = (y[i] - acc) / lu[d];
+  }
+}
+
 template <typename T, bool FWD>
 __device__ __forceinline__ void tri_mid(int i, int lane, const TriProg& P,
                                         const T* __restrict__ lu, T* y, T* z) {
@@ -926,7 +940,6 @@ __global__ void __launch_bounds__(512) k_tri_narrow(int l0, int l1, const int32_
 template <typename T, bool FWD>
 __global__ void __launch_bounds__(256) k_tri_band(int w, int ptr_off, int hm_off, BandProg B,
                                                   TriProg P, const T* __restrict__ lu, T* y, T* z) {
-  __shared__ T red[RB + 1];
   __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
   const int t = blockIdx.x;
   const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
@@ -947,191 +960,180 @@ __global__ void __launch_bounds__(256) k_tri_band(int w, int ptr_off, int hm_off
       tri_light<T, FWD>(B.items[g], P, lu, y, z);
     for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp)
       tri_mid<T, FWD>(B.items[lo + nh + grp], lane, P, lu, y, z);
-    for (int h = 0; h < nh; ++h) tri_heavy<T, FWD>(B.items[lo + h], P, lu, y, z, red);
+    for (int h = (int)threadIdx.x / kWaveLanes; h < nh; h += (int)blockDim.x / kWaveLanes)
+      tri_wave<T, FWD>(B.items[lo + h], (int)threadIdx.x % kWaveLanes, P, lu, y, z);
     __syncthreads();
   }
 }
 
-// The same band task with its vector values in LDS: the task's own rows and
-// every row they read are loaded once, the levels run on LDS, and the own
-// rows are written back.  Entry e of an item's row reads the LDS value at
-// lds_col[c0 + e - e0]; the row's values and the sums' order are those of
-// the HBM form, so results are bit-identical.
+// The band task with everything its levels read in LDS.  The prologue issues
+// all of the task's global loads at once -- vector values of its rows and of
+// the rows they read, its row records, the values and local columns of its
+// entries (a contiguous run of lds_col / lds_src), and for the backward solve
+// each row's right-hand side and pivot -- so the task waits on memory latency
+// once instead of once per level; the levels then run on LDS and barriers
+// only.  Rows, classes and sums are those of the HBM band form (light rows
+// sequential, medium rows 16-lane sums, long rows one-wavefront sums), so
+// results are bit-identical.
+//
+// Dynamic LDS (the host sizes it from the band's largest task):
+//   v[nv] T | ent[ne] T | (16-byte aligned) rec[nr] int4 |
+//   backward only: rhs[nr] T, piv[nr] T | ecol[ne] uint16
 template <typename T>
-struct TriLdsTerm {
-  const T* lu;
+struct LdeTerm {
+  const T* ent;
   const T* v;
-  const uint16_t* lcol;
-  int shift;  // c0 - e0
-  __device__ __forceinline__ T operator()(int e) const { return lu[e] * v[lcol[e + shift]]; }
+  const uint16_t* ecol;
+  int shift;  // LDS entry index of LU slot e is e + shift
+  __device__ __forceinline__ T operator()(int e) const {
+    return ent[e + shift] * v[ecol[e + shift]];
+  }
 };
 
+__host__ __device__ inline size_t lde_rec_offset(int nv, int ne, size_t tsize) {
+  return (((size_t)nv + (size_t)ne) * tsize + 15) & ~(size_t)15;
+}
+
+__host__ __device__ inline size_t lde_col_offset(int nv, int ne, int nr, size_t tsize, bool fwd) {
+  return lde_rec_offset(nv, ne, tsize) + (size_t)nr * 16 + (fwd ? 0 : 2 * (size_t)nr * tsize);
+}
+
 template <typename T, bool FWD>
-__global__ void __launch_bounds__(256) k_tri_band_lds(int w, int ptr_off, int hm_off, int task0,
-                                                      BandProg B, const int4* __restrict__ recs,
+__global__ void __launch_bounds__(256) k_tri_band_lde(int w, int ptr_off, int hm_off, int task0,
+                                                      int nv, int ne_max, int nr_max, BandProg B,
+                                                      const int4* __restrict__ recs,
                                                       const uint16_t* __restrict__ lcol,
+                                                      const int32_t* __restrict__ esrc,
                                                       const int4* __restrict__ tasks,
                                                       const int32_t* __restrict__ slots,
                                                       const T* __restrict__ lu, T* y, T* z) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* v = reinterpret_cast<T*>(lds_raw);
-  __shared__ T red[RB + 1];
+  T* ent = v + nv;
+  int4* rec = reinterpret_cast<int4*>(lds_raw + lde_rec_offset(nv, ne_max, sizeof(T)));
+  T* rhs = reinterpret_cast<T*>(rec + nr_max);
+  T* piv = rhs + nr_max;
+  uint16_t* ecol =
+      reinterpret_cast<uint16_t*>(lds_raw + lde_col_offset(nv, ne_max, nr_max, sizeof(T), FWD));
   __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
   const int t = blockIdx.x;
+  const int tid = (int)threadIdx.x;
   const int4 tk = tasks[task0 + t];  // slot offset, slot count, own rows
   const int32_t* tp = B.tl_ptr + ptr_off + (size_t)t * (w + 1);
   const int32_t* th = B.tl_heavy + hm_off + (size_t)t * w;
   const int32_t* tm = B.tl_mid + hm_off + (size_t)t * w;
-  for (int j = threadIdx.x; j <= w; j += blockDim.x) s_ptr[j] = tp[j];
-  for (int j = threadIdx.x; j < w; j += blockDim.x) {
+  for (int j = tid; j <= w; j += blockDim.x) s_ptr[j] = tp[j];
+  for (int j = tid; j < w; j += blockDim.x) {
     s_h[j] = th[j];
     s_m[j] = tm[j];
   }
+  const int g0 = tp[0];
+  const int nr = tk.z;  // the task's rows: items [g0, g0 + nr)
+  if (nr == 0) return;  // uniform across the block
+  const int4 rf = recs[g0];
+  const int4 rl = recs[g0 + nr - 1];
+  const int cb = rf.w;                          // first entry in lds_col / lds_src
+  const int ne = rl.w + (rl.z - rl.y) - cb;     // the task's entries (<= ne_max)
   const T* src = FWD ? y : z;
-  for (int i = threadIdx.x; i < tk.y; i += blockDim.x) v[i] = src[slots[tk.x + i]];
-  __syncthreads();
-  const int lane = (int)threadIdx.x % kGroupLanes;
-  const int ngrp = (int)blockDim.x / kGroupLanes;
-  // row g of the band: its sum, then the forward / backward update in LDS
-  auto finish = [&](int g, const int4& r, T acc) {
-    if (FWD) {
-      v[r.x] = v[r.x] - acc;
-    } else {
-      const int i = B.items[g];
-      v[r.x] = (y[i] - acc) / lu[r.y - 1];  // the diagonal precedes the U part
-    }
-  };
-  // Software pipeline: for each lane's first short row, its group's first
-  // medium row and the level's first long row, the record is fetched two
-  // levels ahead and the lane's entries (values and local columns; for the
-  // backward solve also the right-hand side and pivot) one level ahead, so a
-  // level mostly waits on LDS and barriers.  A lane's entries of a row are
-  // those its fixed-order sum visits: all of a short row (<= 3), every 16th
-  // of a medium row (<= 128 entries, so <= 8), every 256th of a long row
-  // (the first two; the rest are loaded in the level).
-  const int grp0 = (int)threadIdx.x / kGroupLanes;
-  auto row_of = [&](int kind, int jj) -> int {
-    if (jj >= w) return -1;
-    const int lo = s_ptr[jj], nh = s_h[jj], nm = s_m[jj];
-    if (kind == 0) {
-      const int g = lo + nh + nm + (int)threadIdx.x;
-      return g < s_ptr[jj + 1] ? g : -1;
-    }
-    if (kind == 1) return grp0 < nm ? lo + nh + grp0 : -1;
-    return nh > 0 ? lo : -1;
-  };
-  struct Pre {
-    int g;
-    int4 r;
-    int i;
-    T a[8];
-    int c[8];
-    T rhs, piv;
-  };
-  auto fetch_rec = [&](Pre& p, int kind, int jj) {
-    p.g = row_of(kind, jj);
-    if (p.g >= 0) {
-      p.r = recs[p.g];
-      if (!FWD) p.i = B.items[p.g];
-    }
-  };
-  // lane offset / stride / count of the lane's entries for each kind
-  auto fetch_entries = [&](Pre& p, int off, int stride, int cnt) {
-    if (p.g < 0) return;
+  const int bd = (int)blockDim.x;
+  // Every load below is independent of the others in its round: each thread
+  // first issues a round's index loads, then the value loads they address,
+  // then stores to LDS, so a round costs two memory latencies.
+  constexpr int EC = 16;  // entries per thread and round
+  int si[EC] = {};
+  uint16_t cc[EC] = {};
+  auto entry_indices = [&](int k0) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int e = p.r.y + off + stride * k;
-      if (k < cnt && e < p.r.z) {
-        p.a[k] = lu[e];
-        p.c[k] = lcol[p.r.w + (e - p.r.y)];
+    for (int u = 0; u < EC; ++u) {
+      const int k = k0 + u * bd;
+      if (k < ne) {
+        si[u] = esrc[cb + k];
+        cc[u] = lcol[cb + k];
       }
     }
-    if (!FWD) {
-      p.rhs = y[p.i];
-      p.piv = lu[p.r.y - 1];
-    }
   };
-  auto part = [&](const Pre& p, int off, int stride, int cnt) {
-    T acc = (T)0;
+  entry_indices(tid);  // in flight while the vector values load
+  for (int i0 = tid; i0 < tk.y; i0 += 8 * bd) {
+    int sl[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int e = p.r.y + off + stride * k;
-      if (k < cnt && e < p.r.z) acc += p.a[k] * v[p.c[k]];
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * bd;
+      sl[u] = i < tk.y ? slots[tk.x + i] : 0;
     }
-    return acc;
-  };
-  auto finish_pre = [&](const Pre& p, T acc) {
+    T vv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u * bd < tk.y) vv[u] = src[sl[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u * bd < tk.y) v[i0 + u * bd] = vv[u];
+  }
+  for (int k = tid; k < nr; k += bd) {
+    const int4 r = recs[g0 + k];
+    rec[k] = r;
+    if (!FWD) {
+      rhs[k] = y[B.items[g0 + k]];
+      piv[k] = lu[r.y - 1];  // the diagonal precedes the U part
+    }
+  }
+  // entries: this round's values, then the next round's indices, then the
+  // stores (the next indices stay in flight across the stores)
+  for (int k0 = tid; k0 < ne; k0 += EC * bd) {
+    T val[EC];
+    uint16_t cur[EC];
+#pragma unroll
+    for (int u = 0; u < EC; ++u) {
+      if (k0 + u * bd < ne) val[u] = lu[si[u]];
+      cur[u] = cc[u];
+    }
+    entry_indices(k0 + EC * bd);
+#pragma unroll
+    for (int u = 0; u < EC; ++u) {
+      const int k = k0 + u * bd;
+      if (k < ne) {
+        ent[k] = val[u];
+        ecol[k] = cur[u];
+      }
+    }
+  }
+  __syncthreads();
+  auto finish = [&](int k, const int4& r, T acc) {
     if (FWD)
-      v[p.r.x] = v[p.r.x] - acc;
+      v[r.x] = v[r.x] - acc;
     else
-      v[p.r.x] = (p.rhs - acc) / p.piv;
+      v[r.x] = (rhs[k] - acc) / piv[k];
   };
-  const int tid = (int)threadIdx.x;
-  Pre s_cur, s_nxt, s_far, m_cur, m_nxt, m_far, h_cur, h_nxt, h_far;
-  fetch_rec(s_cur, 0, 0);
-  fetch_rec(m_cur, 1, 0);
-  fetch_rec(h_cur, 2, 0);
-  fetch_rec(s_nxt, 0, 1);
-  fetch_rec(m_nxt, 1, 1);
-  fetch_rec(h_nxt, 2, 1);
-  fetch_entries(s_cur, 0, 1, 3);
-  fetch_entries(m_cur, lane, kGroupLanes, 8);
-  fetch_entries(h_cur, tid, RB, 2);
+  const int lane = tid % kGroupLanes;
+  const int ngrp = (int)blockDim.x / kGroupLanes;
   for (int j = 0; j < w; ++j) {
     const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
-    fetch_rec(s_far, 0, j + 2);
-    fetch_rec(m_far, 1, j + 2);
-    fetch_rec(h_far, 2, j + 2);
-    fetch_entries(s_nxt, 0, 1, 3);
-    fetch_entries(m_nxt, lane, kGroupLanes, 8);
-    fetch_entries(h_nxt, tid, RB, 2);
-    if (s_cur.g >= 0) finish_pre(s_cur, part(s_cur, 0, 1, 3));
-    // further short rows of this lane (levels wider than the workgroup)
-    for (int g = lo + nh + nm + tid + (int)blockDim.x; g < hi; g += blockDim.x) {
-      const int4 r = recs[g];
-      const TriLdsTerm<T> term{lu, v, lcol, r.w - r.y};
+    if (lo == hi) continue;  // uniform across the block
+    for (int g = lo + nh + nm + tid; g < hi; g += blockDim.x) {
+      const int k = g - g0;
+      const int4 r = rec[k];
+      const LdeTerm<T> term{ent, v, ecol, r.w - cb - r.y};
       T acc = (T)0;
       for (int e = r.y; e < r.z; ++e) acc += term(e);
-      finish(g, r, acc);
+      finish(k, r, acc);
     }
-    // medium rows: the group's first from the pipeline, the rest direct
-    if (grp0 < nm) {
-      T acc = part(m_cur, lane, kGroupLanes, 8);
-#pragma unroll
-      for (int s = kGroupLanes / 2; s > 0; s >>= 1) acc += __shfl_down(acc, s, kGroupLanes);
-      if (lane == 0) finish_pre(m_cur, acc);
+    for (int grp = tid / kGroupLanes; grp < nm; grp += ngrp) {
+      const int k = lo + nh + grp - g0;
+      const int4 r = rec[k];
+      const T acc =
+          group_sum<kGroupLanes, T>(r.y, r.z, lane, LdeTerm<T>{ent, v, ecol, r.w - cb - r.y});
+      if (lane == 0) finish(k, r, acc);
     }
-    for (int grp = grp0 + ngrp; grp < nm; grp += ngrp) {
-      const int g = lo + nh + grp;
-      const int4 r = recs[g];
-      const T acc = group_sum<kGroupLanes, T>(r.y, r.z, lane, TriLdsTerm<T>{lu, v, lcol, r.w - r.y});
-      if (lane == 0) finish(g, r, acc);
-    }
-    // long rows: the first from the pipeline (two entries per lane, the rest
-    // of its strided sum loaded here), the others direct
-    if (nh > 0) {
-      T acc = part(h_cur, tid, RB, 2);
-      const TriLdsTerm<T> term{lu, v, lcol, h_cur.r.w - h_cur.r.y};
-      for (int e = h_cur.r.y + tid + 2 * RB; e < h_cur.r.z; e += RB) acc += term(e);
-      acc = block_tree<T>(acc, red);
-      if (tid == 0) finish_pre(h_cur, acc);
-    }
-    for (int h = 1; h < nh; ++h) {
-      const int g = lo + h;
-      const int4 r = recs[g];
-      const T acc = block_sum<T>(r.y, r.z, TriLdsTerm<T>{lu, v, lcol, r.w - r.y}, red);
-      if (tid == 0) finish(g, r, acc);
+    for (int h = tid / kWaveLanes; h < nh; h += (int)blockDim.x / kWaveLanes) {
+      const int k = lo + h - g0;
+      const int4 r = rec[k];
+      const T acc = group_sum<kWaveLanes, T>(r.y, r.z, tid % kWaveLanes,
+                                             LdeTerm<T>{ent, v, ecol, r.w - cb - r.y});
+      if (tid % kWaveLanes == 0) finish(k, r, acc);
     }
     lds_barrier();  // the levels exchange values through LDS only
-    s_cur = s_nxt;
-    s_nxt = s_far;
-    m_cur = m_nxt;
-    m_nxt = m_far;
-    h_cur = h_nxt;
-    h_nxt = h_far;
   }
   T* dst = FWD ? y : z;
-  for (int i = threadIdx.x; i < tk.z; i += blockDim.x) dst[slots[tk.x + i]] = v[i];
+  for (int i = tid; i < nr; i += blockDim.x) dst[slots[tk.x + i]] = v[i];
 }
 
 // Rows longer than kHubChunk: chunk partials over the grid, then one lane per
@@ -1423,10 +1425,13 @@ static void run_tri_bands(DevLU& L, const BandSchedule& H, DevBand& D, const T* 
     if (ntask > 0) {
       ktimer_begin(s);
       const int nv = H.band_lds.empty() ? 0 : H.band_lds[b];
-      if (nv > 0 && D.use_lds)
-        k_tri_band_lds<T, FWD><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
-            H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], B,
-            reinterpret_cast<const int4*>(D.lds_rec.p), D.lds_col.p,
+      const int ne = nv > 0 ? H.band_ent[b] : 0, nr = nv > 0 ? H.band_rows[b] : 0;
+      const size_t lde =
+          nv > 0 ? lde_col_offset(nv, ne, nr, sizeof(T), FWD) + 2 * (size_t)ne : 0;
+      if (nv > 0 && D.use_lds && lde <= (size_t)D.lde_bytes)
+        k_tri_band_lde<T, FWD><<<ntask, 256, lde, s>>>(
+            H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], nv, ne, nr, B,
+            reinterpret_cast<const int4*>(D.lds_rec.p), D.lds_col.p, D.lds_src.p,
             reinterpret_cast<const int4*>(D.lds_task.p), D.lds_slots.p, lu, y, z);
       else
         k_tri_band<T, FWD><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b],

@@ -146,6 +146,7 @@ static hipError_t upload_band(DevBand& d, const BandSchedule& h) {
   if (e == hipSuccess) e = d.lds_col.upload(h.lds_col);
   if (e == hipSuccess) e = d.lds_task.upload(h.lds_task);
   if (e == hipSuccess) e = d.lds_slots.upload(h.lds_slots);
+  if (e == hipSuccess) e = d.lds_src.upload(h.lds_src);
   return e;
 }
 
@@ -181,9 +182,15 @@ int pattern_upload(vjx_pattern* pat) {
   // VJX_LU_LDS=0 keeps every LU band task in HBM (A/B measurements)
   const char* lds_env = std::getenv("VJX_LU_LDS");
   L.use_lds = !(lds_env && lds_env[0] == '0');
-  // VJX_SOLVE_LDS=0 keeps every solve band task in HBM (A/B measurements)
+  // Solve band tasks run from LDS when everything they read fits the
+  // device's LDS per workgroup (less the kernel's static arrays), else from
+  // HBM; VJX_SOLVE_LDS=0 keeps every task in HBM (A/B measurements)
   const char* tri_env = std::getenv("VJX_SOLVE_LDS");
   L.fwd_band.use_lds = L.bwd_band.use_lds = !(tri_env && tri_env[0] == '0');
+  int dev = 0, lds_max = 0;
+  VJX_TRY_HIP(hipGetDevice(&dev));
+  VJX_TRY_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  L.fwd_band.lde_bytes = L.bwd_band.lde_bytes = std::max(0, lds_max - kTriStaticLds);
   const size_t nnz = s.col.size();
   const size_t n = (size_t)s.n;
   VJX_TRY_HIP(L.lu64.alloc(nnz));

@@ -1094,15 +1094,27 @@ static bool check_tri_lds(const LuSymbolic& s, const BandSchedule& B, Seg seg, s
         return false;
       }
       const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+      // entries-in-LDS form: the task's rows are its first tk[2] local rows
+      // in item order, their entries one contiguous run of lds_col / lds_src
+      if (tp[w] - tp[0] != tk[2] || tk[2] > B.band_rows[b]) {
+        err = "solve LDS program: task rows do not match its record";
+        return false;
+      }
+      const int64_t cb = tp[w] > tp[0] ? B.lds_rec[(size_t)tp[0] * 4 + 3] : 0;
+      int64_t next = cb;
       for (int g = tp[0]; g < tp[w]; ++g) {
         const int i = B.items[g];
         const int32_t* rec = &B.lds_rec[(size_t)g * 4];
         auto r = seg(i);
-        bool ok = rec[0] < tk[2] && sl[rec[0]] == i && rec[1] == r.first && rec[2] == r.second;
+        bool ok = rec[0] == g - tp[0] && sl[rec[0]] == i && rec[1] == r.first &&
+                  rec[2] == r.second && rec[3] == next;
         for (int e = r.first; ok && e < r.second; ++e) {
-          const int lc = B.lds_col[(size_t)rec[3] + (e - r.first)];
-          ok = lc < tk[1] && sl[lc] == s.col[e];
+          const size_t k = (size_t)rec[3] + (e - r.first);
+          const int lc = B.lds_col[k];
+          ok = lc < tk[1] && sl[lc] == s.col[e] && B.lds_src[k] == e;
         }
+        next += r.second - r.first;
+        if (ok && next - cb > B.band_ent[b]) ok = false;
         if (!ok) {
           err = "solve LDS program: row " + std::to_string(i) + " encoded wrongly";
           return false;
@@ -1238,12 +1250,16 @@ static void build_tri_lds(const LuSymbolic& s, Seg seg, BandSchedule& B) {
   B.lds_task.clear();
   B.lds_slots.clear();
   B.band_lds.clear();
+  B.lds_src.clear();
+  B.band_ent.clear();
+  B.band_rows.clear();
   std::vector<int32_t> local(s.n, -1), rows;
   for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
     const int w = B.band_w[b];
-    int need = 0;
+    int need = 0, ent_max = 0, rows_max = 0;
     for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
       const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
+      const size_t col0 = B.lds_col.size();
       rows.clear();
       for (int g = tp[0]; g < tp[w]; ++g) {
         local[B.items[g]] = (int32_t)rows.size();
@@ -1273,12 +1289,18 @@ static void build_tri_lds(const LuSymbolic& s, Seg seg, BandSchedule& B) {
         rec[1] = r.first;
         rec[2] = r.second;
         rec[3] = (int32_t)B.lds_col.size();
-        for (int e = r.first; e < r.second; ++e)
+        for (int e = r.first; e < r.second; ++e) {
           B.lds_col.push_back(fits ? (uint16_t)local[s.col[e]] : (uint16_t)0);
+          B.lds_src.push_back(e);
+        }
       }
+      ent_max = std::max(ent_max, (int)(B.lds_col.size() - col0));
+      rows_max = std::max(rows_max, nown);
       for (int j : rows) local[j] = -1;
     }
     B.band_lds.push_back(need > 0 ? need : 0);
+    B.band_ent.push_back(need > 0 ? ent_max : 0);
+    B.band_rows.push_back(need > 0 ? rows_max : 0);
   }
 }
 
@@ -1735,11 +1757,14 @@ static T class_sum(int cls, int start, int end, F term) {
   return acc;
 }
 
-// Sum of one triangular-solve row: by length class, or for rows longer than
-// kHubChunk under band programs, chunk sums (workgroup class) added in order.
+// Sum of one triangular-solve row: by length class; under band programs the
+// long rows up to kHubChunk take the one-wavefront sum and longer rows chunk
+// sums (workgroup class) added in order.
 template <typename T, typename F>
 static T row_sum(int start, int end, bool chunk_hubs, F term) {
   const int len = end - start;
+  if (chunk_hubs && len > kHeavySegment && len <= kHubChunk)
+    return lane_sum<kWaveLanes, T>(start, end, term);
   if (!chunk_hubs || len <= kHubChunk) return class_sum<T>(reduction_class(len), start, end, term);
   T acc = (T)0;
   for (int c = start; c < end; c += kHubChunk)

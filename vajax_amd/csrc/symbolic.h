@@ -56,6 +56,10 @@ constexpr int kHeavySegment = 128;    // longer segments: one workgroup
 constexpr int kShortSegment = 3;      // shorter or equal: one lane
 constexpr int kGroupLanes = 16;       // lanes per medium segment
 constexpr int kReduceLanes = 256;     // lanes of the fixed-order workgroup sum
+// Long rows (kHeavySegment < length <= kHubChunk) of the triangular-solve
+// band programs are summed by one wavefront (strided partials + shuffle
+// tree), one row per wavefront, so a level needs no barrier beyond its own.
+constexpr int kWaveLanes = 64;
 constexpr int kMaxNarrowLevels = 4096;  // levels per single-block launch (LDS cache)
 
 inline int reduction_class(int len) {
@@ -70,6 +74,9 @@ constexpr int kMaxBandWidth = 64;  // levels per band (LDS table of one task)
 // LU values a band task may hold in LDS: 60 KiB in FP64, so that with the
 // kernel's own reduction and level tables one workgroup stays within 64 KiB
 constexpr int kLdsSlots = 7680;
+// Static LDS of the entries-in-LDS solve kernel (reduction tree and level
+// tables), kept out of the dynamic budget with margin.
+constexpr int kTriStaticLds = 4096;
 
 // Band program of a triangular solve.  The levels are cut into bands of at
 // most `width` levels; inside a band the rows form connected components
@@ -101,6 +108,10 @@ struct BandSchedule {
   std::vector<int32_t> lds_task;      // 4 ints per task: slot offset, count, own rows, 0
   std::vector<int32_t> lds_slots;     // per task: rows, own rows first
   std::vector<int32_t> band_lds;      // per band: LDS values needed (0: band runs from HBM)
+  // Entries-in-LDS form: lds_src[k] is the LU slot of the entry whose local
+  // column is lds_col[k]; per band the largest task's entry and row counts.
+  std::vector<int32_t> lds_src;
+  std::vector<int32_t> band_ent, band_rows;
 };
 
 struct LuSymbolic {
