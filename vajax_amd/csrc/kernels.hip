@@ -543,6 +543,42 @@ __global__ void __launch_bounds__(256) k_lu_gather(int nnz, const int32_t* __res
   lu[e] = s >= 0 ? (T)csr[s] : (T)0;
 }
 
+// v[i] = src[slots[i]] for i < n by the whole block, 8 per thread and round:
+// the round's slot loads, then the value loads they address, then the LDS
+// stores, so a round costs two memory latencies whatever n is.
+template <typename T>
+__device__ __forceinline__ void gather_slots(int n, const int32_t* __restrict__ slots,
+                                             const T* src, T* v) {
+  const int bd = (int)blockDim.x;
+  for (int i0 = (int)threadIdx.x; i0 < n; i0 += 8 * bd) {
+    int sl[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sl[u] = i0 + u * bd < n ? slots[i0 + u * bd] : 0;
+    T vv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u * bd < n) vv[u] = src[sl[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u * bd < n) v[i0 + u * bd] = vv[u];
+  }
+}
+
+// dst[slots[i]] = v[i] for i < n by the whole block, 8 per thread and round.
+template <typename T>
+__device__ __forceinline__ void scatter_slots(int n, const int32_t* __restrict__ slots,
+                                              const T* v, T* dst) {
+  const int bd = (int)blockDim.x;
+  for (int i0 = (int)threadIdx.x; i0 < n; i0 += 8 * bd) {
+    int sl[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sl[u] = i0 + u * bd < n ? slots[i0 + u * bd] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u * bd < n) dst[sl[u]] = v[i0 + u * bd];
+  }
+}
+
 // Task tables of a band program: task t of a band with w levels has item
 // offsets tl_ptr[t*(w+1) ...] and per-level workgroup / lane-group counts.
 struct BandProg {
@@ -712,58 +748,80 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
     s_h[j] = th[j];
     s_m[j] = tm[j];
   }
-  for (int i = threadIdx.x; i < tk.y; i += blockDim.x) vals[i] = lu[slots[tk.x + i]];
+  gather_slots<T>(tk.y, slots + tk.x, lu, vals);
   __syncthreads();
   const LdsTerm<T> term{vals, upd};
   const int lane = (int)threadIdx.x % kGroupLanes;
   const int ngrp = (int)blockDim.x / kGroupLanes;
-  // Software pipeline for each lane's first short item of a level: record two
-  // levels ahead, its (at most kShortSegment = 3) packed updates one level
-  // ahead, so a level waits only on LDS and the barrier.
-  auto first_short = [&](int jj) -> int {
-    if (jj >= w) return -1;
-    const int g = s_ptr[jj] + s_h[jj] + s_m[jj] + (int)threadIdx.x;
-    return g < s_ptr[jj + 1] ? g : -1;
-  };
-  struct Pre {
-    int g;
-    int4 r;
-    uint2 u[3];
-  };
-  auto fetch_rec = [&](Pre& p, int g) {
-    p.g = g;
-    if (g >= 0) p.r = recs[g];
-  };
-  auto fetch_upd = [&](Pre& p) {
-    if (p.g < 0) return;
+  // Short items (at most kShortSegment = 3 updates).  An item with one
+  // update carries it in its record ({packed word, pivot}, z = 1), so it
+  // needs no dependent load; others read their updates from lds_upd.  Each
+  // lane's first KP items of a level (tid, tid + 256, ...) are software
+  // pipelined: records RD levels ahead, multi-update items' updates UD levels
+  // ahead.  Items beyond KP per lane go in batches of KB whose loads are all
+  // in flight together.  An absent item has w = -1 and z = 0.
+  constexpr int KP = 4, RD = 3, UD = 1, KB = 8;
+  const int bd = (int)blockDim.x;
+  const int tid = (int)threadIdx.x;
+  auto fetch_recs = [&](int4 (&R)[KP], int jj) {
+    const bool in = jj < w;
+    const int a = in ? s_ptr[jj] + s_h[jj] + s_m[jj] + tid : 0;
+    const int e = in ? s_ptr[jj + 1] : 0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (k < p.r.z) p.u[k] = upd[p.r.y + k];
+    for (int k = 0; k < KP; ++k) {
+      const int g = a + k * bd;
+      R[k] = (in && g < e) ? recs[g] : make_int4(0, 0, 0, -1);
+    }
   };
-  Pre cur, nxt, far;
-  fetch_rec(cur, first_short(0));
-  fetch_rec(nxt, first_short(1));
-  fetch_upd(cur);
+  auto fetch_upds = [&](uint2 (&U)[KP][3], const int4 (&R)[KP]) {
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (R[k].z > 1 && q < R[k].z) U[k][q] = upd[R[k].y + q];
+  };
+  // the sum of one short item and its write; u holds the updates of a
+  // multi-update item (unused for z <= 1)
+  auto short_item = [&](const int4& r, const uint2 (&u)[3]) {
+    if (r.w < 0) return;
+    T acc = (T)0;
+    if (r.z == 1) {
+      acc += vals[(uint32_t)r.x & 0xffffu] * vals[(uint32_t)r.x >> 16] / vals[r.y];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (q < r.z) acc += vals[u[q].x & 0xffffu] * vals[u[q].x >> 16] / vals[u[q].y];
+    }
+    vals[r.w] -= acc;
+  };
+  int4 R[RD + 1][KP];      // R[s]: records of level j + s
+  uint2 U[UD + 1][KP][3];  // U[s]: updates of level j + s
+#pragma unroll
+  for (int s = 0; s < RD; ++s) fetch_recs(R[s], s);
+#pragma unroll
+  for (int s = 0; s < UD; ++s) fetch_upds(U[s], R[s]);
   for (int j = 0; j < w; ++j) {
     const int lo = s_ptr[j], hi = s_ptr[j + 1], nh = s_h[j], nm = s_m[j];
-    fetch_rec(far, first_short(j + 2));
-    fetch_upd(nxt);
-    if (cur.g >= 0) {
-      T acc = (T)0;
+    fetch_recs(R[RD], j + RD);
+    fetch_upds(U[UD], R[UD]);
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (k < cur.r.z) {
-          const uint2 a = cur.u[k];
-          acc += vals[a.x & 0xffffu] * vals[a.x >> 16] / vals[a.y];
-        }
-      vals[cur.r.w] -= acc;
-    }
-    // further short items of this lane (levels wider than the workgroup)
-    for (int g = lo + nh + nm + (int)threadIdx.x + (int)blockDim.x; g < hi; g += blockDim.x) {
-      const int4 r = recs[g];
-      T acc = (T)0;
-      for (int k = r.y; k < r.y + r.z; ++k) acc += term(k);
-      vals[r.w] -= acc;
+    for (int k = 0; k < KP; ++k) short_item(R[0][k], U[0][k]);
+    // further short items of this lane (levels wider than KP workgroups)
+    for (int g0 = lo + nh + nm + tid + KP * bd; g0 < hi; g0 += KB * bd) {
+      int4 rb[KB];
+#pragma unroll
+      for (int u = 0; u < KB; ++u) {
+        const int g = g0 + u * bd;
+        rb[u] = g < hi ? recs[g] : make_int4(0, 0, 0, -1);
+      }
+      uint2 ub[KB][3];
+#pragma unroll
+      for (int u = 0; u < KB; ++u)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          if (rb[u].z > 1 && q < rb[u].z) ub[u][q] = upd[rb[u].y + q];
+#pragma unroll
+      for (int u = 0; u < KB; ++u) short_item(rb[u], ub[u]);
     }
     for (int grp = (int)threadIdx.x / kGroupLanes; grp < nm; grp += ngrp) {
       const int4 r = recs[lo + nh + grp];
@@ -776,11 +834,18 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
       if (threadIdx.x == 0) vals[r.w] -= acc;
     }
     lds_barrier();  // the levels exchange values through LDS only
-    cur = nxt;
-    nxt = far;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+#pragma unroll
+      for (int s = 0; s < RD; ++s) R[s][k] = R[s + 1][k];
+#pragma unroll
+      for (int s = 0; s < UD; ++s)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) U[s][k][q] = U[s + 1][k][q];
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < tk.z; i += blockDim.x) lu[slots[tk.x + i]] = vals[i];
+  scatter_slots<T>(tk.z, slots + tk.x, vals, lu);
 }
 
 template <typename T>
@@ -1053,21 +1118,7 @@ __global__ void __launch_bounds__(256) k_tri_band_lde(int w, int ptr_off, int hm
     }
   };
   entry_indices(tid);  // in flight while the vector values load
-  for (int i0 = tid; i0 < tk.y; i0 += 8 * bd) {
-    int sl[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = i0 + u * bd;
-      sl[u] = i < tk.y ? slots[tk.x + i] : 0;
-    }
-    T vv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (i0 + u * bd < tk.y) vv[u] = src[sl[u]];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (i0 + u * bd < tk.y) v[i0 + u * bd] = vv[u];
-  }
+  gather_slots<T>(tk.y, slots + tk.x, src, v);
   for (int k = tid; k < nr; k += bd) {
     const int4 r = recs[g0 + k];
     rec[k] = r;
@@ -1133,7 +1184,7 @@ __global__ void __launch_bounds__(256) k_tri_band_lde(int w, int ptr_off, int hm
     lds_barrier();  // the levels exchange values through LDS only
   }
   T* dst = FWD ? y : z;
-  for (int i = tid; i < nr; i += blockDim.x) dst[slots[tk.x + i]] = v[i];
+  scatter_slots<T>(nr, slots + tk.x, v, dst);
 }
 
 // Rows longer than kHubChunk: chunk partials over the grid, then one lane per

@@ -1169,10 +1169,12 @@ bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
         const int32_t* rec = &s.lds_item[(size_t)g * 4];
         const int tg = B.items[g];
         const int u0 = s.tgt_seg[tg];
-        bool ok = rec[0] == tg && rec[2] == s.tgt_seg[tg + 1] - u0 && rec[3] < tk[2] &&
+        const bool inl = rec[2] == 1;  // single update inline: {packed, pivot}
+        bool ok = (inl || rec[0] == tg) && rec[2] == s.tgt_seg[tg + 1] - u0 && rec[3] < tk[2] &&
                   sl[rec[3]] == s.tgt_pos[tg];
         for (int k = 0; ok && k < rec[2]; ++k) {
-          const uint32_t a = s.lds_upd[(size_t)(rec[1] + k) * 2], d = s.lds_upd[(size_t)(rec[1] + k) * 2 + 1];
+          const uint32_t a = inl ? (uint32_t)rec[0] : s.lds_upd[(size_t)(rec[1] + k) * 2];
+          const uint32_t d = inl ? (uint32_t)rec[1] : s.lds_upd[(size_t)(rec[1] + k) * 2 + 1];
           const int l = (int)(a & 0xffffu), v = (int)(a >> 16);
           ok = l < tk[1] && v < tk[1] && (int)d < tk[1] && sl[l] == s.upd_l[u0 + k] &&
                sl[v] == s.upd_u[u0 + k] && sl[d] == s.upd_d[u0 + k];
@@ -1491,6 +1493,13 @@ static void build_fact_lds(LuSymbolic& s) {
         rec[1] = (int32_t)(s.lds_upd.size() / 2);
         rec[2] = u1 - u0;
         rec[3] = fits ? local[s.tgt_pos[tg]] : 0;
+        if (fits && u1 - u0 == 1) {
+          // a single update travels in the record itself: {packed L | U << 16, pivot}
+          const uint32_t l = (uint32_t)local[s.upd_l[u0]], v = (uint32_t)local[s.upd_u[u0]];
+          rec[0] = (int32_t)(l | (v << 16));
+          rec[1] = local[s.upd_d[u0]];
+          continue;
+        }
         for (int u = u0; u < u1; ++u) {
           const uint32_t l = fits ? (uint32_t)local[s.upd_l[u]] : 0u;
           const uint32_t v = fits ? (uint32_t)local[s.upd_u[u]] : 0u;

@@ -140,7 +140,9 @@ struct LuSymbolic {
   // operand its updates read) fit in kLdsSlots values loads them into LDS,
   // runs its levels there and writes its targets back.  Per task item
   // (parallel to fact_band.items over task items; unused for Schur items):
-  // {target id, first packed update, update count, local target index}.
+  // {target id, first packed update, update count, local target index}, or
+  // for an item with exactly one update {its packed word, its pivot index,
+  // 1, local target index} with the update itself not in lds_upd.
   std::vector<int32_t> lds_item;      // 4 ints per fact_band item
   std::vector<uint32_t> lds_upd;      // 2 words per update: (l | u << 16), d
   std::vector<int32_t> lds_task;      // 4 ints per task: slot offset, slot count, written count, 0
