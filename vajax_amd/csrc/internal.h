@@ -115,6 +115,8 @@ struct DevLU {
   DBuf<int32_t> lds_item, lds_task, lds_slots;  // LDS form of the LU band tasks
   DBuf<uint32_t> lds_upd;
   bool use_lds = true;  // run LU band tasks that fit from LDS
+  int n_cu = 256;       // compute units of the device
+  int lu_deep = 0;      // LU band pipeline: 0 by task count, 1 shallow, 2 deep
   DBuf<double> lu64;
   DBuf<float> lu32;
   DBuf<double> y64, z64, r64, x64;

@@ -728,7 +728,11 @@ struct LdsTerm {
   }
 };
 
-template <typename T>
+// KP, RD and KB set the short-item pipeline (items per lane, levels ahead,
+// batch of further items): the deep form hides more latency per task, the
+// shallow one needs fewer registers and so keeps more tasks per CU when a
+// band has many of them.
+template <typename T, int KP, int RD, int KB>
 __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_off, int task0,
                                                      BandProg B, const int4* __restrict__ recs,
                                                      const uint2* __restrict__ upd,
@@ -760,7 +764,7 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
   // pipelined: records RD levels ahead, multi-update items' updates UD levels
   // ahead.  Items beyond KP per lane go in batches of KB whose loads are all
   // in flight together.  An absent item has w = -1 and z = 0.
-  constexpr int KP = 4, RD = 3, UD = 1, KB = 8;
+  constexpr int UD = 1;
   const int bd = (int)blockDim.x;
   const int tid = (int)threadIdx.x;
   auto fetch_recs = [&](int4 (&R)[KP], int jj) {
@@ -1385,8 +1389,14 @@ static void run_fact_bands(DevLU& L, T* lu, hipStream_t s) {
     if (ntask > 0) {
       ktimer_begin(s);
       const int nv = S.band_lds.empty() ? 0 : S.band_lds[b];
-      if (nv > 0 && L.use_lds)
-        k_lu_band_lds<T><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
+      const bool deep = L.lu_deep > 0 ? L.lu_deep == 2 : ntask <= 2 * L.n_cu;
+      if (nv > 0 && L.use_lds && deep)
+        k_lu_band_lds<T, 4, 3, 8><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
+            H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], B,
+            reinterpret_cast<const int4*>(L.lds_item.p), reinterpret_cast<const uint2*>(L.lds_upd.p),
+            reinterpret_cast<const int4*>(L.lds_task.p), L.lds_slots.p, lu);
+      else if (nv > 0 && L.use_lds)
+        k_lu_band_lds<T, 1, 2, 2><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
             H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], B,
             reinterpret_cast<const int4*>(L.lds_item.p), reinterpret_cast<const uint2*>(L.lds_upd.p),
             reinterpret_cast<const int4*>(L.lds_task.p), L.lds_slots.p, lu);

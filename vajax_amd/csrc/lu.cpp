@@ -182,6 +182,11 @@ int pattern_upload(vjx_pattern* pat) {
   // VJX_LU_LDS=0 keeps every LU band task in HBM (A/B measurements)
   const char* lds_env = std::getenv("VJX_LU_LDS");
   L.use_lds = !(lds_env && lds_env[0] == '0');
+  // VJX_LU_DEEP=1/2 forces the shallow / deep LU band pipeline (A/B
+  // measurements); by default bands with more than two tasks per CU take the
+  // shallow one
+  const char* deep_env = std::getenv("VJX_LU_DEEP");
+  L.lu_deep = deep_env && (deep_env[0] == '1' || deep_env[0] == '2') ? deep_env[0] - '0' : 0;
   // Solve band tasks run from LDS when everything they read fits the
   // device's LDS per workgroup (less the kernel's static arrays), else from
   // HBM; VJX_SOLVE_LDS=0 keeps every task in HBM (A/B measurements)
@@ -190,6 +195,7 @@ int pattern_upload(vjx_pattern* pat) {
   int dev = 0, lds_max = 0;
   VJX_TRY_HIP(hipGetDevice(&dev));
   VJX_TRY_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  VJX_TRY_HIP(hipDeviceGetAttribute(&L.n_cu, hipDeviceAttributeMultiprocessorCount, dev));
   L.fwd_band.lde_bytes = L.bwd_band.lde_bytes = std::max(0, lds_max - kTriStaticLds);
   const size_t nnz = s.col.size();
   const size_t n = (size_t)s.n;
