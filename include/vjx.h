@@ -154,6 +154,7 @@ int vjx_lu_download(vjx_pattern* pat, double* lu_host);
 #define VJX_MODEL_RESISTOR 0
 #define VJX_MODEL_CAPACITOR 1
 #define VJX_MODEL_MOSFET 2
+#define VJX_MODEL_DIODE 3 /* junction diode with $limit pnjlim: one limit state */
 #define VJX_MAX_MODELS 8
 
 /* One model group.  All arrays are column-major [k][count] (structure of
@@ -205,22 +206,43 @@ void vjx_circuit_destroy(vjx_circuit* c);
 /* Sparse LU handle bound to the circuit's CSR pattern (owned by the circuit). */
 vjx_pattern* vjx_circuit_pattern(vjx_circuit* c);
 
+/* Limit states of the circuit's limited devices (one per $limit call), flat
+ * over model groups in order, instances, states (mna_builder.py:387-401). */
+int vjx_circuit_limit_states(const vjx_circuit* c);
+
 /* build_system (mna_builder.py:528-596): evaluate all devices at X and
- * assemble csr_data, f (augmented), Q, I_vsource and max_res_contrib.
- * Device pointers; csr_data_dev may be NULL to skip the Jacobian. */
+ * assemble csr_data, f (augmented), Q, I_vsource and max_res_contrib; with
+ * limited devices also limit_state_out from limit_state_in (NULL: zeros) and
+ * the lim_rhs corrections of f (mna_builder.py:1039, mna.py:673-719).
+ * Device pointers; csr_data_dev may be NULL to skip the Jacobian,
+ * limit_state_out_dev may be NULL when the circuit has no limit states. */
 int vjx_build_system(vjx_circuit* c, const double* X_dev, const double* vsrc_dev,
                      const double* isrc_dev, const double* Q_prev_dev, const double* dQdt_prev_dev,
                      const double* Q_prev2_dev, double c0, double c1, double d1, double c2,
                      double gmin, double gshunt, double* csr_data_dev, double* f_dev, double* Q_dev,
-                     double* I_vsource_dev, double* max_res_contrib_dev);
+                     double* I_vsource_dev, double* max_res_contrib_dev,
+                     const double* limit_state_in_dev, double* limit_state_out_dev);
 
 /* Vectorised device eval for one model group (the openvaf_jax eval_fn
- * contract restricted to these models): branch voltages in, residuals and
- * Jacobians out, row-major per instance: res [count][n_res],
- * jac [count][n_jac].  params as in vjx_model_desc. */
+ * contract, openvaf_jax/codegen/function_builder.py:967-996, restricted to
+ * these models): branch voltages in, residuals and Jacobians out, row-major
+ * per instance: res [count][n_res], jac [count][n_jac].  params as in
+ * vjx_model_desc.  For a limited model (the diode) limit_state_in
+ * [count][n_lim] (NULL: zeros) gives each $limit's old value, and
+ * lim_rhs_resist / lim_rhs_react [count][n_res] and limit_state_out
+ * [count][n_lim] receive the limiting outputs (each may be NULL); the
+ * limiting arguments are ignored for the other models. */
 int vjx_model_eval(int32_t model_id, int32_t count, const double* vbranch_dev /*[n_v][count]*/,
-                   const double* params_dev, double* res_resist_dev, double* res_react_dev,
-                   double* jac_resist_dev, double* jac_react_dev);
+                   const double* params_dev, const double* limit_state_in_dev,
+                   double* res_resist_dev, double* res_react_dev, double* jac_resist_dev,
+                   double* jac_react_dev, double* lim_rhs_resist_dev, double* lim_rhs_react_dev,
+                   double* limit_state_out_dev);
+
+/* The limiting functions, elementwise on device arrays (vajax/analysis/
+ * limiting.py:22-107): kind 0 = pnjlim(vnew, vold, vt = p1, vcrit = p2),
+ * kind 1 = fetlim(vnew, vold, vto = p1). */
+int vjx_limit_eval(int32_t kind, int32_t n, const double* vnew_dev, const double* vold_dev,
+                   double p1, double p2, double* out_dev);
 
 /* Evaluate source waveforms at time t into vals_dev. */
 int vjx_eval_sources(vjx_circuit* c, double t, double* vsrc_dev, double* isrc_dev);

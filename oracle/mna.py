@@ -15,11 +15,14 @@ and ``_build_residual`` (mna_builder.py:1012-1076):
 1. current sources stamp +I at p and -I at n into f_resist and max|.| (:830-837);
 2. per model (in model order): branch voltages V[n1]-V[n2], eval, stamp
    ``jac_resist + c0*jac_react`` into csr_data, res_resist into f_resist and
-   max|.|, res_react into Q -- NaN -> 0 on every stamped value; sums are
-   sequential in (device, key) order, the order of XLA's scatter-add on CPU;
+   max|.|, res_react into Q, and for limited models lim_rhs_resist /
+   lim_rhs_react into their own sums (:965-966) -- NaN -> 0 on every stamped
+   value; sums are sequential in (device, key) order, the order of XLA's
+   scatter-add on CPU;
 3. csr_data[diag] += gmin_opt + gshunt; B/B^T +-1 (:969-989);
-4. I_vsource = -f_resist[p-1] (mna.py:564-596); f_resist += B*I_branch
-   (mna.py:489-536); f_node = f_resist - 0 + c0*(Q - 0) + c1*Q_prev
+4. f_resist -= lim_rhs_resist (:1039); I_vsource = -f_resist[p-1]
+   (mna.py:564-596); f_resist += B*I_branch (mna.py:489-536);
+   f_node = f_resist - 0 + c0*(Q - lim_rhs_react) + c1*Q_prev
    + d1*dQdt_prev + c2*Q_prev2, then + (gmin_opt+gshunt)*V[1:]
    (mna.py:673-719); f_branch = V[p] - V[n] - E (mna.py:539-561).
 
@@ -38,8 +41,11 @@ from vajax_amd.netlist import Netlist
 
 from . import models as M
 
-TERMS = {"resistor": 2, "capacitor": 2, "mosfet": 4}
-VPAIRS = {"resistor": [(0, 1)], "capacitor": [(0, 1)], "mosfet": [(1, 3), (2, 3), (0, 3)]}
+TERMS = {"resistor": 2, "capacitor": 2, "mosfet": 4, "diode": 2}
+VPAIRS = {"resistor": [(0, 1)], "capacitor": [(0, 1)], "mosfet": [(1, 3), (2, 3), (0, 3)],
+          "diode": [(0, 1)]}
+# limit states per instance (one per $limit call)
+N_LIM = {"resistor": 0, "capacitor": 0, "mosfet": 0, "diode": 1}
 
 
 @dataclass
@@ -50,6 +56,7 @@ class OracleModel:
     csr_pos: np.ndarray  # [N, T*T]
     params: Dict[str, np.ndarray]
     cache: Dict[str, np.ndarray]
+    lim_off: int = 0  # first limit state of this group in the flat array
 
 
 @dataclass
@@ -70,6 +77,7 @@ class OracleCircuit:
     b_pos: List[tuple]  # (csr position, value) in stamping order
     net: Netlist
     gmin: float = 1e-12
+    n_lim: int = 0  # limit states: model order, instance, state (mna_builder.py:387-401)
 
     @property
     def nse(self):
@@ -133,8 +141,13 @@ def setup(net: Netlist, gmin: float = 1e-12) -> OracleCircuit:
         for k, (rr, cc) in enumerate(rows):
             csr[:, k] = [pos_of[(a, b)] if a >= 0 else -1 for a, b in zip(rr.tolist(), cc.tolist())]
         res_idx = np.where(nodes > 0, nodes - 1, -1)
-        cache = M.mosfet_cache(grp.params) if mname == "mosfet" else {}
+        cache = (M.mosfet_cache(grp.params) if mname == "mosfet" else
+                 M.diode_cache(grp.params) if mname == "diode" else {})
         models.append(OracleModel(mname, nodes, res_idx, csr, grp.params, cache))
+    n_lim = 0
+    for om in models:
+        om.lim_off = n_lim
+        n_lim += N_LIM[om.name] * om.nodes.shape[0]
     diag_pos = np.array([pos_of[(i, i)] for i in range(n_unk)], dtype=np.int64)
     # B/B^T in the reference's stamping order: all b_p, all b_n, all bt_p, all bt_n
     b_pos = []
@@ -154,10 +167,21 @@ def setup(net: Netlist, gmin: float = 1e-12) -> OracleCircuit:
     is_p = gidx[net.isources.terminals[:, 0]] if k else np.zeros(0, np.int64)
     is_n = gidx[net.isources.terminals[:, 1]] if k else np.zeros(0, np.int64)
     return OracleCircuit(names, n_total, n_unk, m, n_aug, models, vs_p, vs_n, is_p, is_n,
-                         indptr, indices, diag_pos, b_pos, net, gmin)
+                         indptr, indices, diag_pos, b_pos, net, gmin, n_lim)
 
 
-def eval_model(om: OracleModel, V: np.ndarray):
+def eval_model(om: OracleModel, V: np.ndarray, lim_in=None):
+    """(res_resist, res_react, jac_resist, jac_react, lim_rhs_resist,
+    lim_rhs_react, limit_state_out) of one model group at node voltages V;
+    lim_in is the group's slice of the limit state."""
+    n = om.nodes
+    if om.name == "diode":
+        return M.eval_diode(V[n[:, 0]] - V[n[:, 1]], lim_in[:, 0], om.cache)
+    rr, rc, jr, jc = _eval_plain(om, V)
+    return rr, rc, jr, jc, None, None, None
+
+
+def _eval_plain(om: OracleModel, V: np.ndarray):
     n = om.nodes
     if om.name == "resistor":
         return M.eval_resistor(V[n[:, 0]] - V[n[:, 1]], om.params["r"])
@@ -177,7 +201,24 @@ def _nan0(x):
 
 def build_system(cir: OracleCircuit, X, vsrc, isrc, Q_prev, c0, gshunt=0.0, c1=0.0, d1=0.0,
                  dQdt_prev=None, c2=0.0, Q_prev2=None):
-    """Returns (csr_data, f_aug, Q, I_vsource, max_res_contrib)."""
+    """Returns (csr_data, f_aug, Q, I_vsource, max_res_contrib) with the limit
+    state at zero."""
+    return build_system_lim(cir, X, vsrc, isrc, Q_prev, c0, gshunt, c1, d1, dQdt_prev, c2,
+                            Q_prev2)[:5]
+
+
+def build_system_lim(cir: OracleCircuit, X, vsrc, isrc, Q_prev, c0, gshunt=0.0, c1=0.0, d1=0.0,
+                     dQdt_prev=None, c2=0.0, Q_prev2=None, lim_in=None):
+    """Returns (csr_data, f_aug, Q, I_vsource, max_res_contrib, limit_state_out).
+
+    lim_in (default zeros) is the flat limit state; the lim_rhs sums of the
+    limited models are scattered like their residuals and subtracted from
+    f_resist and, inside c0*(Q - .), from the charges (mna_builder.py:1039,
+    mna.py:673-719)."""
+    lim_in = np.zeros(cir.n_lim) if lim_in is None else np.asarray(lim_in, dtype=np.float64)
+    lim_out = np.zeros(cir.n_lim)
+    lim_r = np.zeros(cir.n_unk)
+    lim_q = np.zeros(cir.n_unk)
     n_total, n_unk = cir.n_total, cir.n_unk
     V = X[:n_total]
     I_branch = X[n_total:]
@@ -196,7 +237,12 @@ def build_system(cir: OracleCircuit, X, vsrc, isrc, Q_prev, c0, gshunt=0.0, c1=0
         np.maximum.at(mrc, idx[ok], _nan0(np.abs(val[ok])))
 
     for om in cir.models:
-        rr, rc, jr, jc = eval_model(om, V)
+        N = om.nodes.shape[0]
+        nl = N_LIM[om.name]
+        li = lim_in[om.lim_off:om.lim_off + N * nl].reshape(N, nl) if nl else None
+        rr, rc, jr, jc, lr, lc, lo = eval_model(om, V, li)
+        if nl:
+            lim_out[om.lim_off:om.lim_off + N * nl] = lo.ravel()
         comb = jr + c0 * jc
         pos = om.csr_pos.ravel()
         ok = pos >= 0
@@ -206,12 +252,15 @@ def build_system(cir: OracleCircuit, X, vsrc, isrc, Q_prev, c0, gshunt=0.0, c1=0
         np.add.at(f_res, ridx[ok], _nan0(rr.ravel()[ok]))
         np.maximum.at(mrc, ridx[ok], _nan0(np.abs(rr.ravel()[ok])))
         np.add.at(Q, ridx[ok], _nan0(rc.ravel()[ok]))
+        if lr is not None:
+            np.add.at(lim_r, ridx[ok], _nan0(lr.ravel()[ok]))
+            np.add.at(lim_q, ridx[ok], _nan0(lc.ravel()[ok]))
 
     csr[cir.diag_pos] += cir.gmin + gshunt
     for p, v in cir.b_pos:
         csr[p] += v
 
-    f_res = f_res - np.zeros(n_unk)
+    f_res = f_res - lim_r
     I_vs = np.where(cir.vs_p > 0, -f_res[np.maximum(cir.vs_p - 1, 0)], 0.0) if cir.m else np.zeros(0)
     if cir.m:
         contrib = np.zeros(n_unk)
@@ -223,11 +272,11 @@ def build_system(cir: OracleCircuit, X, vsrc, isrc, Q_prev, c0, gshunt=0.0, c1=0
         f_res = f_res + contrib
     dQ = dQdt_prev if dQdt_prev is not None else np.zeros(n_unk)
     Q2 = Q_prev2 if Q_prev2 is not None else np.zeros(n_unk)
-    f_node = (f_res - np.zeros(n_unk) + c0 * (Q - np.zeros(n_unk)) + c1 * Q_prev
+    f_node = (f_res - np.zeros(n_unk) + c0 * (Q - lim_q) + c1 * Q_prev
               + d1 * dQ + c2 * Q2)
     f_node = f_node + (cir.gmin + gshunt) * V[1:]
     f_branch = V[cir.vs_p] - V[cir.vs_n] - np.asarray(vsrc) if cir.m else np.zeros(0)
-    return csr, np.concatenate([f_node, f_branch]), Q, I_vs, mrc
+    return csr, np.concatenate([f_node, f_branch]), Q, I_vs, mrc, lim_out
 
 
 def csr_matrix(cir: OracleCircuit, data):

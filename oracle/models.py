@@ -2,8 +2,10 @@
 
 Each ``eval_*`` returns the openvaf_jax eval contract restricted to what these
 models produce: ``(res_resist [N,R], res_react [N,R], jac_resist [N,J],
-jac_react [N,J])`` -- lim_rhs, small-signal and limit-state outputs are zero
-(openvaf_jax/codegen/function_builder.py:967-996).  Residual node order is the
+jac_react [N,J])``; the diode, the one model with a ``$limit``, also returns
+``lim_rhs_resist [N,R], lim_rhs_react [N,R], limit_state_out [N,1]``
+(openvaf_jax/codegen/function_builder.py:967-996); for the others lim_rhs,
+small-signal and limit-state outputs are zero.  Residual node order is the
 terminal order; Jacobian keys are all (row, col) terminal pairs, row-major.
 
 * resistor: ``I(A,B) <+ V(A,B)/r`` (vajax/devices/models/resistor.va:23-27).
@@ -19,6 +21,13 @@ terminal order; Jacobian keys are all (row, col) terminal pairs, row-major.
   PSP103 is not available, so MOSFET arithmetic against the reference's model
   is parity unpinned; the formula itself is pinned by closed-form and
   finite-difference tests.
+* diode: the junction diode as a Verilog-A model with ``$limit`` compiles
+  under openvaf_jax (instruction.py:628-660, 705-870;
+  function_builder.py:1126-1212):
+  vd = V(A,K); vdl = $limit(vd, "pnjlim", n*vt, vcrit) with
+  vcrit = n*vt*ln(n*vt/(sqrt(2)*is)) and vold = limit_state_in;
+  I(A,K) <+ is*(exp(vdl/(n*vt)) - 1) + ddt(cj*vdl); limit_state_out = vdl;
+  lim_rhs[i] = J[i, A] * (vdl - vd) for the resistive and reactive parts.
 """
 
 from __future__ import annotations
@@ -50,6 +59,33 @@ def eval_capacitor(v_ab: np.ndarray, c: np.ndarray):
     jac_c = np.stack([c, -c, -c, c], axis=1)
     jac_r = np.zeros_like(jac_c)
     return res_r, res_c, jac_r, jac_c
+
+
+def diode_cache(p: dict) -> dict:
+    """Per-instance constants of the diode (its init cache)."""
+    nvt = p["n"] * VT
+    return {"is": np.asarray(p["is"], dtype=np.float64), "nvt": nvt,
+            "vcrit": nvt * np.log(nvt / (np.sqrt(2.0) * p["is"])),
+            "cj": np.asarray(p["cj"], dtype=np.float64)}
+
+
+def eval_diode(vd: np.ndarray, vold: np.ndarray, k: dict):
+    from .limiting import pnjlim
+
+    vdl = pnjlim(vd, vold, k["nvt"], k["vcrit"])
+    e = np.exp(vdl / k["nvt"])
+    i = k["is"] * (e - 1.0)
+    g = k["is"] * e / k["nvt"]
+    q = k["cj"] * vdl
+    cj = k["cj"]
+    dl = vdl - vd
+    res_r = np.stack([i, -i], axis=1)
+    res_c = np.stack([q, -q], axis=1)
+    jac_r = np.stack([g, -g, -g, g], axis=1)
+    jac_c = np.stack([cj, -cj, -cj, cj], axis=1)
+    lim_r = np.stack([g * dl, -(g * dl)], axis=1)
+    lim_c = np.stack([cj * dl, -(cj * dl)], axis=1)
+    return res_r, res_c, jac_r, jac_c, lim_r, lim_c, vdl[:, None]
 
 
 def mosfet_cache(p: dict) -> dict:

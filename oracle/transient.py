@@ -6,12 +6,18 @@
   (|d| < max(|X|*reltol, vntol | abstol), skipped at iteration 0) -> X += d;
   stop when both tests pass or after ``max_iterations``; then one more build
   at X_final for Q, I_vsource and max_res_contrib, and
-  dQdt = c0*Q + c1*Q_prev + d1*dQdt_prev + c2*Q_prev2.
+  dQdt = c0*Q + c1*Q_prev + d1*dQdt_prev + c2*Q_prev2.  With limited devices
+  the limit state of each iteration's build feeds the next, and convergence
+  also needs max|ls_out - ls_in| < max(max|ls_in|*reltol, vntol) and an
+  iteration index >= 1 (solver_factories.py:317-323); the final build reads
+  the last iteration's state.
 * ``predict`` / ``lte_timestep`` -- adaptive.py:94-218 and 221-339.
 * ``run_adaptive`` -- FullMNAStrategy.run (full_mna.py:780-1196) and body_fn
   (full_mna.py:1840-2194), including tran_minpts / tran_fs handling, the
   first-step zero history, warm-up, NR-failure cuts, min_dt force-accept,
-  LTE rejection with the consecutive-reject cap, and the output rule.
+  LTE rejection with the consecutive-reject cap, and the output rule.  The
+  limit state starts at zero (full_mna.py:1028-1033; the DC solve's is not
+  kept) and advances only with a new solution (full_mna.py:2162-2167).
 * ``run_fixed`` -- _run_fixed_step (full_mna.py:1198-1441).
 
 The linear solve is SuperLU (scipy, full partial pivoting) on the CSR matrix -- the sparse paths of
@@ -129,11 +135,12 @@ class NRResult:
     dQdt: np.ndarray
     I_vsource: np.ndarray
     max_res_contrib: np.ndarray
+    lim: np.ndarray = None  # limit state after the last iteration
 
 
 def nr_solve(cir, solver: LinearSolver, opts: Options, X_init, vsrc, isrc, Q_prev, c0,
              gshunt=0.0, c1=0.0, d1=0.0, dQdt_prev=None, c2=0.0, Q_prev2=None,
-             res_tol_floor=None, max_iterations=None) -> NRResult:
+             res_tol_floor=None, max_iterations=None, lim_in=None) -> NRResult:
     n_unk, m = cir.n_unk, cir.m
     n_total = cir.n_total
     max_it = opts.tran_itl if max_iterations is None else max_iterations
@@ -142,11 +149,13 @@ def nr_solve(cir, solver: LinearSolver, opts: Options, X_init, vsrc, isrc, Q_pre
     floor = res_tol_floor if res_tol_floor is not None else np.full(n_unk, opts.abstol)
     delta_abs = np.concatenate([np.full(n_unk, opts.vntol), np.full(m, opts.abstol)])
     X = np.array(X_init, dtype=np.float64)
+    lim = np.zeros(cir.n_lim) if lim_in is None else np.array(lim_in, dtype=np.float64)
     it = 0
     converged = False
     max_f = math.inf
     while (not converged) and it < max_it:
-        csr, f, Q, _, mrc = mna.build_system(cir, X, vsrc, isrc, Q_prev, c0, gshunt, c1, d1, dQ, c2, Q2)
+        csr, f, Q, _, mrc, lim_out = mna.build_system_lim(cir, X, vsrc, isrc, Q_prev, c0, gshunt,
+                                                          c1, d1, dQ, c2, Q2, lim)
         res_tol = np.concatenate([np.maximum(mrc * opts.reltol, floor), np.full(m, opts.vntol)])
         max_f = float(np.max(np.abs(f))) if f.size else 0.0
         res_ok = bool(np.all(np.abs(f) < res_tol))
@@ -162,10 +171,16 @@ def nr_solve(cir, solver: LinearSolver, opts: Options, X_init, vsrc, isrc, Q_pre
         X[1:n_total] += Vd * v_scale * opts.nr_damping
         X[n_total:] += delta[n_unk:]
         converged = res_ok and delta_ok
+        if cir.n_lim:
+            lim_delta = float(np.max(np.abs(lim_out - lim)))
+            lim_ref = max(float(np.max(np.abs(lim))) * opts.reltol, opts.vntol)
+            converged = converged and lim_delta < lim_ref and it >= 1
+        lim = lim_out
         it += 1
-    _, _, Qf, I_vs, mrc = mna.build_system(cir, X, vsrc, isrc, Q_prev, c0, gshunt, c1, d1, dQ, c2, Q2)
+    _, _, Qf, I_vs, mrc, _ = mna.build_system_lim(cir, X, vsrc, isrc, Q_prev, c0, gshunt, c1, d1,
+                                                  dQ, c2, Q2, lim)
     dQdt = c0 * Qf + c1 * Q_prev + d1 * dQ + c2 * Q2
-    return NRResult(X, it, converged, max_f, Qf, dQdt, I_vs, mrc)
+    return NRResult(X, it, converged, max_f, Qf, dQdt, I_vs, mrc, lim)
 
 
 def predict(V_hist, dt_hist, hist_count, new_dt, max_order):
@@ -291,6 +306,7 @@ def run_adaptive(cir, t_stop: float, dt: float, cfg: AdaptiveConfig, opts: Optio
     total_nr = rejected = consec = 0
     min_dt_used = max_dt_used = dt
     V_max_hist = np.abs(X[:n_total])
+    lim = np.zeros(cir.n_lim)
     method = method_of(cfg.integration_method)
     t0 = time.perf_counter()
     iters = 0
@@ -333,7 +349,7 @@ def run_adaptive(cir, t_stop: float, dt: float, cfg: AdaptiveConfig, opts: Optio
         gshunt = cfg.gshunt_init + ramp * (cfg.gshunt_target - cfg.gshunt_init)
         floor = np.maximum(hist_mrc * cfg.reltol, cfg.abstol)
         r = nr_solve(cir, solver, opts, X_init, vs, isv, Q_prev, c0, gshunt, c1, d1, dQdt_prev, c2,
-                     Q_prev2, res_tol_floor=floor)
+                     Q_prev2, res_tol_floor=floor, lim_in=lim)
         total_nr += r.iterations
         nr_failed = not r.converged
         at_min = dt_cur <= cfg.min_dt
@@ -361,6 +377,7 @@ def run_adaptive(cir, t_stop: float, dt: float, cfg: AdaptiveConfig, opts: Optio
             t = t_next
         if use_new:
             X = r.X.copy()
+            lim = r.lim
             Q_prev2 = Q_prev
             Q_prev = r.Q
             dQdt_prev = r.dQdt
@@ -424,6 +441,7 @@ def run_fixed(cir, t_stop: float, dt: float, method: str, opts: Options, tran_fs
     dQdt_prev = np.zeros(n_unk)
     Q_prev2 = np.zeros(n_unk)
     floor = np.full(n_unk, abstol)
+    lim = np.zeros(cir.n_lim)
     total_nr = 0
     t0 = time.perf_counter()
     for i in range(n_steps):
@@ -431,7 +449,9 @@ def run_fixed(cir, t_stop: float, dt: float, method: str, opts: Options, tran_fs
         vs = source_values(cir.net.vsources, t_next)
         isv = source_values(cir.net.isources, t_next)
         r = nr_solve(cir, solver, opts, X, vs, isv, Q_prev, c0, 0.0, c1, d1, dQdt_prev, c2, Q_prev2,
-                     res_tol_floor=floor)
+                     res_tol_floor=floor, lim_in=lim)
+        if r.converged:  # full_mna.py:1349-1353
+            lim = r.lim
         total_nr += r.iterations
         times[i + 1] = t_next
         Vout[i + 1] = r.X[:n_total]

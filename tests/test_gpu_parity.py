@@ -71,7 +71,8 @@ def test_model_eval_contract(torch, model):
     outs = [torch.empty((n, nr), dtype=torch.float64, device="cuda") for _ in range(2)]
     outs += [torch.empty((n, nj), dtype=torch.float64, device="cuda") for _ in range(2)]
     rc = L.lib.vjx_model_eval(L.MODEL_IDS[model], n, C.c_void_p(vbd.data_ptr()),
-                              C.c_void_p(par.data_ptr()), *[C.c_void_p(o.data_ptr()) for o in outs])
+                              C.c_void_p(par.data_ptr()), None,
+                              *[C.c_void_p(o.data_ptr()) for o in outs], None, None, None)
     assert rc == 0, L.last_error()
     for got, want in zip(outs, ref):
         got = got.cpu().numpy()

@@ -19,7 +19,7 @@ VJX_WARN_NOCONV = 2
 VJX_PREC_F64 = 0
 VJX_PREC_F32 = 1
 VJX_METHOD = {"be": 0, "trap": 1, "gear2": 2}
-MODEL_IDS = {"resistor": 0, "capacitor": 1, "mosfet": 2}
+MODEL_IDS = {"resistor": 0, "capacitor": 1, "mosfet": 2, "diode": 3}
 MAX_MODELS = 8
 
 i32p = C.POINTER(C.c_int32)
@@ -147,10 +147,12 @@ _PROTOS = {
     "vjx_circuit_create": (C.c_int, [C.POINTER(CircuitDesc), C.POINTER(vp)]),
     "vjx_circuit_destroy": (None, [vp]),
     "vjx_circuit_pattern": (vp, [vp]),
+    "vjx_circuit_limit_states": (C.c_int, [vp]),
     "vjx_build_system": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp,
-                                   vp, vp]),
-    "vjx_model_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp]),
+                                   vp, vp, vp, vp]),
+    "vjx_model_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "vjx_limit_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, C.c_double, C.c_double, vp]),
     "vjx_eval_sources": (C.c_int, [vp, C.c_double, vp, vp]),
     "vjx_transient_run": (C.c_int, [vp, C.POINTER(TranConfig), C.POINTER(TranResult)]),
     "vjx_profile_enable": (C.c_int, [C.c_int32]),

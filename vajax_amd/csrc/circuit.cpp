@@ -15,8 +15,15 @@ namespace vjx {
 void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
                          const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
                          const double* res_floor, double* csr, double* f, double* Q, double* mrc,
-                         double* I_vs, DevStatus* st, hipStream_t s) {
+                         double* I_vs, DevStatus* st, const double* ls_in, double* ls_out,
+                         hipStream_t s) {
   EvalOut out{csr ? c->jac_contrib.p : nullptr, c->rf.p, c->rq.p};
+  if (c->n_lim > 0) {
+    out.lr = c->lr.p;
+    out.lq = c->lq.p;
+    out.ls_in = ls_in;
+    out.ls_out = ls_out ? ls_out : c->ls_scratch.p;
+  }
   for (auto& m : c->models) launch_eval(*m, X, c->d_sc.p, out, s);
   launch_isrc_contrib(c->k, isrc, c->is_dst.p, c->rf.p, c->rq.p, s);
   if (csr)
@@ -29,6 +36,8 @@ void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, co
   a.res_seg = c->res_seg.p;
   a.rf = c->rf.p;
   a.rq = c->rq.p;
+  a.lr = c->n_lim > 0 ? c->lr.p : nullptr;
+  a.lq = c->n_lim > 0 ? c->lq.p : nullptr;
   a.br_ptr = c->br_ptr.p;
   a.br_ent = c->br_ent.p;
   a.vs_p = c->vs_p.p;
@@ -87,8 +96,12 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
   VJX_TRY_HIP(c->slot_b.upload(d->slot_b, c->nse));
   VJX_TRY_HIP(c->jac_seg.upload(d->jac_seg, c->nse + 1));
   VJX_TRY_HIP(c->res_seg.upload(d->res_seg, nu + 1));
+  for (int i = 0; i < d->n_models; ++i)
+    if (d->models[i].model_id >= 0 && d->models[i].model_id <= VJX_MODEL_DIODE &&
+        d->models[i].count > 0)
+      c->n_lim += model_n_lim(d->models[i].model_id) * d->models[i].count;
   VJX_TRY_HIP(c->hub_slots.build(d->jac_seg, c->nse, 1));
-  VJX_TRY_HIP(c->hub_rows.build(d->res_seg, nu, 3));
+  VJX_TRY_HIP(c->hub_rows.build(d->res_seg, nu, c->n_lim > 0 ? 5 : 3));
   VJX_TRY_HIP(c->br_ptr.upload(d->br_ptr, nu + 1));
   VJX_TRY_HIP(c->br_ent.upload(d->br_ent, d->br_ptr[nu]));
   VJX_TRY_HIP(c->vs_p.upload(d->vs_node_p, m));
@@ -111,9 +124,18 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
   VJX_TRY_HIP(c->jac_contrib.alloc(d->n_jac_contrib));
   VJX_TRY_HIP(c->rf.alloc(d->n_res_contrib));
   VJX_TRY_HIP(c->rq.alloc(d->n_res_contrib));
+  if (c->n_lim > 0) {
+    // slots of unlimited models keep their zero lim_rhs
+    VJX_TRY_HIP(c->lr.alloc(d->n_res_contrib));
+    VJX_TRY_HIP(c->lq.alloc(d->n_res_contrib));
+    VJX_TRY_HIP(c->ls_scratch.alloc(c->n_lim));
+    VJX_TRY_HIP(hipMemset(c->lr.p, 0, sizeof(double) * d->n_res_contrib));
+    VJX_TRY_HIP(hipMemset(c->lq.p, 0, sizeof(double) * d->n_res_contrib));
+  }
+  int64_t lim_off = 0;
   for (int i = 0; i < d->n_models; ++i) {
     const vjx_model_desc& md = d->models[i];
-    if (md.model_id < 0 || md.model_id > VJX_MODEL_MOSFET || md.count < 0) {
+    if (md.model_id < 0 || md.model_id > VJX_MODEL_DIODE || md.count < 0) {
       set_error("vjx_circuit_create: unknown model id");
       return VJX_ERR_ARG;
     }
@@ -124,6 +146,9 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
     dm->n_jac = model_n_jac(md.model_id);
     dm->n_res = model_n_res(md.model_id);
     dm->n_par = model_n_par(md.model_id);
+    dm->n_lim = model_n_lim(md.model_id);
+    dm->lim_off = lim_off;
+    lim_off += (int64_t)dm->n_lim * md.count;
     const size_t N = (size_t)md.count;
     // bounds checks on every index the kernels will follow
     for (size_t e = 0; e < N * dm->n_v; ++e)
@@ -178,10 +203,13 @@ void vjx_circuit_destroy(vjx_circuit* c) { delete c; }
 
 vjx_pattern* vjx_circuit_pattern(vjx_circuit* c) { return c ? c->pat : nullptr; }
 
+int vjx_circuit_limit_states(const vjx_circuit* c) { return c ? c->n_lim : 0; }
+
 int vjx_build_system(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
                      const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
                      double c0, double c1, double d1, double c2, double gmin, double gshunt,
-                     double* csr, double* f, double* Q, double* I_vs, double* mrc) {
+                     double* csr, double* f, double* Q, double* I_vs, double* mrc,
+                     const double* ls_in, double* ls_out) {
   if (!c || !X || !f || (c->m && (!vsrc || !I_vs)) || (c->k && !isrc)) {
     set_error("vjx_build_system: null argument");
     return VJX_ERR_ARG;
@@ -202,15 +230,16 @@ int vjx_build_system(vjx_circuit* c, const double* X, const double* vsrc, const 
   h->iteration = 0;
   VJX_TRY_HIP(hipMemcpyAsync(c->d_sc.p, h, sizeof(DevScalars), hipMemcpyHostToDevice, s));
   circuit_build_async(c, X, vsrc, isrc, Q_prev, dQdt_prev, Q_prev2, nullptr, csr, f, Q, mrc, I_vs,
-                      nullptr, s);
+                      nullptr, ls_in, ls_out, s);
   VJX_TRY_HIP(hipGetLastError());
   VJX_TRY_HIP(hipStreamSynchronize(s));
   return VJX_OK;
 }
 
 int vjx_model_eval(int32_t model_id, int32_t count, const double* vb, const double* par,
-                   double* rr, double* rc, double* jr, double* jc) {
-  if (model_id < 0 || model_id > VJX_MODEL_MOSFET || count < 0) {
+                   const double* li, double* rr, double* rc, double* jr, double* jc, double* lr,
+                   double* lc, double* lo) {
+  if (model_id < 0 || model_id > VJX_MODEL_DIODE || count < 0) {
     set_error("vjx_model_eval: bad model id or count");
     return VJX_ERR_ARG;
   }
@@ -218,7 +247,19 @@ int vjx_model_eval(int32_t model_id, int32_t count, const double* vb, const doub
     set_error("vjx_model_eval: null argument");
     return VJX_ERR_ARG;
   }
-  launch_model_eval_raw(model_id, count, vb, par, rr, rc, jr, jc, 0);
+  launch_model_eval_raw(model_id, count, vb, par, li, rr, rc, jr, jc, lr, lc, lo, 0);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipDeviceSynchronize());
+  return VJX_OK;
+}
+
+int vjx_limit_eval(int32_t kind, int32_t n, const double* vnew, const double* vold, double p1,
+                   double p2, double* out) {
+  if ((kind != 0 && kind != 1) || n < 0 || (n && (!vnew || !vold || !out))) {
+    set_error("vjx_limit_eval: kind must be 0 (pnjlim) or 1 (fetlim), arrays non-null");
+    return VJX_ERR_ARG;
+  }
+  launch_limit(kind, n, vnew, vold, p1, p2, out, 0);
   VJX_TRY_HIP(hipGetLastError());
   VJX_TRY_HIP(hipDeviceSynchronize());
   return VJX_OK;

@@ -38,6 +38,11 @@ struct vjx_circuit {
   std::vector<double> h_vs_par, h_is_par;
   int n_jac_contrib = 0, n_res_contrib = 0;
   vjx::DBuf<double> jac_contrib, rf, rq;
+  // limited devices: flat limit-state count, lim_rhs contributions (same
+  // slots as rf / rq, zero where no limited device writes) and a scratch
+  // limit-state output for builds whose caller discards it
+  int n_lim = 0;
+  vjx::DBuf<double> lr, lq, ls_scratch;
   vjx_pattern* pat = nullptr;
   vjx::DBuf<vjx::DevScalars> d_sc;
   vjx::DevScalars* h_sc = nullptr;  // pinned
@@ -66,9 +71,11 @@ void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const dou
 void launch_status_reset(vjx::DevStatus* st, hipStream_t s);
 
 // Enqueue one system build: eval, stamping reductions, residual.
+// ls_in (NULL: zeros) / ls_out: flat limit states of the limited devices.
 void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
                          const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
                          const double* res_floor, double* csr, double* f, double* Q, double* mrc,
-                         double* I_vs, vjx::DevStatus* st, hipStream_t s);
+                         double* I_vs, vjx::DevStatus* st, const double* ls_in, double* ls_out,
+                         hipStream_t s);
 
 }  // namespace vjx

@@ -67,6 +67,8 @@ struct DevStatus {
   unsigned long long maxdv_bits;
   unsigned long long maxdelta_bits;
   unsigned long long maxl_bits;
+  unsigned long long limd_bits;  // max |limit_state_out - limit_state_in|
+  unsigned long long limr_bits;  // max |limit_state_in|
   unsigned int res_fail;
   unsigned int delta_fail;
   unsigned int nonfinite;
@@ -77,6 +79,8 @@ struct DevModel {
   int32_t model_id = 0;
   int32_t count = 0;
   int32_t n_v = 0, n_jac = 0, n_res = 0, n_par = 0;
+  int32_t n_lim = 0;    // limit states per instance
+  int64_t lim_off = 0;  // first limit state of the group in the flat array
   DBuf<int32_t> vn1, vn2, jac_dst, res_dst;
   DBuf<double> par;
 };
@@ -138,15 +142,17 @@ struct HubChunks {
   double* part = nullptr;  // per-chunk partials (sum)
   double* part2 = nullptr;  // second sum (residual rows: charge)
   double* part3 = nullptr;  // max |.| (residual rows)
+  double* part4 = nullptr;  // lim_rhs sums (residual rows of limited circuits)
+  double* part5 = nullptr;
   int n_items = 0;
   int n_chunks = 0;
 };
 
 struct HubPlan {
   DBuf<int32_t> item, ptr, cs, ce;
-  DBuf<double> part, part2, part3;
+  DBuf<double> part, part2, part3, part4, part5;
   int n_items = 0, n_chunks = 0;
-  // seg: host segment pointers of n segments; nparts: 1 or 3 partial arrays.
+  // seg: host segment pointers of n segments; nparts: 1, 3 or 5 partial arrays.
   hipError_t build(const int32_t* seg, int n, int nparts);
   HubChunks view() const {
     HubChunks h;
@@ -157,6 +163,8 @@ struct HubPlan {
     h.part = part.p;
     h.part2 = part2.p;
     h.part3 = part3.p;
+    h.part4 = part4.p;
+    h.part5 = part5.p;
     h.n_items = n_items;
     h.n_chunks = n_chunks;
     return h;
@@ -188,6 +196,10 @@ struct EvalOut {
   double* jac;  // contribution arrays (sorted slots), may be null
   double* rf;
   double* rq;
+  double* lr = nullptr;  // lim_rhs contributions (resistive, reactive) of limited models
+  double* lq = nullptr;
+  const double* ls_in = nullptr;  // flat limit state in / out (limited models)
+  double* ls_out = nullptr;
 };
 
 void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalOut out,
@@ -202,6 +214,8 @@ struct ResidualArgs {
   const int32_t* res_seg;
   const double* rf;
   const double* rq;
+  const double* lr;  // lim_rhs contributions, or null when no model limits
+  const double* lq;
   const int32_t* br_ptr;
   const int32_t* br_ent;
   const int32_t* vs_p;
@@ -231,11 +245,17 @@ void launch_lu_solve(DevLU& lu, const T* lu_vals, const double* rhs, double rhs_
 void launch_axpy(int n, double* x, const double* d, hipStream_t s);
 
 void launch_model_eval_raw(int model_id, int count, const double* vb, const double* par,
-                           double* rr, double* rc, double* jr, double* jc, hipStream_t s);
+                           const double* li, double* rr, double* rc, double* jr, double* jc,
+                           double* lr, double* lc, double* lo, hipStream_t s);
+void launch_limit(int kind, int n, const double* vnew, const double* vold, double p1, double p2,
+                  double* out, hipStream_t s);
+void launch_lim_stats(int n, const double* ls_in, const double* ls_out, DevStatus* st,
+                      hipStream_t s);
 void launch_sources(int m, const int32_t* wave, const double* params, double t, double* out,
                     hipStream_t s);
 
 int model_n_v(int model_id);
+int model_n_lim(int model_id);
 int model_n_res(int model_id);
 int model_n_jac(int model_id);
 int model_n_par(int model_id);

@@ -43,15 +43,19 @@ template <int MID>
 struct ModelTraits;
 template <>
 struct ModelTraits<VJX_MODEL_RESISTOR> {
-  static constexpr int NV = 1, NR = 2, NJ = 4, NP = 1;
+  static constexpr int NV = 1, NR = 2, NJ = 4, NP = 1, NL = 0;
 };
 template <>
 struct ModelTraits<VJX_MODEL_CAPACITOR> {
-  static constexpr int NV = 1, NR = 2, NJ = 4, NP = 1;
+  static constexpr int NV = 1, NR = 2, NJ = 4, NP = 1, NL = 0;
 };
 template <>
 struct ModelTraits<VJX_MODEL_MOSFET> {
-  static constexpr int NV = 3, NR = 4, NJ = 16, NP = 11;
+  static constexpr int NV = 3, NR = 4, NJ = 16, NP = 11, NL = 0;
+};
+template <>
+struct ModelTraits<VJX_MODEL_DIODE> {
+  static constexpr int NV = 1, NR = 2, NJ = 4, NP = 4, NL = 1;
 };
 
 // resistor.va:23-27 -- I(A,B) <+ V(A,B)/r
@@ -197,12 +201,76 @@ __device__ __forceinline__ void eval_mosfet(const double* vb, const double* par,
   jc[15] = cdb + csb + cgb;
 }
 
+// SPICE junction-voltage limiting, vajax/analysis/limiting.py:22-82 (VACASK
+// DEVpnjlim): each branch as written there, logs of max(., 1e-30).
+__device__ __forceinline__ double pnjlim(double vnew, double vold, double vt, double vcrit) {
+  const double delta = vnew - vold;
+  const double arg = delta / vt;
+  if (vnew > vcrit && fabs(delta) > 2 * vt) {
+    if (vold > 0) {
+      return arg >= 0 ? vold + vt * log(fmax(1.0 + arg, 1e-30))
+                      : vold - vt * log(fmax(1.0 - arg, 1e-30));
+    }
+    return vnew > 0 ? vt * log(fmax(vnew / vt, 1e-30)) : vcrit;
+  }
+  if (vnew < 0) return fmax(vnew, vold > 0 ? -vold - 1.0 : 2.0 * vold - 1.0);
+  return vnew;
+}
+
+// FET gate-voltage limiting, limiting.py:85-107.
+__device__ __forceinline__ double fetlim(double vnew, double vold, double vto) {
+  const double step = vold >= vto ? 2 * fabs(vold - vto) + 2 : 0.5;
+  return vold + fmin(fmax(vnew - vold, -step), step);
+}
+
+// Junction diode with $limit: vdl = pnjlim(V(A,K), limit_state_in, nvt,
+// vcrit); I(A,K) <+ is*(exp(vdl/nvt) - 1) + ddt(cj*vdl); limit_state_out =
+// vdl; lim_rhs[r] = J[r, A] * (vdl - V(A,K)) for the resistive and reactive
+// Jacobians (openvaf_jax/codegen/function_builder.py:1126-1212).
+// Parameter columns: is, nvt, vcrit, cj.
+__device__ __forceinline__ void eval_diode(const double* vb, const double* par, int i, int N,
+                                           const double* li, double* rr, double* rc, double* jr,
+                                           double* jc, double* lr, double* lc, double* lo) {
+  const double is = par[0 * (size_t)N + i];
+  const double nvt = par[1 * (size_t)N + i];
+  const double vcrit = par[2 * (size_t)N + i];
+  const double cj = par[3 * (size_t)N + i];
+  const double vd = vb[0];
+  const double vdl = pnjlim(vd, li[0], nvt, vcrit);
+  const double e = exp(vdl / nvt);
+  const double cur = is * (e - 1.0);
+  const double g = is * e / nvt;
+  const double q = cj * vdl;
+  const double dl = vdl - vd;
+  rr[0] = cur;
+  rr[1] = -cur;
+  rc[0] = q;
+  rc[1] = -q;
+  jr[0] = g;
+  jr[1] = -g;
+  jr[2] = -g;
+  jr[3] = g;
+  jc[0] = cj;
+  jc[1] = -cj;
+  jc[2] = -cj;
+  jc[3] = cj;
+  lr[0] = g * dl;
+  lr[1] = -(g * dl);
+  lc[0] = cj * dl;
+  lc[1] = -(cj * dl);
+  lo[0] = vdl;
+}
+
+// One instance of model MID; li / lr / lc / lo (limit state in, lim_rhs
+// resistive / reactive, limit state out) are used by limited models only.
 template <int MID>
 __device__ __forceinline__ void eval_model(const double* vb, const double* par, int i, int N,
-                                           double* rr, double* rc, double* jr, double* jc) {
+                                           const double* li, double* rr, double* rc, double* jr,
+                                           double* jc, double* lr, double* lc, double* lo) {
   if constexpr (MID == VJX_MODEL_RESISTOR) eval_resistor(vb, par, i, N, rr, rc, jr, jc);
   if constexpr (MID == VJX_MODEL_CAPACITOR) eval_capacitor(vb, par, i, N, rr, rc, jr, jc);
   if constexpr (MID == VJX_MODEL_MOSFET) eval_mosfet(vb, par, i, N, rr, rc, jr, jc);
+  if constexpr (MID == VJX_MODEL_DIODE) eval_diode(vb, par, i, N, li, rr, rc, jr, jc, lr, lc, lo);
 }
 
 // Eval + stamp: lane = device instance.
@@ -214,8 +282,10 @@ __global__ void __launch_bounds__(256) k_eval(int N, const int32_t* __restrict__
                                               const double* __restrict__ par,
                                               const double* __restrict__ X,
                                               const DevScalars* __restrict__ sc, double* jac,
-                                              double* rf, double* rq) {
+                                              double* rf, double* rq, double* lrf, double* lrq,
+                                              const double* __restrict__ ls_in, double* ls_out) {
   using Tr = ModelTraits<MID>;
+  constexpr int NL = Tr::NL > 0 ? Tr::NL : 1;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double vb[Tr::NV];
@@ -223,7 +293,24 @@ __global__ void __launch_bounds__(256) k_eval(int N, const int32_t* __restrict__
   for (int k = 0; k < Tr::NV; ++k)
     vb[k] = X[vn1[(size_t)k * N + i]] - X[vn2[(size_t)k * N + i]];
   double rr[Tr::NR], rc[Tr::NR], jr[Tr::NJ], jc[Tr::NJ];
-  eval_model<MID>(vb, par, i, N, rr, rc, jr, jc);
+  double lr[Tr::NR], lc[Tr::NR], li[NL], lo[NL];
+  if constexpr (Tr::NL > 0) {
+#pragma unroll
+    for (int s = 0; s < Tr::NL; ++s) li[s] = ls_in ? ls_in[(size_t)i * Tr::NL + s] : 0.0;
+  }
+  eval_model<MID>(vb, par, i, N, li, rr, rc, jr, jc, lr, lc, lo);
+  if constexpr (Tr::NL > 0) {
+#pragma unroll
+    for (int s = 0; s < Tr::NL; ++s) ls_out[(size_t)i * Tr::NL + s] = lo[s];
+#pragma unroll
+    for (int r = 0; r < Tr::NR; ++r) {
+      const int d = rdst[(size_t)r * N + i];
+      if (d >= 0) {
+        lrf[d] = nan0(lr[r]);
+        lrq[d] = nan0(lc[r]);
+      }
+    }
+  }
   if (jac) {
     const double c0 = sc->c0;
 #pragma unroll
@@ -245,15 +332,32 @@ __global__ void __launch_bounds__(256) k_eval(int N, const int32_t* __restrict__
 // Raw eval (openvaf_jax contract), row-major outputs per instance.
 template <int MID>
 __global__ void k_model_eval(int N, const double* __restrict__ vbr, const double* __restrict__ par,
-                             double* rro, double* rco, double* jro, double* jco) {
+                             const double* __restrict__ lio, double* rro, double* rco, double* jro,
+                             double* jco, double* lro, double* lco, double* loo) {
   using Tr = ModelTraits<MID>;
+  constexpr int NL = Tr::NL > 0 ? Tr::NL : 1;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double vb[Tr::NV];
 #pragma unroll
   for (int k = 0; k < Tr::NV; ++k) vb[k] = vbr[(size_t)k * N + i];
   double rr[Tr::NR], rc[Tr::NR], jr[Tr::NJ], jc[Tr::NJ];
-  eval_model<MID>(vb, par, i, N, rr, rc, jr, jc);
+  double lr[Tr::NR], lc[Tr::NR], li[NL], lo[NL];
+  if constexpr (Tr::NL > 0) {
+#pragma unroll
+    for (int s = 0; s < Tr::NL; ++s) li[s] = lio ? lio[(size_t)i * Tr::NL + s] : 0.0;
+  }
+  eval_model<MID>(vb, par, i, N, li, rr, rc, jr, jc, lr, lc, lo);
+  if constexpr (Tr::NL > 0) {
+#pragma unroll
+    for (int r = 0; r < Tr::NR; ++r) {
+      if (lro) lro[(size_t)i * Tr::NR + r] = lr[r];
+      if (lco) lco[(size_t)i * Tr::NR + r] = lc[r];
+    }
+#pragma unroll
+    for (int s = 0; s < Tr::NL; ++s)
+      if (loo) loo[(size_t)i * Tr::NL + s] = lo[s];
+  }
 #pragma unroll
   for (int r = 0; r < Tr::NR; ++r) {
     rro[(size_t)i * Tr::NR + r] = rr[r];
@@ -391,17 +495,29 @@ __global__ void __launch_bounds__(256) k_chunk_sum(const int32_t* __restrict__ c
 __global__ void __launch_bounds__(256) k_chunk_res(const int32_t* __restrict__ cs,
                                                    const int32_t* __restrict__ ce,
                                                    const double* __restrict__ rf,
-                                                   const double* __restrict__ rq, double* pf,
-                                                   double* pq, double* pm) {
+                                                   const double* __restrict__ rq,
+                                                   const double* __restrict__ lr,
+                                                   const double* __restrict__ lq, double* pf,
+                                                   double* pq, double* pm, double* pl,
+                                                   double* plq) {
   __shared__ double red[RB + 1];
   const int c = blockIdx.x;
   const double f = block_sum<double>(cs[c], ce[c], [&](int u) { return rf[u]; }, red);
   const double q = block_sum<double>(cs[c], ce[c], [&](int u) { return rq[u]; }, red);
   const double m = block_max(cs[c], ce[c], [&](int u) { return fabs(rf[u]); }, red);
+  double l = 0.0, lc = 0.0;
+  if (lr) {  // uniform across the block
+    l = block_sum<double>(cs[c], ce[c], [&](int u) { return lr[u]; }, red);
+    lc = block_sum<double>(cs[c], ce[c], [&](int u) { return lq[u]; }, red);
+  }
   if (threadIdx.x == 0) {
     pf[c] = f;
     pq[c] = q;
     pm[c] = m;
+    if (lr) {
+      pl[c] = l;
+      plq[c] = lc;
+    }
   }
 }
 
@@ -446,10 +562,12 @@ __global__ void __launch_bounds__(256) k_reduce_jac(int nse, const int32_t* __re
 
 // Residual of one node row from its reduced sums (mna_builder.py:1012-1076);
 // returns the value entering the convergence test and whether it fails.
+// f, q: the row's sums of resistive and reactive residual contributions; lf,
+// lq: its lim_rhs sums (zero without limited models).
 __device__ __forceinline__ bool finish_row(const ResidualArgs& a, int i, double f, double q,
-                                           double mx, double& fchk) {
+                                           double mx, double lf, double lq, double& fchk) {
   const DevScalars* sc = a.sc;
-  double fr = f - 0.0;
+  double fr = f - lf;
   if (a.m > 0) {
     double br = 0.0;
     for (int e = a.br_ptr[i]; e < a.br_ptr[i + 1]; ++e) {
@@ -467,7 +585,7 @@ __device__ __forceinline__ bool finish_row(const ResidualArgs& a, int i, double 
   const double dq = a.dQdt_prev ? a.dQdt_prev[i] : 0.0;
   const double q2 = a.Q_prev2 ? a.Q_prev2[i] : 0.0;
   double t = fr - 0.0;
-  t = t + sc->c0 * (q - 0.0);
+  t = t + sc->c0 * (q - lq);
   t = t + sc->c1 * qp;
   t = t + sc->d1 * dq;
   t = t + sc->c2 * q2;
@@ -490,14 +608,18 @@ __global__ void __launch_bounds__(256) k_residual(ResidualArgs a) {
   if ((int)blockIdx.x < hub.n_items) {
     if (threadIdx.x == 0) {
       const int i = hub.item[blockIdx.x];
-      double f = 0.0, q = 0.0, mx = 0.0;
+      double f = 0.0, q = 0.0, mx = 0.0, lf = 0.0, lq = 0.0;
       for (int c = hub.ptr[blockIdx.x]; c < hub.ptr[blockIdx.x + 1]; ++c) {
         f += hub.part[c];
         q += hub.part2[c];
         mx = fmax(mx, hub.part3[c]);
+        if (a.lr) {
+          lf += hub.part4[c];
+          lq += hub.part5[c];
+        }
       }
       double fchk;
-      const bool fail = finish_row(a, i, f, q, mx, fchk);
+      const bool fail = finish_row(a, i, f, q, mx, lf, lq, fchk);
       if (a.st) {
         if (fchk > 0.0) atomicMax(&a.st->maxf_bits, dbits(fchk));
         if (fail) atomicAdd(&a.st->res_fail, 1u);
@@ -511,14 +633,19 @@ __global__ void __launch_bounds__(256) k_residual(ResidualArgs a) {
   if (i < a.n_unk) {
     const int u0 = a.res_seg[i], u1 = a.res_seg[i + 1];
     if (u1 - u0 <= kHeavySegment) {
-      double f = 0.0, q = 0.0, mx = 0.0;
+      double f = 0.0, q = 0.0, mx = 0.0, lf = 0.0, lq = 0.0;
       for (int u = u0; u < u1; ++u) {
         const double v = a.rf[u];
         f += v;
         q += a.rq[u];
         mx = fmax(mx, fabs(v));
       }
-      fail = finish_row(a, i, f, q, mx, fchk);
+      if (a.lr)
+        for (int u = u0; u < u1; ++u) {
+          lf += a.lr[u];
+          lq += a.lq[u];
+        }
+      fail = finish_row(a, i, f, q, mx, lf, lq, fchk);
     }
   } else if (i < a.n_unk + a.m) {
     const int k = i - a.n_unk;
@@ -1288,6 +1415,8 @@ __global__ void k_status_reset(DevStatus* st) {
     st->maxdv_bits = 0;
     st->maxdelta_bits = 0;
     st->maxl_bits = 0;
+    st->limd_bits = 0;
+    st->limr_bits = 0;
     st->res_fail = 0;
     st->delta_fail = 0;
     st->nonfinite = 0;
@@ -1304,49 +1433,100 @@ int model_n_v(int id) {
 }
 int model_n_res(int id) { return id == VJX_MODEL_MOSFET ? 4 : 2; }
 int model_n_jac(int id) { return id == VJX_MODEL_MOSFET ? 16 : 4; }
-int model_n_par(int id) { return id == VJX_MODEL_MOSFET ? 11 : 1; }
+int model_n_par(int id) {
+  return id == VJX_MODEL_MOSFET ? 11 : id == VJX_MODEL_DIODE ? 4 : 1;
+}
+int model_n_lim(int id) { return id == VJX_MODEL_DIODE ? 1 : 0; }
 
 void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalOut out,
                  hipStream_t s) {
   if (m.count == 0) return;
   const int B = 256;
   const int G = nblk(m.count, B);
+  const double* li = out.ls_in ? out.ls_in + m.lim_off : nullptr;
+  double* lo = out.ls_out ? out.ls_out + m.lim_off : nullptr;
   ktimer_begin(s);
   switch (m.model_id) {
     case VJX_MODEL_RESISTOR:
       k_eval<VJX_MODEL_RESISTOR><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
                                                  m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
-                                                 out.rq);
+                                                 out.rq, nullptr, nullptr, nullptr, nullptr);
       break;
     case VJX_MODEL_CAPACITOR:
       k_eval<VJX_MODEL_CAPACITOR><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
                                                   m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
-                                                  out.rq);
+                                                  out.rq, nullptr, nullptr, nullptr, nullptr);
       break;
     case VJX_MODEL_MOSFET:
       k_eval<VJX_MODEL_MOSFET><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
                                                m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
-                                               out.rq);
+                                               out.rq, nullptr, nullptr, nullptr, nullptr);
+      break;
+    case VJX_MODEL_DIODE:
+      k_eval<VJX_MODEL_DIODE><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
+                                              m.res_dst.p, m.par.p, X, sc, out.jac, out.rf, out.rq,
+                                              out.lr, out.lq, li, lo);
       break;
   }
   ktimer_end(KC_EVAL, s);
 }
 
-void launch_model_eval_raw(int id, int count, const double* vb, const double* par, double* rr,
-                           double* rc, double* jr, double* jc, hipStream_t s) {
+void launch_model_eval_raw(int id, int count, const double* vb, const double* par,
+                           const double* li, double* rr, double* rc, double* jr, double* jc,
+                           double* lr, double* lc, double* lo, hipStream_t s) {
   if (count == 0) return;
   const int B = 256, G = nblk(count, B);
   switch (id) {
     case VJX_MODEL_RESISTOR:
-      k_model_eval<VJX_MODEL_RESISTOR><<<G, B, 0, s>>>(count, vb, par, rr, rc, jr, jc);
+      k_model_eval<VJX_MODEL_RESISTOR><<<G, B, 0, s>>>(count, vb, par, li, rr, rc, jr, jc, lr, lc,
+                                                       lo);
       break;
     case VJX_MODEL_CAPACITOR:
-      k_model_eval<VJX_MODEL_CAPACITOR><<<G, B, 0, s>>>(count, vb, par, rr, rc, jr, jc);
+      k_model_eval<VJX_MODEL_CAPACITOR><<<G, B, 0, s>>>(count, vb, par, li, rr, rc, jr, jc, lr,
+                                                        lc, lo);
       break;
     case VJX_MODEL_MOSFET:
-      k_model_eval<VJX_MODEL_MOSFET><<<G, B, 0, s>>>(count, vb, par, rr, rc, jr, jc);
+      k_model_eval<VJX_MODEL_MOSFET><<<G, B, 0, s>>>(count, vb, par, li, rr, rc, jr, jc, lr, lc,
+                                                     lo);
+      break;
+    case VJX_MODEL_DIODE:
+      k_model_eval<VJX_MODEL_DIODE><<<G, B, 0, s>>>(count, vb, par, li, rr, rc, jr, jc, lr, lc,
+                                                    lo);
       break;
   }
+}
+
+// Elementwise pnjlim (kind 0: p1 = vt, p2 = vcrit) or fetlim (kind 1: p1 = vto).
+__global__ void k_limit(int kind, int n, const double* __restrict__ vnew,
+                        const double* __restrict__ vold, double p1, double p2, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = kind == 0 ? pnjlim(vnew[i], vold[i], p1, p2) : fetlim(vnew[i], vold[i], p1);
+}
+
+void launch_limit(int kind, int n, const double* vnew, const double* vold, double p1, double p2,
+                  double* out, hipStream_t s) {
+  if (n == 0) return;
+  k_limit<<<nblk(n, 256), 256, 0, s>>>(kind, n, vnew, vold, p1, p2, out);
+}
+
+// Limit-state convergence terms (solver_factories.py:319-321): max
+// |ls_out - ls_in| and max |ls_in| into the status (order-independent maxima).
+__global__ void __launch_bounds__(256) k_lim_stats(int n, const double* __restrict__ ls_in,
+                                                   const double* __restrict__ ls_out,
+                                                   DevStatus* st) {
+  double dmax = 0.0, rmax = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    dmax = fmax(dmax, fabs(ls_out[i] - ls_in[i]));
+    rmax = fmax(rmax, fabs(ls_in[i]));
+  }
+  block_flags(dbits(dmax), dbits(rmax), 0u, &st->limd_bits, &st->limr_bits, nullptr);
+}
+
+void launch_lim_stats(int n, const double* ls_in, const double* ls_out, DevStatus* st,
+                      hipStream_t s) {
+  if (n == 0) return;
+  k_lim_stats<<<std::min(nblk(n, 256), 64), 256, 0, s>>>(n, ls_in, ls_out, st);
 }
 
 void launch_isrc_contrib(int k, const double* isrc, const int32_t* is_dst, double* rf, double* rq,
@@ -1372,7 +1552,8 @@ void launch_residual(const ResidualArgs& a, hipStream_t s) {
   const HubChunks& h = a.hub;
   ktimer_begin(s);
   if (h.n_chunks > 0)
-    k_chunk_res<<<h.n_chunks, 256, 0, s>>>(h.cs, h.ce, a.rf, a.rq, h.part, h.part2, h.part3);
+    k_chunk_res<<<h.n_chunks, 256, 0, s>>>(h.cs, h.ce, a.rf, a.rq, a.lr, a.lq, h.part, h.part2,
+                                           h.part3, h.part4, h.part5);
   k_residual<<<h.n_items + nblk(n, 256), 256, 0, s>>>(a);
   ktimer_end(KC_STAMP, s);
 }

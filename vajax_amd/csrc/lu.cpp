@@ -112,6 +112,8 @@ hipError_t HubPlan::build(const int32_t* seg, int n, int nparts) {
   if (e == hipSuccess) e = part.alloc(n_chunks);
   if (e == hipSuccess && nparts > 1) e = part2.alloc(n_chunks);
   if (e == hipSuccess && nparts > 2) e = part3.alloc(n_chunks);
+  if (e == hipSuccess && nparts > 4) e = part4.alloc(n_chunks);
+  if (e == hipSuccess && nparts > 4) e = part5.alloc(n_chunks);
   return e;
 }
 

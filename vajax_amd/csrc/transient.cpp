@@ -49,6 +49,9 @@ double wave_value(int w, const std::vector<double>& p, int i, int m, double t) {
 struct Work {
   DBuf<double> Xa, Xw, Xsave, Vp, Qp, dQ, Q2, Qn, dQn, mrc, hmrc, floor, Vmax, f, csr, delta, vsrc, isrc,
       Ivs, hist, vout, iout, zeros;
+  // limit states (limited devices): at the step's start, of the current
+  // Newton iterate, written by the last build, saved for a re-pivot redo
+  DBuf<double> ls_step, ls_iter, ls_out, ls_save;
   DBuf<DevStatus> st;
   DevStatus* h_st = nullptr;
   ~Work() {
@@ -62,6 +65,7 @@ struct Driver {
   hipStream_t s;
   Work w;
   int n_total, n_unk, m, k, n_aug;
+  int n_lim = 0;
   int64_t total_evals = 0;
   int repivots = 0;
 
@@ -139,7 +143,7 @@ struct Driver {
     vjx_pattern* pat = c->pat;
     if (pat->lu.analyzed && pat->uploaded) return VJX_OK;
     circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, nullptr, w.csr.p, w.f.p,
-                        w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, s);
+                        w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, nullptr, nullptr, s);
     VJX_TRY_HIP(hipGetLastError());
     VJX_TRY_HIP(hipStreamSynchronize(s));
     int rc = pattern_ensure_analyzed(pat, w.csr.p);
@@ -162,11 +166,18 @@ struct Driver {
     // per-launch events only in eager profiling passes (never under capture)
     ktimer_enable(prof);
     launch_copy(n_total + m, w.Xw.p, w.Xsave.p, s);
+    if (n_lim) launch_copy(n_lim, w.ls_iter.p, w.ls_save.p, s);
     launch_status_reset(pat->d_status.p, s);
     launch_status_reset(w.st.p, s);
     if (prof) VJX_TRY_HIP(hipEventRecord(ev[0], s));
     circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, floor, w.csr.p, w.f.p, w.Qn.p,
-                        w.mrc.p, w.Ivs.p, w.st.p, s);
+                        w.mrc.p, w.Ivs.p, w.st.p, n_lim ? w.ls_iter.p : nullptr,
+                        n_lim ? w.ls_out.p : nullptr, s);
+    if (n_lim) {
+      // limit-state convergence terms, then the state feeds the next iteration
+      launch_lim_stats(n_lim, w.ls_iter.p, w.ls_out.p, w.st.p, s);
+      launch_copy(n_lim, w.ls_out.p, w.ls_iter.p, s);
+    }
     if (prof) VJX_TRY_HIP(hipEventRecord(ev[1], s));
     int rc = pattern_refactor_async(pat, w.csr.p, cfg->precision, false);
     if (rc < 0) return rc;
@@ -247,6 +258,7 @@ struct Driver {
         drop_graph();  // factor buffers were reallocated
         ++repivots;
         launch_copy(n_total + m, w.Xsave.p, w.Xw.p, s);
+        if (n_lim) launch_copy(n_lim, w.ls_save.p, w.ls_iter.p, s);
         --total_evals;
         continue;
       }
@@ -254,13 +266,20 @@ struct Driver {
       const bool delta_ok = (iters == 0) || w.h_st->delta_fail == 0;
       max_f = bits_to_double(w.h_st->maxf_bits);
       conv = res_ok && delta_ok;
+      if (n_lim) {
+        // VACASK preventedConvergence (solver_factories.py:317-323)
+        const double lim_delta = bits_to_double(w.h_st->limd_bits);
+        const double lim_ref =
+            std::max(bits_to_double(w.h_st->limr_bits) * cfg->reltol, cfg->vntol);
+        conv = conv && lim_delta < lim_ref && iters >= 1;
+      }
       ++iters;
     }
     // final build at X_final for Q, I_vsource, max_res_contrib (:689-713)
     int rc = upload_scalars(c0, c1, d1, c2, gshunt, iters);
     if (rc < 0) return rc;
     circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, floor, nullptr, w.f.p, w.Qn.p,
-                        w.mrc.p, w.Ivs.p, nullptr, s);
+                        w.mrc.p, w.Ivs.p, nullptr, n_lim ? w.ls_iter.p : nullptr, nullptr, s);
     ++total_evals;
     launch_dqdt(n_unk, c0, c1, d1, c2, w.Qn.p, Qp, dQ, Q2, w.dQn.p, s);
     VJX_TRY_HIP(hipGetLastError());
@@ -320,6 +339,15 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   VJX_TRY_HIP(w.iout.alloc((size_t)cfg->max_steps * mo));
   VJX_TRY_HIP(w.st.alloc(1));
   VJX_TRY_HIP(hipHostMalloc((void**)&w.h_st, sizeof(DevStatus), hipHostMallocDefault));
+  const int nl = c->n_lim;
+  D.n_lim = nl;
+  if (nl) {
+    for (auto* b : {&w.ls_step, &w.ls_iter, &w.ls_out, &w.ls_save}) VJX_TRY_HIP(b->alloc(nl));
+    // the DC solve and the time loop both start from zero limit states
+    // (full_mna.py:913, 1028-1033)
+    launch_fill(nl, 0.0, w.ls_iter.p, s);
+    launch_fill(nl, 0.0, w.ls_step.p, s);
+  }
   launch_fill(n_unk, 0.0, w.zeros.p, s);
   launch_fill((int)((size_t)cfg->max_steps * mo), 0.0, w.iout.p, s);
   launch_fill(m, 0.0, w.Ivs.p, s);
@@ -341,7 +369,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     rc = D.upload_scalars(0.0, 0.0, 0.0, 0.0, 0.0, 1);
     if (rc < 0) return rc;
     circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, w.zeros.p, w.zeros.p, w.zeros.p, nullptr,
-                        nullptr, w.f.p, w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, s);
+                        nullptr, w.f.p, w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, nullptr, nullptr, s);
     ++D.total_evals;
     launch_fill(n_unk, 0.0, w.mrc.p, s);
   } else {
@@ -416,9 +444,12 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       int iters;
       bool conv;
       double maxf;
+      if (nl) launch_copy(nl, w.ls_step.p, w.ls_iter.p, s);
       rc = D.nr_solve(c0, c1, d1, c2, 0.0, w.Qp.p, w.dQ.p, w.Q2.p, w.floor.p, iters, conv, maxf);
       if (rc < 0) return rc;
       total_nr += iters;
+      // the limit state advances with a converged step (full_mna.py:1349-1353)
+      if (nl && conv) launch_copy(nl, w.ls_iter.p, w.ls_step.p, s);
       times[i + 1] = t_next;
       launch_copy(nx, w.Xw.p, w.Xa.p, s);
       launch_output(n_total, m, w.Xa.p, w.Ivs.p, 0, w.vout.p + (size_t)(i + 1) * n_total,
@@ -527,6 +558,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       int iters;
       bool conv;
       double maxf;
+      if (nl) launch_copy(nl, w.ls_step.p, w.ls_iter.p, s);
       rc = D.nr_solve(c0, c1, d1, c2, gshunt, w.Qp.p, w.dQ.p, w.Q2.p, w.floor.p, iters, conv, maxf);
       if (rc < 0) return rc;
       total_nr += iters;
@@ -563,6 +595,8 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       const bool use_new = accept && !fail_min;
       if (accept) t = t_next;
       if (use_new) {
+        // the limit state advances with the new solution (full_mna.py:2162-2167)
+        if (nl) launch_copy(nl, w.ls_iter.p, w.ls_step.p, s);
         // roll history: the oldest slot receives V_new and becomes slot 0
         int last = slot[max_hist - 1];
         for (int j = max_hist - 1; j > 0; --j) slot[j] = slot[j - 1];

@@ -222,19 +222,28 @@ class DeviceCircuit:
         ph = L.lib.vjx_circuit_pattern(self.h)
         return SparseLU(self.topo.indptr, self.topo.indices, _handle=ph, _owner=self)
 
+    @property
+    def n_limit_states(self) -> int:
+        return int(L.lib.vjx_circuit_limit_states(self.h))
+
     def build_system(self, X, vsrc, isrc, Q_prev=None, dQdt_prev=None, Q_prev2=None, c0=0.0,
-                     c1=0.0, d1=0.0, c2=0.0, gmin=1e-12, gshunt=0.0, want_jacobian=True):
-        """Device build_system on torch CUDA tensors; returns a dict of tensors."""
+                     c1=0.0, d1=0.0, c2=0.0, gmin=1e-12, gshunt=0.0, want_jacobian=True,
+                     limit_state=None):
+        """Device build_system on torch CUDA tensors; returns a dict of tensors
+        (with ``limit_state_out`` when the circuit has limited devices;
+        ``limit_state`` None means zeros)."""
         import torch
 
         t = self.topo
         dev = X.device
+        nl = self.n_limit_states
         out = {
             "csr_data": torch.empty(t.nse, dtype=torch.float64, device=dev) if want_jacobian else None,
             "f": torch.empty(t.n_augmented, dtype=torch.float64, device=dev),
             "Q": torch.empty(t.n_unknowns, dtype=torch.float64, device=dev),
             "I_vsource": torch.empty(max(t.n_vsources, 1), dtype=torch.float64, device=dev),
             "max_res_contrib": torch.empty(t.n_unknowns, dtype=torch.float64, device=dev),
+            "limit_state_out": torch.empty(nl, dtype=torch.float64, device=dev) if nl else None,
         }
 
         def p(x):
@@ -244,7 +253,8 @@ class DeviceCircuit:
         L.check(L.lib.vjx_build_system(self.h, p(X), p(vsrc), p(isrc), p(Q_prev), p(dQdt_prev),
                                        p(Q_prev2), c0, c1, d1, c2, gmin, gshunt,
                                        p(out["csr_data"]), p(out["f"]), p(out["Q"]),
-                                       p(out["I_vsource"]), p(out["max_res_contrib"])),
+                                       p(out["I_vsource"]), p(out["max_res_contrib"]),
+                                       p(limit_state), p(out["limit_state_out"])),
                 "vjx_build_system")
         return out
 

@@ -86,6 +86,41 @@ def rc_ladder(sections: int = 200, seed: int = 0, t_stop: float = 10e-9, dt: flo
                    options={"tran_method": "gear2", "icmode": "op"})
 
 
+def diode_ladder(sections: int = 100, seed: int = 0, t_stop: float = 5e-9,
+                 dt: float = 1e-12) -> Netlist:
+    """RC ladder with clamp diodes, the carrier of device limiting (pnjlim).
+
+    in -> R_i -> n_i, C_i from n_i to ground, and at every node a diode to
+    ground (anode n_i) plus, at every other node, one from ground (anode 0):
+    R ~ U[100, 1k] ohm, C ~ U[10, 100] fF, is ~ U[1e-15, 1e-13] A,
+    n ~ U[1, 1.5], cj ~ U[1, 10] fF.  Driven by a -1.5 -> 2 V pulse, so both
+    diode polarities are driven hard into forward bias (already at the DC
+    point) and the junction voltages go through pnjlim.
+    """
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(100.0, 1e3, sections)
+    c = rng.uniform(10e-15, 100e-15, sections)
+    width = max(3, len(str(sections)))
+    b = NetlistBuilder(f"diode_ladder_{sections}_s{seed}")
+    b.source("vsource", "v1", "in", "0", type="pulse", val0=-1.5, val1=2.0, rise=20e-12,
+             fall=20e-12, width=1e-9, delay=0.2e-9, period=1.0)
+    prev = "in"
+    for i in range(sections):
+        node = f"n{i + 1:0{width}d}"
+        b.device("resistor", f"r{i + 1}", [prev, node], r=r[i])
+        b.device("capacitor", f"c{i + 1}", [node, "0"], c=c[i])
+        b.device("diode", f"d{i + 1}", [node, "0"], **{"is": rng.uniform(1e-15, 1e-13),
+                                                     "n": rng.uniform(1.0, 1.5),
+                                                     "cj": rng.uniform(1e-15, 10e-15)})
+        if i % 2 == 0:
+            b.device("diode", f"dr{i + 1}", ["0", node], **{"is": rng.uniform(1e-15, 1e-13),
+                                                           "n": rng.uniform(1.0, 1.5),
+                                                           "cj": rng.uniform(1e-15, 10e-15)})
+        prev = node
+    return b.build(analysis={"stop": t_stop, "step": dt},
+                   options={"tran_method": "gear2", "icmode": "op"})
+
+
 def ring_oscillator(stages: int = 101, seed: int = 0, t_stop: float = 50e-9, dt: float = 1e-12) -> Netlist:
     """CMOS inverter ring with per-stage random W/L and load caps.
 

@@ -12,6 +12,9 @@ once at setup, the analogue of the cache that the reference's
     vgoff = (phi - vto) + gamma*sqrt(phi)   so that VG' = VG + vgoff
     phi, gamma, beta0 = kp*(w/l), theta
     cgs = cgd = cox*w*l/2 + cov*w,  cgb = 0,  cdb = csb = cj*w
+
+and for the diode its init constants is, nvt = n*vt, the pnjlim critical
+voltage vcrit = nvt*ln(nvt/(sqrt(2)*is)), and cj.
 """
 
 from __future__ import annotations
@@ -44,6 +47,19 @@ def mosfet_cache(p: Dict[str, np.ndarray]) -> np.ndarray:
                                           for c in MOSFET_CACHE_COLUMNS]))
 
 
+# thermal voltage at the reference's default 27 C (k T / q, exact SI constants)
+VT = 1.380649e-23 * 300.15 / 1.602176634e-19
+DIODE_COLUMNS = ("is", "nvt", "vcrit", "cj")
+
+
+def diode_cache(p: Dict[str, np.ndarray]) -> np.ndarray:
+    nvt = p["n"] * VT
+    cols = {"is": p["is"], "nvt": nvt, "vcrit": nvt * np.log(nvt / (np.sqrt(2.0) * p["is"])),
+            "cj": p["cj"]}
+    return np.ascontiguousarray(np.stack([np.asarray(cols[c], dtype=np.float64)
+                                          for c in DIODE_COLUMNS]))
+
+
 def param_block(model: str, params: Dict[str, np.ndarray]) -> np.ndarray:
     if model == "resistor":
         return np.ascontiguousarray(params["r"][None, :], dtype=np.float64)
@@ -51,4 +67,6 @@ def param_block(model: str, params: Dict[str, np.ndarray]) -> np.ndarray:
         return np.ascontiguousarray(params["c"][None, :], dtype=np.float64)
     if model == "mosfet":
         return mosfet_cache(params)
+    if model == "diode":
+        return diode_cache(params)
     raise ValueError(f"no eval kernel for model {model!r}")

@@ -31,6 +31,9 @@ MODEL_TERMINALS: Dict[str, List[str]] = {
     "resistor": ["A", "B"],
     "capacitor": ["A", "B"],
     "mosfet": ["d", "g", "s", "b"],
+    # junction diode, anode A, cathode K; its junction voltage goes through
+    # $limit(..., "pnjlim", ...) (vajax/analysis/limiting.py:22-82)
+    "diode": ["A", "K"],
 }
 
 # Per-model instance parameters and defaults.
@@ -54,6 +57,9 @@ MODEL_PARAMS: Dict[str, Dict[str, float]] = {
         "cov": 0.0,  # F/m, gate overlap capacitance per width
         "cj": 0.0,  # F/m, drain/source junction capacitance per width
     },
+    # SPICE diode defaults: saturation current, emission coefficient, a
+    # linear junction capacitance
+    "diode": {"is": 1e-14, "n": 1.0, "cj": 0.0},
 }
 
 SOURCE_DEFAULTS: Dict[str, float] = {
