@@ -103,15 +103,17 @@ def kernel_bytes(topo, info, precision: str):
 
 
 KERNEL_NAMES = {"eval": "k_eval<MOSFET> (+R/C)", "stamp": "k_chunk_sum/k_reduce_jac/k_chunk_res/k_residual",
-                "lu_gather": "k_lu_gather", "lu_tasks": "k_lu_band", "lu_schur": "k_lu_wide (Schur step)",
-                "lu_scale": "k_lu_scale", "solve_tasks": "k_tri_band (forward + backward)",
+                "lu_gather": "k_lu_gather", "lu_tasks": "k_lu_band_lds (band tasks)",
+                "lu_schur": "k_lu_wide (Schur step)",
+                "lu_scale": "k_lu_scale", "solve_tasks": "k_tri_band_lde (forward + backward)",
                 "solve_hub": "k_tri_hub_part/k_tri_hub_finish"}
 
 
 # kernel class -> kernel names in the rocprofv3 traffic summaries
 TRAFFIC_KERNELS = {
     "eval": ["k_eval<2>"], "stamp": ["k_reduce_jac", "k_residual", "k_chunk_sum", "k_chunk_res"],
-    "lu_gather": ["k_lu_gather<double>"], "lu_tasks": ["k_lu_band_lds<double>", "k_lu_band<double>"],
+    "lu_gather": ["k_lu_gather<double>"], "lu_tasks": ["k_lu_band_lds<double, 4, 3, 8>", "k_lu_band_lds<double, 1, 2, 2>",
+                 "k_lu_band<double>"],
     "lu_schur": ["k_lu_wide<double>"], "lu_scale": ["k_lu_scale<double>"],
     "solve_tasks": ["k_tri_band_lde<double, true>", "k_tri_band_lde<double, false>",
                     "k_tri_band<double, true>", "k_tri_band<double, false>"]}
