@@ -199,6 +199,12 @@ typedef struct {
   const int32_t* conv_mask; /* [n_unknowns] 1 = checked in residual test (may be NULL) */
   int32_t n_models;
   vjx_model_desc models[VJX_MAX_MODELS];
+  /* noise-only internal nodes (solver_factories.py:37-137, 1268-1276): their
+   * residual rows; in the Newton solve their rows and their columns in node
+   * rows become the identity, their right-hand side 0, and they are held at
+   * 0 V after each update.  n_noi may be 0 (noi_rows NULL). */
+  int32_t n_noi;
+  const int32_t* noi_rows;
 } vjx_circuit_desc;
 
 int vjx_circuit_create(const vjx_circuit_desc* desc, vjx_circuit** out);

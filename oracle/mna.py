@@ -78,6 +78,11 @@ class OracleCircuit:
     net: Netlist
     gmin: float = 1e-12
     n_lim: int = 0  # limit states: model order, instance, state (mna_builder.py:387-401)
+    # residual rows of noise-only internal nodes, sorted, and the residual
+    # convergence mask (False for internal device nodes and NOI nodes),
+    # solver_factories.py:37-137
+    noi_res: np.ndarray = None
+    conv_mask: np.ndarray = None
 
     @property
     def nse(self):
@@ -166,8 +171,12 @@ def setup(net: Netlist, gmin: float = 1e-12) -> OracleCircuit:
     k = net.isources.count
     is_p = gidx[net.isources.terminals[:, 0]] if k else np.zeros(0, np.int64)
     is_n = gidx[net.isources.terminals[:, 1]] if k else np.zeros(0, np.int64)
+    noi_res = np.array(sorted(names[nm] - 1 for nm in set(net.noi_nodes)), dtype=np.int64)
+    conv_mask = np.ones(n_unk, dtype=bool)
+    for nm in list(net.internal_nodes) + list(net.noi_nodes):
+        conv_mask[names[nm] - 1] = False
     return OracleCircuit(names, n_total, n_unk, m, n_aug, models, vs_p, vs_n, is_p, is_n,
-                         indptr, indices, diag_pos, b_pos, net, gmin, n_lim)
+                         indptr, indices, diag_pos, b_pos, net, gmin, n_lim, noi_res, conv_mask)
 
 
 def eval_model(om: OracleModel, V: np.ndarray, lim_in=None):

@@ -6,7 +6,11 @@
   (|d| < max(|X|*reltol, vntol | abstol), skipped at iteration 0) -> X += d;
   stop when both tests pass or after ``max_iterations``; then one more build
   at X_final for Q, I_vsource and max_res_contrib, and
-  dQdt = c0*Q + c1*Q_prev + d1*dQdt_prev + c2*Q_prev2.  With limited devices
+  dQdt = c0*Q + c1*Q_prev + d1*dQdt_prev + c2*Q_prev2.  Internal device nodes
+  are left out of the residual test and of max_f; noise-only internal (NOI)
+  nodes get identity rows and columns and a zero right-hand side in the
+  solve, no delta test, and are clamped to 0 V after the update
+  (solver_factories.py:37-137, 271-313, 1268-1276).  With limited devices
   the limit state of each iteration's build feeds the next, and convergence
   also needs max|ls_out - ls_in| < max(max|ls_in|*reltol, vntol) and an
   iteration index >= 1 (solver_factories.py:317-323); the final build reads
@@ -157,10 +161,18 @@ def nr_solve(cir, solver: LinearSolver, opts: Options, X_init, vsrc, isrc, Q_pre
         csr, f, Q, _, mrc, lim_out = mna.build_system_lim(cir, X, vsrc, isrc, Q_prev, c0, gshunt,
                                                           c1, d1, dQ, c2, Q2, lim)
         res_tol = np.concatenate([np.maximum(mrc * opts.reltol, floor), np.full(m, opts.vntol)])
-        max_f = float(np.max(np.abs(f))) if f.size else 0.0
-        res_ok = bool(np.all(np.abs(f) < res_tol))
+        f_check = f
+        if cir.conv_mask is not None and not cir.conv_mask.all():
+            f_check = np.where(np.concatenate([cir.conv_mask, np.ones(m, bool)]), f, 0.0)
+        max_f = float(np.max(np.abs(f_check))) if f.size else 0.0
+        res_ok = bool(np.all(np.abs(f_check) < res_tol))
+        noi = cir.noi_res if cir.noi_res is not None and len(cir.noi_res) else None
+        if noi is not None:
+            csr, f = _apply_noi(cir, csr, f, noi)
         delta = solver.solve(csr, f)
         conv_delta = np.concatenate([delta[:n_unk] * opts.nr_damping, delta[n_unk:]])
+        if noi is not None:
+            conv_delta[noi] = 0.0
         X_ref = np.concatenate([X[1:n_total], X[n_total:]])
         tol = np.maximum(np.abs(X_ref) * opts.reltol, delta_abs)
         delta_ok = (it == 0) or bool(np.all(np.abs(conv_delta) < tol))
@@ -170,6 +182,8 @@ def nr_solve(cir, solver: LinearSolver, opts: Options, X_init, vsrc, isrc, Q_pre
         X = X.copy()
         X[1:n_total] += Vd * v_scale * opts.nr_damping
         X[n_total:] += delta[n_unk:]
+        if noi is not None:
+            X[noi + 1] = 0.0
         converged = res_ok and delta_ok
         if cir.n_lim:
             lim_delta = float(np.max(np.abs(lim_out - lim)))
@@ -181,6 +195,26 @@ def nr_solve(cir, solver: LinearSolver, opts: Options, X_init, vsrc, isrc, Q_pre
                                                   dQ, c2, Q2, lim)
     dQdt = c0 * Qf + c1 * Q_prev + d1 * dQ + c2 * Q2
     return NRResult(X, it, converged, max_f, Qf, dQdt, I_vs, mrc, lim)
+
+
+def _apply_noi(cir, csr, f, noi):
+    """NOI rows and columns zeroed, their diagonals 1, their right-hand side
+    0 (solver_factories.py:1268-1276, masks of :105-135: columns are masked
+    in node rows only, branch rows keep theirs)."""
+    csr = csr.copy()
+    f = f.copy()
+    noi_set = set(int(r) for r in noi)
+    cols = np.asarray(cir.indices)
+    for r in range(cir.n_unk):
+        for e in range(int(cir.indptr[r]), int(cir.indptr[r + 1])):
+            if r in noi_set or int(cols[e]) in noi_set:
+                csr[e] = 0.0
+    for r in noi:
+        for e in range(int(cir.indptr[r]), int(cir.indptr[r + 1])):
+            if int(cols[e]) == r:
+                csr[e] = 1.0
+    f[noi] = 0.0
+    return csr, f
 
 
 def predict(V_hist, dt_hist, hist_count, new_dt, max_order):

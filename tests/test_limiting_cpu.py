@@ -146,3 +146,64 @@ def test_limited_transient_with_rejections_stays_clamped():
     assert out.V[:, idx].max() < 1.2 and out.V[:, idx].min() > -1.2
     # every node voltage stays finite and the run reaches t_stop
     assert np.isfinite(out.V).all() and out.times[-1] == pytest.approx(2e-9)
+
+
+# ---- internal device nodes and noise-only internal (NOI) nodes ----------
+
+def _noi_chain(noi=("a",), internal=()):
+    b = NetlistBuilder("noi_chain")
+    b.source("vsource", "v1", "in", "0", type="dc", dc=1.0)
+    b.device("resistor", "r1", ["in", "a"], r=1e3)
+    b.device("resistor", "r2", ["a", "b"], r=2e3)
+    b.device("resistor", "r3", ["b", "0"], r=3e3)
+    b.device("capacitor", "c1", ["b", "0"], c=1e-13)
+    return b.build(analysis={"stop": 1e-9, "step": 1e-12}, options={"icmode": "op"},
+                   internal_nodes=internal, noi_nodes=noi)
+
+
+def test_noi_node_is_held_at_zero_and_decoupled():
+    net = _noi_chain()
+    cir = mna.setup(net)
+    assert list(cir.noi_res) == [cir.node_names["a"] - 1]
+    assert not cir.conv_mask[cir.node_names["a"] - 1]
+    X0, _, I_dc, _, info = T.initial_state(cir, T.LinearSolver(cir), T.Options())
+    assert info.converged
+    # a is clamped to 0 V; b then sits between two grounded resistors; the
+    # source drives r1 into the clamped node
+    assert X0[cir.node_names["a"]] == 0.0
+    assert abs(X0[cir.node_names["b"]]) < 1e-12
+    assert I_dc[0] == pytest.approx(-1.0 / 1e3, rel=1e-9)
+
+
+def test_noi_solve_is_identity_on_noi_rows():
+    net = _noi_chain()
+    cir = mna.setup(net)
+    rng = np.random.default_rng(0)
+    csr = rng.uniform(-1, 1, cir.nse)
+    f = rng.uniform(-1, 1, cir.n_aug)
+    A0 = mna.csr_matrix(cir, csr).toarray()
+    csr2, f2 = T._apply_noi(cir, csr, f, cir.noi_res)
+    A = mna.csr_matrix(cir, csr2).toarray()
+    r = cir.noi_res[0]
+    want_row = np.zeros(cir.n_aug)
+    want_row[r] = 1.0
+    assert np.array_equal(A[r], want_row)
+    # the column is zeroed in node rows, kept in branch rows
+    assert np.all(A[:cir.n_unk, r][np.arange(cir.n_unk) != r] == 0.0)
+    assert np.array_equal(A[cir.n_unk:, r], A0[cir.n_unk:, r])
+    assert f2[r] == 0.0 and np.array_equal(np.delete(f2, r), np.delete(f, r))
+
+
+def test_internal_nodes_are_left_out_of_the_residual_test():
+    net = _noi_chain(noi=(), internal=("b",))
+    cir = mna.setup(net)
+    assert not cir.conv_mask[cir.node_names["b"] - 1] and len(cir.noi_res) == 0
+    X0, _, _, _, info = T.initial_state(cir, T.LinearSolver(cir), T.Options())
+    assert info.converged
+    # the linear divider's solution is unchanged by the mask (gmin = 1e-12 S
+    # on each node moves it by ~1e-9 relative)
+    want_b = 1.0 * 3e3 / (1e3 + 2e3 + 3e3)
+    assert X0[cir.node_names["b"]] == pytest.approx(want_b, rel=1e-8)
+    plain = mna.setup(_noi_chain(noi=(), internal=()))
+    ref = T.initial_state(plain, T.LinearSolver(plain), T.Options())
+    assert X0[cir.node_names["b"]] == ref[0][cir.node_names["b"]]

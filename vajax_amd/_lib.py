@@ -72,6 +72,8 @@ class CircuitDesc(C.Structure):
         ("conv_mask", i32p),
         ("n_models", C.c_int32),
         ("models", ModelDesc * MAX_MODELS),
+        ("n_noi", C.c_int32),
+        ("noi_rows", i32p),
     ]
 
 

@@ -192,6 +192,41 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
       return VJX_ERR_ARG;
     }
   }
+  if (d->n_noi < 0 || (d->n_noi > 0 && !d->noi_rows)) {
+    set_error("vjx_circuit_create: bad NOI list");
+    return VJX_ERR_ARG;
+  }
+  if (d->n_noi > 0) {
+    std::vector<char> is_noi(nu, 0);
+    for (int i = 0; i < d->n_noi; ++i) {
+      const int r = d->noi_rows[i];
+      if (r < 0 || r >= nu) {
+        set_error("vjx_circuit_create: NOI row out of range");
+        return VJX_ERR_ARG;
+      }
+      is_noi[r] = 1;
+    }
+    std::vector<int32_t> rows, zero, diag;
+    for (int r = 0; r < nu; ++r)
+      if (is_noi[r]) rows.push_back(r);
+    for (int r = 0; r < nu; ++r)
+      for (int e = d->indptr[r]; e < d->indptr[r + 1]; ++e) {
+        const int col = d->indices[e];
+        if (is_noi[r] && col == r)
+          diag.push_back(e);
+        else if (is_noi[r] || (col < nu && is_noi[col]))
+          zero.push_back(e);
+      }
+    if (diag.size() != rows.size()) {
+      set_error("vjx_circuit_create: a NOI row has no diagonal entry");
+      return VJX_ERR_ARG;
+    }
+    c->n_noi = (int)rows.size();
+    c->n_noi_zero = (int)zero.size();
+    VJX_TRY_HIP(c->noi_rows.upload(rows));
+    VJX_TRY_HIP(c->noi_zero.upload(zero));
+    VJX_TRY_HIP(c->noi_diag.upload(diag));
+  }
   VJX_TRY_HIP(c->d_sc.alloc(1));
   VJX_TRY_HIP(hipHostMalloc((void**)&c->h_sc, sizeof(DevScalars), hipHostMallocDefault));
   std::memset(c->h_sc, 0, sizeof(DevScalars));

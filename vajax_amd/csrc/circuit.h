@@ -43,6 +43,10 @@ struct vjx_circuit {
   // limit-state output for builds whose caller discards it
   int n_lim = 0;
   vjx::DBuf<double> lr, lq, ls_scratch;
+  // noise-only internal nodes: their rows, the CSR slots zeroed (their rows,
+  // their columns in node rows, diagonals excluded) and their diagonals
+  int n_noi = 0, n_noi_zero = 0;
+  vjx::DBuf<int32_t> noi_rows, noi_zero, noi_diag;
   vjx_pattern* pat = nullptr;
   vjx::DBuf<vjx::DevScalars> d_sc;
   vjx::DevScalars* h_sc = nullptr;  // pinned

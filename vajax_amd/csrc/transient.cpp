@@ -144,6 +144,9 @@ struct Driver {
     if (pat->lu.analyzed && pat->uploaded) return VJX_OK;
     circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, nullptr, w.csr.p, w.f.p,
                         w.Qn.p, w.mrc.p, w.Ivs.p, nullptr, nullptr, nullptr, s);
+    // pivots are chosen on the matrix the solves will see
+    launch_noi_apply(c->n_noi_zero, c->noi_zero.p, c->n_noi, c->noi_diag.p, c->noi_rows.p,
+                     w.csr.p, w.f.p, s);
     VJX_TRY_HIP(hipGetLastError());
     VJX_TRY_HIP(hipStreamSynchronize(s));
     int rc = pattern_ensure_analyzed(pat, w.csr.p);
@@ -178,6 +181,11 @@ struct Driver {
       launch_lim_stats(n_lim, w.ls_iter.p, w.ls_out.p, w.st.p, s);
       launch_copy(n_lim, w.ls_out.p, w.ls_iter.p, s);
     }
+    // NOI rows / columns of the system the solve sees (the residual test above
+    // used the unmodified f); their delta is then exactly 0, so the delta test
+    // needs no mask
+    launch_noi_apply(c->n_noi_zero, c->noi_zero.p, c->n_noi, c->noi_diag.p, c->noi_rows.p,
+                     w.csr.p, w.f.p, s);
     if (prof) VJX_TRY_HIP(hipEventRecord(ev[1], s));
     int rc = pattern_refactor_async(pat, w.csr.p, cfg->precision, false);
     if (rc < 0) return rc;
@@ -190,6 +198,7 @@ struct Driver {
     }
     launch_delta(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
     launch_update(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
+    launch_noi_clamp(c->n_noi, c->noi_rows.p, w.Xw.p, s);
     return VJX_OK;
   }
 

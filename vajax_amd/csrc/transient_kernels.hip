@@ -175,6 +175,38 @@ void launch_floor(int n, const double* hmrc, double reltol, double abstol, doubl
                   hipStream_t s) {
   if (n) k_floor<<<nb(n), 256, 0, s>>>(n, hmrc, reltol, abstol, floor);
 }
+// Noise-only internal nodes in the Newton solve (solver_factories.py:1268-1276):
+// zero their rows and their columns in node rows, put 1 on their diagonals,
+// 0 in their right-hand side (three disjoint index sets).
+__global__ void k_noi_apply(int nz, const int32_t* __restrict__ zero, int nd,
+                            const int32_t* __restrict__ diag, const int32_t* __restrict__ rows,
+                            double* csr, double* f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nz) {
+    csr[zero[i]] = 0.0;
+  } else if (i < nz + nd) {
+    csr[diag[i - nz]] = 1.0;
+  } else if (i < nz + 2 * nd) {
+    f[rows[i - nz - nd]] = 0.0;
+  }
+}
+
+// NOI nodes held at 0 V after the update (solver_factories.py:311-313).
+__global__ void k_noi_clamp(int nd, const int32_t* __restrict__ rows, double* X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nd) X[1 + rows[i]] = 0.0;
+}
+
+void launch_noi_apply(int nz, const int32_t* zero, int nd, const int32_t* diag,
+                      const int32_t* rows, double* csr, double* f, hipStream_t s) {
+  const int n = nz + 2 * nd;
+  if (n) k_noi_apply<<<nb(n), 256, 0, s>>>(nz, zero, nd, diag, rows, csr, f);
+}
+
+void launch_noi_clamp(int nd, const int32_t* rows, double* X, hipStream_t s) {
+  if (nd) k_noi_clamp<<<nb(nd), 256, 0, s>>>(nd, rows, X);
+}
+
 void launch_fill(int n, double v, double* x, hipStream_t s) {
   if (n) k_fill<<<nb(n), 256, 0, s>>>(n, v, x);
 }

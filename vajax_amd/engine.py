@@ -191,7 +191,18 @@ class DeviceCircuit:
         d.res_seg = L.ptr(hold(np.ascontiguousarray(red.res_seg, np.int32)))
         d.br_ptr = L.ptr(hold(br_ptr))
         d.br_ent = L.ptr(hold(br_ent))
-        d.conv_mask = None
+        # residual test mask: internal device nodes and NOI nodes are left out
+        # (solver_factories.py:86-92); NOI rows for the solve
+        skip = [topo.node_names[n] - 1 for n in list(net.internal_nodes) + list(net.noi_nodes)]
+        if skip:
+            mask = np.ones(topo.n_unknowns, dtype=np.int32)
+            mask[skip] = 0
+            d.conv_mask = L.ptr(hold(mask))
+        else:
+            d.conv_mask = None
+        noi = sorted({topo.node_names[n] - 1 for n in net.noi_nodes})
+        d.n_noi = len(noi)
+        d.noi_rows = L.ptr(hold(np.asarray(noi, dtype=np.int32))) if noi else None
         models = list(topo.models.items())
         if len(models) > L.MAX_MODELS:
             raise ValueError("too many model groups")

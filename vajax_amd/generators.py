@@ -325,8 +325,12 @@ def pack(nets, title: str = "batch") -> Netlist:
     groups: Dict[str, list] = {}
     vs: list = []
     is_: list = []
+    internal: list = []
+    noi: list = []
     for b, net in enumerate(nets):
         pre = f"i{b:0{bw}d}."
+        internal += [pre + n for n in net.internal_nodes]
+        noi += [pre + n for n in net.noi_nodes]
         gidx = net.nodes.index(net.ground)
         remap = np.empty(len(net.nodes), dtype=np.int64)
         for k, name in enumerate(net.nodes):
@@ -370,6 +374,8 @@ def pack(nets, title: str = "batch") -> Netlist:
         isources=cat_sources("isource", is_),
         analysis=dict(first.analysis),
         options=dict(first.options),
+        internal_nodes=internal,
+        noi_nodes=noi,
     )
 
 

@@ -130,6 +130,12 @@ class Netlist:
     isources: SourceGroup
     analysis: Dict[str, float] = field(default_factory=dict)
     options: Dict[str, object] = field(default_factory=dict)
+    # Internal device nodes are left out of the Newton residual test, and
+    # noise-only internal (NOI) nodes are also held at 0 V with their rows and
+    # columns of the linear system replaced by the identity
+    # (vajax/analysis/solver_factories.py:37-137, 1268-1276).
+    internal_nodes: List[str] = field(default_factory=list)
+    noi_nodes: List[str] = field(default_factory=list)
 
     def device_count(self) -> int:
         return sum(g.count for g in self.groups.values())
@@ -193,7 +199,7 @@ class NetlistBuilder:
         for k, d in SOURCE_DEFAULTS.items():
             s[k].append(float(params.get(k, d)))
 
-    def build(self, analysis=None, options=None) -> Netlist:
+    def build(self, analysis=None, options=None, internal_nodes=(), noi_nodes=()) -> Netlist:
         groups = {}
         for model, terms in self._dev_terms.items():
             groups[model] = DeviceGroup(
@@ -220,4 +226,6 @@ class NetlistBuilder:
             isources=srcs["isource"],
             analysis=dict(analysis or {}),
             options=dict(options or {}),
+            internal_nodes=[n for n in internal_nodes],
+            noi_nodes=[n for n in noi_nodes],
         )
