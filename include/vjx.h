@@ -74,12 +74,29 @@ int vjx_lu_solve(vjx_pattern* pat, const double* data_dev, const double* b_dev, 
 /* y = A x (umfpack_ffi.cpp:345-370, UmfpackDotF64Impl). */
 int vjx_spmv(vjx_pattern* pat, const double* data_dev, const double* x_dev, double* y_dev);
 
+/* Solve A^T x = b (umfpack_ffi.cpp:388-481, UmfpackSolveTransposeF64Impl,
+ * UMFPACK_At; the reverse-mode rule of vajax/sparse/umfpack_jax.py:148-177).
+ * A^T is gathered from data_dev on the device into a second handle that the
+ * pattern keeps, with its own ordering and pivots (analysed on the first
+ * call, re-pivoted on a pivot warning), so each call costs one numeric
+ * refactorisation and one solve, as the reference's numeric factor per call
+ * does.  precision and refine_steps as for vjx_lu_refactor / vjx_lu_solve. */
+int vjx_lu_solve_transpose(vjx_pattern* pat, const double* data_dev, const double* b_dev,
+                           double* x_dev, int32_t precision, int32_t refine_steps);
+
 /* One-shot host-buffer solve with the reference FFI's semantics:
  * solve(indptr, indices, data, b) -> x (umfpack_ffi.cpp:329-338).  Keeps a
  * handle cached on the (n, nnz, pattern) key like the reference's global
  * symbolic cache; vjx_clear_cache() drops it. */
 int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr_host, const int32_t* indices_host,
                    const double* data_host, const double* b_host, double* x_host);
+/* Host-buffer forms of the FFI's transposed solve and product
+ * (umfpack_ffi.cpp:388-481 solve_transpose, :345-370 dot), on the same cache. */
+int vjx_solve_transpose_host(int32_t n, int32_t nnz, const int32_t* indptr_host,
+                             const int32_t* indices_host, const double* data_host,
+                             const double* b_host, double* x_host);
+int vjx_dot_host(int32_t n, int32_t nnz, const int32_t* indptr_host, const int32_t* indices_host,
+                 const double* data_host, const double* x_host, double* y_host);
 void vjx_clear_cache(void);
 
 typedef struct {
@@ -318,6 +335,48 @@ typedef struct {
 } vjx_tran_result;
 
 int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx_tran_result* res);
+
+/* One Newton solve at fixed integration coefficients: the reference's
+ * nr_solve(X_init, vsource_vals, isource_vals, Q_prev, integ_c0, ..., gmin,
+ * gshunt, integ_c1, integ_d1, dQdt_prev, integ_c2, Q_prev2, limit_state_in,
+ * res_tol_floor) -> (X, iterations, converged, max_f, Q, dQdt, I_vsource,
+ * limit_state, max_res_contrib) (vajax/analysis/solver_factories.py:604-727),
+ * iterating build -> residual / delta / limit tests -> NOI -> LU refactor ->
+ * solve -> damped update on the device until converged or cfg->tran_itl
+ * iterations, then one more build at the final X for Q, I_vsource and
+ * max_res_contrib.  Of cfg only reltol, abstol, vntol, nr_damping, tran_itl,
+ * precision, refine_steps, use_graph, autotune and repivot_growth are read;
+ * gmin comes from the arguments.  All vectors are device buffers; NULL
+ * inputs mean zeros (Q_prev, dQdt_prev, Q_prev2, limit_state_in) or abstol
+ * (res_tol_floor). */
+typedef struct {
+  const double* X_init;          /* [n_total + m] (ground first) */
+  const double* vsource_vals;    /* [m] */
+  const double* isource_vals;    /* [k] */
+  const double* Q_prev;          /* [n_unk] */
+  const double* dQdt_prev;       /* [n_unk] */
+  const double* Q_prev2;         /* [n_unk] */
+  const double* limit_state_in;  /* [vjx_circuit_limit_states(c)] */
+  const double* res_tol_floor;   /* [n_unk] */
+  double c0, c1, d1, c2;
+  double gmin, gshunt;
+} vjx_newton_args;
+
+typedef struct {
+  double* X;               /* [n_total + m] */
+  double* Q;               /* [n_unk] */
+  double* dQdt;            /* [n_unk]: c0 Q + c1 Q_prev + d1 dQdt_prev + c2 Q_prev2 */
+  double* I_vsource;       /* [max(m, 1)] */
+  double* limit_state;     /* [n_lim] (may be NULL) */
+  double* max_res_contrib; /* [n_unk] */
+  int32_t iterations;
+  int32_t converged;
+  double max_f;
+  int32_t repivots;
+} vjx_newton_result;
+
+int vjx_newton_solve(vjx_circuit* c, const vjx_tran_config* cfg, const vjx_newton_args* args,
+                     vjx_newton_result* res);
 
 /* Phase timing of the Newton iterations of vjx_transient_run, from HIP events
  * recorded on the library's stream (only when use_graph = 0).  Times are

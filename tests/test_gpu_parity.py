@@ -205,6 +205,39 @@ def test_solve_host_matches_scipy():
     assert np.allclose(x, xs, rtol=1e-10, atol=1e-14)
 
 
+def test_solve_transpose_and_dot_match_scipy():
+    """solve_transpose / dot of the linear-solver FFI (umfpack_ffi.cpp:345-481)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+
+    from vajax_amd import _lib as L
+    from vajax_amd.sparse import SparseLU, dot, solve_transpose
+
+    net = G.gate_dag(300, seed=2)
+    cir = omna.setup(net)
+    rng = np.random.default_rng(5)
+    for it in range(2):  # the second call reuses the cached A^T handle
+        X = np.zeros(cir.n_total + cir.m)
+        X[1:cir.n_total] = rng.uniform(0.0, 1.0, cir.n_unk)  # an operating-point-like state
+        csr, *_ = omna.build_system(cir, X, np.ones(cir.m), np.zeros(0), np.zeros(cir.n_unk), 1e12)
+        A = sp.csr_matrix((csr, cir.indices, cir.indptr), shape=(cir.n_aug, cir.n_aug))
+        assert (A - A.T).count_nonzero() > 0  # the test means something
+        b = rng.standard_normal(cir.n_aug)
+        xt = solve_transpose(cir.indptr, cir.indices, csr, b)
+        want = spla.spsolve(A.T.tocsc(), b)
+        assert np.allclose(xt, want, rtol=1e-9, atol=1e-14), it
+        assert np.allclose(dot(cir.indptr, cir.indices, csr, b), A @ b, rtol=1e-14, atol=0)
+    lu = SparseLU(cir.indptr, cir.indices)
+    d = torch.as_tensor(csr, device="cuda")
+    bd = torch.as_tensor(b, device="cuda")
+    for prec, refine, tol in ((L.VJX_PREC_F64, 0, 1e-9), (L.VJX_PREC_F32, 2, 1e-8)):
+        x = torch.empty_like(bd)
+        lu.solve_transpose(d, bd, x, prec, refine)
+        res = np.linalg.norm(A.T @ x.cpu().numpy() - b) / np.linalg.norm(b)
+        assert res <= tol, (prec, res)
+
+
 def test_transient_rc_fixed_step():
     net = G.rc_ladder(200, seed=0)
     _, ref = run_oracle(net, 1e-9, 1e-12, fixed=True, tran_fs=1.0)

@@ -109,6 +109,23 @@ class TranResult(C.Structure):
     ]
 
 
+class NewtonArgs(C.Structure):
+    _fields_ = [
+        ("X_init", vp), ("vsource_vals", vp), ("isource_vals", vp), ("Q_prev", vp),
+        ("dQdt_prev", vp), ("Q_prev2", vp), ("limit_state_in", vp), ("res_tol_floor", vp),
+        ("c0", C.c_double), ("c1", C.c_double), ("d1", C.c_double), ("c2", C.c_double),
+        ("gmin", C.c_double), ("gshunt", C.c_double),
+    ]
+
+
+class NewtonResult(C.Structure):
+    _fields_ = [
+        ("X", vp), ("Q", vp), ("dQdt", vp), ("I_vsource", vp), ("limit_state", vp),
+        ("max_res_contrib", vp), ("iterations", C.c_int32), ("converged", C.c_int32),
+        ("max_f", C.c_double), ("repivots", C.c_int32),
+    ]
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("lu_ms", C.c_double), ("solve_ms", C.c_double),
                 ("n_iterations", C.c_int64), ("kernel_ms", C.c_double * 8),
@@ -135,6 +152,9 @@ _PROTOS = {
     "vjx_lu_solve": (C.c_int, [vp, vp, vp, vp, C.c_int32]),
     "vjx_spmv": (C.c_int, [vp, vp, vp, vp]),
     "vjx_solve_host": (C.c_int, [C.c_int32, C.c_int32, i32p, i32p, f64p, f64p, f64p]),
+    "vjx_lu_solve_transpose": (C.c_int, [vp, vp, vp, vp, C.c_int32, C.c_int32]),
+    "vjx_solve_transpose_host": (C.c_int, [C.c_int32, C.c_int32, i32p, i32p, f64p, f64p, f64p]),
+    "vjx_dot_host": (C.c_int, [C.c_int32, C.c_int32, i32p, i32p, f64p, f64p, f64p]),
     "vjx_clear_cache": (None, []),
     "vjx_lu_get_info": (C.c_int, [vp, C.POINTER(LuInfo)]),
     "vjx_lu_get_perm": (C.c_int, [vp, i32p, i32p]),
@@ -157,6 +177,8 @@ _PROTOS = {
     "vjx_limit_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, C.c_double, C.c_double, vp]),
     "vjx_eval_sources": (C.c_int, [vp, C.c_double, vp, vp]),
     "vjx_transient_run": (C.c_int, [vp, C.POINTER(TranConfig), C.POINTER(TranResult)]),
+    "vjx_newton_solve": (C.c_int, [vp, C.POINTER(TranConfig), C.POINTER(NewtonArgs),
+                                   C.POINTER(NewtonResult)]),
     "vjx_profile_enable": (C.c_int, [C.c_int32]),
     "vjx_profile_reset": (C.c_int, []),
     "vjx_profile_read": (C.c_int, [C.POINTER(PhaseTimes)]),

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "circuit.h"
+#include "transient.h"
 #include "internal.h"
 
 namespace vjx {
@@ -58,6 +59,7 @@ int vjx_pattern_create(int32_t n, int32_t nnz, const int32_t* indptr, const int3
 
 void vjx_pattern_destroy(vjx_pattern* pat) {
   if (!pat) return;
+  if (pat->trans) vjx_pattern_destroy(pat->trans);
   if (pat->stream) (void)hipStreamDestroy(pat->stream);
   delete pat;
 }
@@ -328,6 +330,64 @@ int vjx_spmv(vjx_pattern* pat, const double* data_dev, const double* x_dev, doub
   return VJX_OK;
 }
 
+// A^T through the handle's second pattern: gather the values on the device,
+// factorise (analysis on the first call, re-pivot on a pivot warning), solve.
+int vjx_lu_solve_transpose(vjx_pattern* pat, const double* data_dev, const double* b_dev,
+                           double* x_dev, int32_t prec, int32_t refine_steps) {
+  if (!pat || (!data_dev && pat->nnz) || !b_dev || !x_dev) {
+    set_error("vjx_lu_solve_transpose: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (prec != VJX_PREC_F64 && prec != VJX_PREC_F32) {
+    set_error("vjx_lu_solve_transpose: precision must be VJX_PREC_F64 or VJX_PREC_F32");
+    return VJX_ERR_ARG;
+  }
+  if (!pat->trans) {
+    const int n = pat->n, nnz = pat->nnz;
+    std::vector<int32_t> tp(n + 1, 0), ti(nnz), src(nnz);
+    for (int e = 0; e < nnz; ++e) ++tp[pat->indices[e] + 1];
+    for (int j = 0; j < n; ++j) tp[j + 1] += tp[j];
+    std::vector<int32_t> next(tp.begin(), tp.end() - 1);
+    // rows of A in increasing order: columns of each row of A^T come out sorted
+    for (int i = 0; i < n; ++i)
+      for (int e = pat->indptr[i]; e < pat->indptr[i + 1]; ++e) {
+        const int pos = next[pat->indices[e]]++;
+        ti[pos] = i;
+        src[pos] = e;
+      }
+    vjx_pattern* t = nullptr;
+    int rc = vjx_pattern_create(n, nnz, tp.data(), ti.data(), &t);
+    if (rc < 0) return rc;
+    t->opts = pat->opts;
+    hipError_t e1 = pat->d_trans_src.upload(src.data(), nnz);
+    hipError_t e2 = pat->d_trans_data.alloc(std::max(nnz, 1));
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      vjx_pattern_destroy(t);
+      VJX_TRY_HIP(e1 != hipSuccess ? e1 : e2);
+    }
+    pat->trans = t;
+  }
+  vjx_pattern* t = pat->trans;
+  int rc = pattern_stream(t);
+  if (rc < 0) return rc;
+  launch_gather(pat->nnz, pat->d_trans_src.p, data_dev, pat->d_trans_data.p, t->stream);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(t->stream));
+  rc = vjx_lu_refactor(t, pat->d_trans_data.p, prec);
+  if (rc < 0) return rc;
+  if (rc == VJX_WARN_PIVOT) {
+    std::vector<double> h(pat->nnz);
+    VJX_TRY_HIP(hipMemcpy(h.data(), pat->d_trans_data.p, sizeof(double) * pat->nnz,
+                          hipMemcpyDeviceToHost));
+    rc = vjx_lu_analyze(t, h.data());
+    if (rc < 0) return rc;
+    rc = vjx_lu_refactor(t, pat->d_trans_data.p, prec);
+    if (rc < 0) return rc;
+  }
+  int rc2 = vjx_lu_solve(t, pat->d_trans_data.p, b_dev, x_dev, refine_steps);
+  return rc2 < 0 ? rc2 : rc;
+}
+
 int vjx_lu_download(vjx_pattern* pat, double* lu_host) {
   if (!pat || !lu_host || pat->lu.factored_prec < 0) {
     set_error("vjx_lu_download: nothing factorised");
@@ -350,10 +410,11 @@ static std::map<std::tuple<int, int, std::vector<int32_t>, std::vector<int32_t>>
                 std::unique_ptr<vjx_pattern, void (*)(vjx_pattern*)>>
     g_cache;
 
-int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* indices,
-                   const double* data, const double* b, double* x) {
+// op 0: A x = b, op 1: A^T x = b, op 2: y = A x
+static int host_op(const char* name, int op, int32_t n, int32_t nnz, const int32_t* indptr,
+                   const int32_t* indices, const double* data, const double* b, double* x) {
   if (!indptr || !indices || !data || !b || !x) {
-    set_error("vjx_solve_host: null argument");
+    set_error(std::string(name) + ": null argument");
     return VJX_ERR_ARG;
   }
   std::lock_guard<std::mutex> lock(g_cache_mu);
@@ -364,7 +425,8 @@ int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t*
   if (it == g_cache.end()) {
     int rc = vjx_pattern_create(n, nnz, indptr, indices, &pat);
     if (rc < 0) return rc;
-    rc = vjx_lu_analyze(pat, data);
+    // the transposed solve and the product analyse nothing on A itself
+    rc = op == 0 ? vjx_lu_analyze(pat, data) : VJX_OK;
     if (rc < 0) {
       vjx_pattern_destroy(pat);
       return rc;
@@ -374,9 +436,16 @@ int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t*
     pat = it->second.get();
   }
   DBuf<double> d_data, d_b, d_x;
-  VJX_TRY_HIP(d_data.upload(data, nnz));
-  VJX_TRY_HIP(d_b.upload(b, n));
-  VJX_TRY_HIP(d_x.alloc(n));
+  VJX_TRY_HIP(d_data.upload(data, std::max(nnz, 1)));
+  VJX_TRY_HIP(d_b.upload(b, std::max(n, 1)));
+  VJX_TRY_HIP(d_x.alloc(std::max(n, 1)));
+  if (op != 0) {
+    int rc = op == 1 ? vjx_lu_solve_transpose(pat, d_data.p, d_b.p, d_x.p, VJX_PREC_F64, 0)
+                     : vjx_spmv(pat, d_data.p, d_b.p, d_x.p);
+    if (rc < 0) return rc;
+    VJX_TRY_HIP(hipMemcpy(x, d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    return rc;
+  }
   int rc = vjx_lu_refactor(pat, d_data.p, VJX_PREC_F64);
   if (rc < 0) return rc;
   if (rc == VJX_WARN_PIVOT) {
@@ -390,6 +459,22 @@ int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t*
   if (rc2 < 0) return rc2;
   VJX_TRY_HIP(hipMemcpy(x, d_x.p, n * sizeof(double), hipMemcpyDeviceToHost));
   return rc;
+}
+
+int vjx_solve_host(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* indices,
+                   const double* data, const double* b, double* x) {
+  return host_op("vjx_solve_host", 0, n, nnz, indptr, indices, data, b, x);
+}
+
+int vjx_solve_transpose_host(int32_t n, int32_t nnz, const int32_t* indptr,
+                             const int32_t* indices, const double* data, const double* b,
+                             double* x) {
+  return host_op("vjx_solve_transpose_host", 1, n, nnz, indptr, indices, data, b, x);
+}
+
+int vjx_dot_host(int32_t n, int32_t nnz, const int32_t* indptr, const int32_t* indices,
+                 const double* data, const double* x, double* y) {
+  return host_op("vjx_dot_host", 2, n, nnz, indptr, indices, data, x, y);
 }
 
 void vjx_clear_cache(void) {

@@ -22,6 +22,11 @@ struct vjx_pattern {
   double analyze_seconds = 0.0;
   int n_analyses = 0;
   double tuned_solve_ms = 0.0;  // solve time at the width the autotuner kept
+  // transposed solves: a second handle for the pattern of A^T and, for each
+  // of its entries, the position of the same value in A's data
+  vjx_pattern* trans = nullptr;
+  vjx::DBuf<int32_t> d_trans_src;
+  vjx::DBuf<double> d_trans_data;
   ~vjx_pattern() {
     if (h_status) (void)hipHostFree(h_status);
   }

@@ -648,6 +648,96 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   return dc_conv ? VJX_OK : VJX_WARN_NOCONV;
 }
 
+extern "C" int vjx_newton_solve(vjx_circuit* c, const vjx_tran_config* cfg,
+                                const vjx_newton_args* a, vjx_newton_result* r) {
+  if (!c || !cfg || !a || !r || !a->X_init || !r->X || !r->Q || !r->dQdt || !r->I_vsource ||
+      !r->max_res_contrib || (c->m && !a->vsource_vals) || (c->k && !a->isource_vals)) {
+    set_error("vjx_newton_solve: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (cfg->tran_itl < 1 || (cfg->precision != VJX_PREC_F64 && cfg->precision != VJX_PREC_F32)) {
+    set_error("vjx_newton_solve: invalid configuration");
+    return VJX_ERR_ARG;
+  }
+  ktimer_enable(false);
+  vjx_tran_config lc = *cfg;
+  lc.gmin = a->gmin;
+  Driver D;
+  D.c = c;
+  D.cfg = &lc;
+  int rc = pattern_stream(c->pat);
+  if (rc < 0) return rc;
+  D.s = c->pat->stream;
+  hipStream_t s = D.s;
+  const int n_total = c->n_total, n_unk = c->n_unk, m = c->m, k = c->k;
+  D.n_total = n_total;
+  D.n_unk = n_unk;
+  D.m = m;
+  D.k = k;
+  D.n_aug = c->n_aug;
+  Work& w = D.w;
+  const int nx = n_total + m;
+  VJX_TRY_HIP(w.Xw.alloc(nx));
+  VJX_TRY_HIP(w.Xsave.alloc(nx));
+  for (auto* b : {&w.Qn, &w.dQn, &w.mrc, &w.floor, &w.zeros})
+    VJX_TRY_HIP(b->alloc(std::max(n_unk, 1)));
+  VJX_TRY_HIP(w.f.alloc(std::max(c->n_aug, 1)));
+  VJX_TRY_HIP(w.delta.alloc(std::max(c->n_aug, 1)));
+  VJX_TRY_HIP(w.csr.alloc(std::max(c->nse, 1)));
+  VJX_TRY_HIP(w.vsrc.alloc(std::max(m, 1)));
+  VJX_TRY_HIP(w.isrc.alloc(std::max(k, 1)));
+  VJX_TRY_HIP(w.Ivs.alloc(std::max(m, 1)));
+  VJX_TRY_HIP(w.st.alloc(1));
+  VJX_TRY_HIP(hipHostMalloc((void**)&w.h_st, sizeof(DevStatus), hipHostMallocDefault));
+  const int nl = c->n_lim;
+  D.n_lim = nl;
+  if (nl) {
+    for (auto* b : {&w.ls_iter, &w.ls_out, &w.ls_save}) VJX_TRY_HIP(b->alloc(nl));
+    if (a->limit_state_in)
+      launch_copy(nl, a->limit_state_in, w.ls_iter.p, s);
+    else
+      launch_fill(nl, 0.0, w.ls_iter.p, s);
+  }
+  launch_fill(n_unk, 0.0, w.zeros.p, s);
+  launch_fill(m, 0.0, w.Ivs.p, s);
+  launch_copy(nx, a->X_init, w.Xw.p, s);
+  launch_copy(m, a->vsource_vals, w.vsrc.p, s);
+  launch_copy(k, a->isource_vals, w.isrc.p, s);
+  if (a->res_tol_floor)
+    launch_copy(n_unk, a->res_tol_floor, w.floor.p, s);
+  else
+    launch_fill(n_unk, cfg->abstol, w.floor.p, s);
+  VJX_TRY_HIP(hipGetLastError());
+  const double* Qp = a->Q_prev ? a->Q_prev : w.zeros.p;
+  const double* dQ = a->dQdt_prev ? a->dQdt_prev : w.zeros.p;
+  const double* Q2 = a->Q_prev2 ? a->Q_prev2 : w.zeros.p;
+  if (!c->pat->lu.analyzed || !c->pat->uploaded) {
+    // symbolic analysis on the Jacobian at X_init with these coefficients
+    rc = D.upload_scalars(a->c0, a->c1, a->d1, a->c2, a->gshunt, 0);
+    if (rc < 0) return rc;
+    rc = D.ensure_analysis(Qp, dQ, Q2);
+    if (rc < 0) return rc;
+  }
+  int iters = 0;
+  bool conv = false;
+  double maxf = 0.0;
+  rc = D.nr_solve(a->c0, a->c1, a->d1, a->c2, a->gshunt, Qp, dQ, Q2, w.floor.p, iters, conv, maxf);
+  if (rc < 0) return rc;
+  launch_copy(nx, w.Xw.p, r->X, s);
+  launch_copy(n_unk, w.Qn.p, r->Q, s);
+  launch_copy(n_unk, w.dQn.p, r->dQdt, s);
+  launch_copy(std::max(m, 1), w.Ivs.p, r->I_vsource, s);
+  launch_copy(n_unk, w.mrc.p, r->max_res_contrib, s);
+  if (nl && r->limit_state) launch_copy(nl, w.ls_iter.p, r->limit_state, s);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(s));
+  r->iterations = iters;
+  r->converged = conv ? 1 : 0;
+  r->max_f = maxf;
+  r->repivots = D.repivots;
+  return conv ? VJX_OK : VJX_WARN_NOCONV;
+}
+
 extern "C" int vjx_profile_enable(int32_t enable) {
   vjx::g_profile = enable != 0;
   return VJX_OK;

@@ -99,6 +99,12 @@ __global__ void k_copy(int n, const double* __restrict__ a, double* b) {
   if (i < n) b[i] = a[i];
 }
 
+__global__ void k_gather(int n, const int32_t* __restrict__ idx, const double* __restrict__ a,
+                         double* b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[idx[i]];
+}
+
 __global__ void k_dqdt(int n, double c0, double c1, double d1, double c2,
                        const double* __restrict__ Q, const double* __restrict__ Qp,
                        const double* __restrict__ dQ, const double* __restrict__ Q2, double* out) {
@@ -212,6 +218,9 @@ void launch_fill(int n, double v, double* x, hipStream_t s) {
 }
 void launch_copy(int n, const double* a, double* b, hipStream_t s) {
   if (n) k_copy<<<nb(n), 256, 0, s>>>(n, a, b);
+}
+void launch_gather(int n, const int32_t* idx, const double* a, double* b, hipStream_t s) {
+  if (n) k_gather<<<nb(n), 256, 0, s>>>(n, idx, a, b);
 }
 void launch_dqdt(int n, double c0, double c1, double d1, double c2, const double* Q,
                  const double* Qp, const double* dQ, const double* Q2, double* out,

@@ -269,6 +269,54 @@ class DeviceCircuit:
                 "vjx_build_system")
         return out
 
+    def nr_solve(self, X_init, vsource_vals, isource_vals, Q_prev, integ_c0, gmin=1e-12,
+                 gshunt=0.0, integ_c1=0.0, integ_d1=0.0, dQdt_prev=None, integ_c2=0.0,
+                 Q_prev2=None, limit_state_in=None, res_tol_floor=None,
+                 options: Optional[SimulationOptions] = None, precision: str = "f64",
+                 refine_steps: Optional[int] = None, use_graph: bool = False):
+        """The reference's nr_solve (solver_factories.py:604-727) on the device:
+        torch CUDA tensors in, ``(X, iterations, converged, max_f, Q, dQdt,
+        I_vsource, limit_state, max_res_contrib)`` out (tensors and Python
+        scalars; ``limit_state`` None when no device is limited)."""
+        import torch
+
+        t = self.topo
+        o = options or SimulationOptions()
+        dev = X_init.device
+        nl = self.n_limit_states
+        tc = L.TranConfig()
+        tc.reltol, tc.abstol, tc.vntol = o.reltol, o.abstol, o.vntol
+        tc.nr_damping, tc.tran_itl = o.nr_damping, int(o.tran_itl)
+        tc.repivot_growth = float(o.repivot_growth)
+        tc.precision = L.VJX_PREC_F32 if precision == "f32" else L.VJX_PREC_F64
+        tc.refine_steps = (1 if precision == "f32" else 0) if refine_steps is None else refine_steps
+        tc.use_graph = 1 if use_graph else 0
+
+        def p(x):
+            return C.c_void_p(x.data_ptr()) if x is not None else None
+
+        def vec(n):
+            return torch.empty(n, dtype=torch.float64, device=dev)
+
+        out = {"X": vec(t.n_total + t.n_vsources), "Q": vec(t.n_unknowns),
+               "dQdt": vec(t.n_unknowns), "I_vsource": vec(max(t.n_vsources, 1)),
+               "limit_state": vec(nl) if nl else None, "max_res_contrib": vec(t.n_unknowns)}
+        a = L.NewtonArgs()
+        a.X_init, a.vsource_vals, a.isource_vals = p(X_init), p(vsource_vals), p(isource_vals)
+        a.Q_prev, a.dQdt_prev, a.Q_prev2 = p(Q_prev), p(dQdt_prev), p(Q_prev2)
+        a.limit_state_in, a.res_tol_floor = p(limit_state_in), p(res_tol_floor)
+        a.c0, a.c1, a.d1, a.c2 = integ_c0, integ_c1, integ_d1, integ_c2
+        a.gmin, a.gshunt = gmin, gshunt
+        r = L.NewtonResult()
+        r.X, r.Q, r.dQdt, r.I_vsource = p(out["X"]), p(out["Q"]), p(out["dQdt"]), p(out["I_vsource"])
+        r.limit_state, r.max_res_contrib = p(out["limit_state"]), p(out["max_res_contrib"])
+        torch.cuda.synchronize()
+        L.check(L.lib.vjx_newton_solve(self.h, C.byref(tc), C.byref(a), C.byref(r)),
+                "vjx_newton_solve")
+        return (out["X"], int(r.iterations), bool(r.converged), float(r.max_f), out["Q"],
+                out["dQdt"], out["I_vsource"][:t.n_vsources], out["limit_state"],
+                out["max_res_contrib"])
+
 
 def init_mid_rail(topo: Topology, net: Netlist) -> np.ndarray:
     """Initial guess for the DC solve (full_mna.py:694-745, sources.py:553-568)."""

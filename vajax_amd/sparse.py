@@ -125,6 +125,14 @@ class SparseLU:
                                    C.c_void_p(b_dev.data_ptr()), C.c_void_p(x_dev.data_ptr()),
                                    refine_steps), "vjx_lu_solve")
 
+    def solve_transpose(self, data_dev, b_dev, x_dev, precision: int = L.VJX_PREC_F64,
+                        refine_steps: int = 0) -> int:
+        """A^T x = b: factorises A^T (a second handle kept by this one) and solves."""
+        return L.check(L.lib.vjx_lu_solve_transpose(self.h, C.c_void_p(data_dev.data_ptr()),
+                                                    C.c_void_p(b_dev.data_ptr()),
+                                                    C.c_void_p(x_dev.data_ptr()), precision,
+                                                    refine_steps), "vjx_lu_solve_transpose")
+
     def spmv(self, data_dev, x_dev, y_dev) -> None:
         L.check(L.lib.vjx_spmv(self.h, C.c_void_p(data_dev.data_ptr()),
                                C.c_void_p(x_dev.data_ptr()), C.c_void_p(y_dev.data_ptr())),
@@ -146,6 +154,27 @@ def solve(indptr, indices, data, b) -> np.ndarray:
     L.check(L.lib.vjx_solve_host(n, len(ix), L.ptr(ip), L.ptr(ix), L.ptr(d, C.c_double),
                                  L.ptr(bb, C.c_double), L.ptr(x, C.c_double)), "vjx_solve_host")
     return x
+
+
+def _host_op(fn, name, indptr, indices, data, v) -> np.ndarray:
+    ip, ix = _i32(indptr), _i32(indices)
+    d, vv = _f64(data), _f64(v)
+    n = len(ip) - 1
+    out = np.zeros(n)
+    L.check(fn(n, len(ix), L.ptr(ip), L.ptr(ix), L.ptr(d, C.c_double), L.ptr(vv, C.c_double),
+               L.ptr(out, C.c_double)), name)
+    return out
+
+
+def solve_transpose(indptr, indices, data, b) -> np.ndarray:
+    """x = A^{-T} b (vajax/sparse/umfpack_jax.py:148-177) on host buffers."""
+    return _host_op(L.lib.vjx_solve_transpose_host, "vjx_solve_transpose_host", indptr, indices,
+                    data, b)
+
+
+def dot(indptr, indices, data, x) -> np.ndarray:
+    """b = A x (vajax/sparse/umfpack_jax.py:118-145) on host buffers."""
+    return _host_op(L.lib.vjx_dot_host, "vjx_dot_host", indptr, indices, data, x)
 
 
 def clear_cache() -> None:
