@@ -238,7 +238,10 @@ int vjx_circuit_limit_states(const vjx_circuit* c);
  * limited devices also limit_state_out from limit_state_in (NULL: zeros) and
  * the lim_rhs corrections of f (mna_builder.py:1039, mna.py:673-719).
  * Device pointers; csr_data_dev may be NULL to skip the Jacobian,
- * limit_state_out_dev may be NULL when the circuit has no limit states. */
+ * limit_state_out_dev may be NULL when the circuit has no limit states.
+ * gmin is both the node-row diagonal regularisation and the device simparam
+ * $gmin: the reference's build closure fixes the former at the options' gmin
+ * (mna_builder.py:384), so pass that value here. */
 int vjx_build_system(vjx_circuit* c, const double* X_dev, const double* vsrc_dev,
                      const double* isrc_dev, const double* Q_prev_dev, const double* dQdt_prev_dev,
                      const double* Q_prev2_dev, double c0, double c1, double d1, double c2,
@@ -296,7 +299,7 @@ typedef struct {
   double reltol;      /* 1e-3 */
   double abstol;      /* 1e-12 */
   double vntol;       /* 1e-6 */
-  double gmin;        /* 1e-12 netlist option: diagonal regularisation */
+  double gmin;        /* 1e-12 netlist option: diagonal regularisation (and $gmin) */
   double nr_damping;  /* 1.0 */
   double lte_ratio;   /* 3.5 */
   double redo_factor; /* 2.5 */
@@ -344,9 +347,13 @@ int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx_tran_resul
  * iterating build -> residual / delta / limit tests -> NOI -> LU refactor ->
  * solve -> damped update on the device until converged or cfg->tran_itl
  * iterations, then one more build at the final X for Q, I_vsource and
- * max_res_contrib.  Of cfg only reltol, abstol, vntol, nr_damping, tran_itl,
- * precision, refine_steps, use_graph, autotune and repivot_growth are read;
- * gmin comes from the arguments.  All vectors are device buffers; NULL
+ * max_res_contrib.  Of cfg only gmin (the options' gmin: diagonal
+ * regularisation of node rows, fixed as in mna_builder.py:384), reltol,
+ * abstol, vntol, nr_damping, tran_itl, precision, refine_steps, use_graph,
+ * autotune and repivot_growth are read.  args->gmin is the device simparam
+ * $gmin (mna_builder.py:486, 649-651) that homotopy gdev stepping raises;
+ * only models that read $simparam("gmin") see it, and none of the models
+ * built here do.  All vectors are device buffers; NULL
  * inputs mean zeros (Q_prev, dQdt_prev, Q_prev2, limit_state_in) or abstol
  * (res_tol_floor). */
 typedef struct {

@@ -59,7 +59,8 @@ def _compare(torch, net, X0, vs, isv, Qp, c0, c1=0.0, d1=0.0, dQ=None, c2=0.0, Q
                               res_tol_floor=_dev(torch, floor), options=o)
     X, iters, conv, max_f, Q, dQdt, Ivs, ls, mrc = got
     assert iters == want.iterations and conv == want.converged
-    assert max_f == pytest.approx(want.max_f, rel=1e-6, abs=1e-15)
+    # max_f of a converged solve is rounding noise (~1e-11 A here): absolute bar
+    assert max_f == pytest.approx(want.max_f, rel=1e-6, abs=1e-12)
     for name, g, w in (("X", X, want.X), ("Q", Q, want.Q), ("dQdt", dQdt, want.dQdt),
                        ("I_vsource", Ivs, want.I_vsource), ("mrc", mrc, want.max_res_contrib)):
         assert _close(g, w), name

@@ -256,6 +256,7 @@ int vjx_build_system(vjx_circuit* c, const double* X, const double* vsrc, const 
   h->c2 = c2;
   h->d1 = d1;
   h->gmin = gmin;
+  h->gmin_dev = gmin;
   h->gshunt = gshunt;
   h->reltol = 1e-3;
   h->vntol = 1e-6;

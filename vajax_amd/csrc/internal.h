@@ -54,7 +54,11 @@ struct DBuf {
 // graphs replay with current values.
 struct DevScalars {
   double c0, c1, c2, d1;
-  double gmin, gshunt;
+  // gmin: diagonal regularisation of node rows (the options' gmin,
+  // mna_builder.py:384, 969); gmin_dev: the device simparam $gmin that
+  // homotopy gdev stepping varies (mna_builder.py:486, 649-651) and that
+  // only models reading $simparam("gmin") see -- none of the models here
+  double gmin, gshunt, gmin_dev;
   double reltol, vntol, abstol, nr_damping, max_step;
   int32_t iteration;
   int32_t pad;

@@ -68,6 +68,7 @@ struct Driver {
   int n_lim = 0;
   int64_t total_evals = 0;
   int repivots = 0;
+  double gmin_dev = -1.0;  // device simparam $gmin; < 0: the options' gmin
 
   int upload_scalars(double c0, double c1, double d1, double c2, double gshunt, int it) {
     DevScalars* h = c->h_sc;
@@ -76,6 +77,7 @@ struct Driver {
     h->c2 = c2;
     h->d1 = d1;
     h->gmin = cfg->gmin;
+    h->gmin_dev = gmin_dev < 0.0 ? cfg->gmin : gmin_dev;
     h->gshunt = gshunt;
     h->reltol = cfg->reltol;
     h->vntol = cfg->vntol;
@@ -660,11 +662,10 @@ extern "C" int vjx_newton_solve(vjx_circuit* c, const vjx_tran_config* cfg,
     return VJX_ERR_ARG;
   }
   ktimer_enable(false);
-  vjx_tran_config lc = *cfg;
-  lc.gmin = a->gmin;
   Driver D;
   D.c = c;
-  D.cfg = &lc;
+  D.cfg = cfg;
+  D.gmin_dev = a->gmin;
   int rc = pattern_stream(c->pat);
   if (rc < 0) return rc;
   D.s = c->pat->stream;

@@ -17,7 +17,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 from dataclasses import dataclass, field, fields, replace
-from typing import Any, Dict, Optional
+from typing import Any, Dict, Optional, Tuple
 
 import numpy as np
 
@@ -71,6 +71,22 @@ class SimulationOptions:
     # re-run the symbolic analysis when the factors' max |L| exceeds this
     # (pivot growth; 0 keeps the library default 1e12)
     repivot_growth: float = 0.0
+    # DC operating point: Newton caps and the homotopy chain (options.py:63-67,
+    # 148-190)
+    op_itl: int = 100
+    op_itlcont: int = 50
+    op_homotopy: Tuple[str, ...] = ("gdev", "gshunt", "src")
+    homotopy_gminsteps: int = 100
+    homotopy_gminfactor: float = 10.0
+    homotopy_maxgminfactor: float = 10.0
+    homotopy_mingminfactor: float = 1.00005
+    homotopy_startgmin: float = 1e-3
+    homotopy_maxgmin: float = 1e2
+    homotopy_mingmin: float = 1e-15
+    homotopy_srcsteps: int = 100
+    homotopy_srcstep: float = 0.01
+    homotopy_srcscale: float = 3.0
+    homotopy_minsrcstep: float = 1e-7
 
 
 @dataclass
@@ -224,8 +240,9 @@ class DeviceCircuit:
         del keep
 
     def __del__(self):
-        if getattr(self, "h", None):
-            L.lib.vjx_circuit_destroy(self.h)
+        lib = getattr(L, "lib", None) if L is not None else None  # gone at shutdown
+        if getattr(self, "h", None) and lib is not None:
+            lib.vjx_circuit_destroy(self.h)
             self.h = None
 
     @property
@@ -286,6 +303,7 @@ class DeviceCircuit:
         nl = self.n_limit_states
         tc = L.TranConfig()
         tc.reltol, tc.abstol, tc.vntol = o.reltol, o.abstol, o.vntol
+        tc.gmin = o.gmin  # node-row diagonal regularisation; `gmin` below is $gmin
         tc.nr_damping, tc.tran_itl = o.nr_damping, int(o.tran_itl)
         tc.repivot_growth = float(o.repivot_growth)
         tc.precision = L.VJX_PREC_F32 if precision == "f32" else L.VJX_PREC_F64
@@ -316,6 +334,42 @@ class DeviceCircuit:
         return (out["X"], int(r.iterations), bool(r.converged), float(r.max_f), out["Q"],
                 out["dQdt"], out["I_vsource"][:t.n_vsources], out["limit_state"],
                 out["max_res_contrib"])
+
+    def make_nr_solve(self, options: Optional[SimulationOptions] = None,
+                      max_iterations: Optional[int] = None, precision: str = "f64",
+                      refine_steps: Optional[int] = None, use_graph: bool = False):
+        """A Newton solver with the reference's positional signature
+        ``nr_solve(X, vsource_vals, isource_vals, Q_prev, integ_c0,
+        device_arrays, gmin, gshunt, integ_c1, integ_d1, dQdt_prev, integ_c2,
+        Q_prev2, limit_state_in=None, res_tol_floor=None)`` -> 9-tuple
+        (solver_factories.py:604-620), capped at ``max_iterations`` (default
+        the options' tran_itl); numpy or CUDA-tensor inputs, CUDA tensors
+        out.  ``device_arrays`` is accepted for the signature and unused: the
+        device data lives in the circuit handle."""
+        import torch
+
+        o = replace(options or SimulationOptions())
+        if max_iterations is not None:
+            o.tran_itl = int(max_iterations)
+
+        def dev(x):
+            if x is None or isinstance(x, torch.Tensor):
+                return x
+            return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device="cuda")
+
+        def nr_solve(X, vsource_vals, isource_vals, Q_prev, integ_c0, device_arrays=None,
+                     gmin=1e-12, gshunt=0.0, integ_c1=0.0, integ_d1=0.0, dQdt_prev=None,
+                     integ_c2=0.0, Q_prev2=None, limit_state_in=None, res_tol_floor=None):
+            return self.nr_solve(dev(X), dev(vsource_vals), dev(isource_vals), dev(Q_prev),
+                                 float(integ_c0), gmin=float(gmin), gshunt=float(gshunt),
+                                 integ_c1=float(integ_c1), integ_d1=float(integ_d1),
+                                 dQdt_prev=dev(dQdt_prev), integ_c2=float(integ_c2),
+                                 Q_prev2=dev(Q_prev2), limit_state_in=dev(limit_state_in),
+                                 res_tol_floor=dev(res_tol_floor), options=o,
+                                 precision=precision, refine_steps=refine_steps,
+                                 use_graph=use_graph)
+
+        return nr_solve
 
 
 def init_mid_rail(topo: Topology, net: Netlist) -> np.ndarray:
@@ -452,6 +506,32 @@ class CircuitEngine:
             # one pair is carried, so they must agree
             raise ValueError("AdaptiveConfig and SimulationOptions tolerances differ")
         return tc
+
+    def compute_dc_operating_point(self, precision: str = "f64", use_graph: bool = False):
+        """DC operating point on the device: direct Newton (op_itl iterations)
+        from the starting voltages of dc_operating_point.py:20-122, then the
+        homotopy chain of the options when it fails
+        (dc_operating_point.py:125-242, engine.py:1195-1228).  Returns the
+        node voltages [n_total]; ``self.dc_result`` keeps the full solution
+        and how it was reached."""
+        from .dc import compute_dc_operating_point, get_dc_source_values, get_vdd_value
+        from .dc import initialize_dc_voltages
+
+        if self.topology is None:
+            self.parse()
+        topo = self.topology
+        V0, _ = initialize_dc_voltages(topo, self.netlist, get_vdd_value(self.netlist))
+        X0 = np.zeros(topo.n_total + topo.n_vsources)
+        X0[:topo.n_total] = V0
+        vs, isr = get_dc_source_values(self.netlist)
+        nr = self.device.make_nr_solve(self.options, self.options.op_itl, precision=precision,
+                                       use_graph=use_graph)
+        X, res = compute_dc_operating_point(nr, X0, vs, isr, np.zeros(topo.n_unknowns),
+                                            self.options)
+        X = X.cpu().numpy() if hasattr(X, "cpu") else np.asarray(X)
+        res.V = X
+        self.dc_result = res
+        return X[:topo.n_total]
 
     def run_transient(self) -> TransientResult:
         if not self._prepared:

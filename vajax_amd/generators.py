@@ -121,6 +121,26 @@ def diode_ladder(sections: int = 100, seed: int = 0, t_stop: float = 5e-9,
                    options={"tran_method": "gear2", "icmode": "op"})
 
 
+def diode_clamp_chain(stages: int = 6, vdc: float = 5.0, r: float = 50.0) -> Netlist:
+    """A hard DC operating point: a 5 V source into ``stages`` sections of
+    series R (50 ohm) with two stacked junction diodes (is 1e-14 A, n 1) and a
+    10 k resistor to ground at each section.  From the mid-rail start, plain
+    Newton needs 16 iterations; with fewer allowed the DC solve goes through
+    the homotopy chain."""
+    b = NetlistBuilder(f"diode_clamp_{stages}")
+    b.source("vsource", "v1", "in", "0", type="dc", dc=vdc)
+    prev = "in"
+    p = {"is": 1e-14, "n": 1.0, "cj": 1e-15}
+    for i in range(stages):
+        node, k = f"d{i + 1}", f"k{i + 1}"
+        b.device("resistor", f"r{i + 1}", [prev, node], r=r)
+        b.device("diode", f"da{i + 1}", [node, k], **p)
+        b.device("diode", f"db{i + 1}", [k, "0"], **p)
+        b.device("resistor", f"rk{i + 1}", [k, "0"], r=1e4)
+        prev = node
+    return b.build(analysis={"stop": 1e-9, "step": 1e-12}, options={"icmode": "op"})
+
+
 def ring_oscillator(stages: int = 101, seed: int = 0, t_stop: float = 50e-9, dt: float = 1e-12) -> Netlist:
     """CMOS inverter ring with per-stage random W/L and load caps.
 
