@@ -317,6 +317,10 @@ typedef struct {
   const double* v_init_host; /* [n_total] initial node voltages for the DC solve */
   int32_t autotune;          /* 1: after the symbolic analysis, vjx_lu_autotune */
   double repivot_growth;     /* re-analyse when max |L| exceeds this (0: 1e12) */
+  /* output rows kept on the device: 0 keeps all max_steps rows resident;
+   * k > 0 keeps k rows and copies them to the host result buffers each time
+   * they fill (the checkpoint mode of full_mna.py:1443-1699) */
+  int32_t output_chunk;
 } vjx_tran_config;
 
 typedef struct {

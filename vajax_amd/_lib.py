@@ -94,6 +94,7 @@ class TranConfig(C.Structure):
         ("v_init_host", f64p),
         ("autotune", C.c_int32),
         ("repivot_growth", C.c_double),
+        ("output_chunk", C.c_int32),
     ]
 
 

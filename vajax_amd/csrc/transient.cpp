@@ -346,8 +346,13 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   VJX_TRY_HIP(w.Ivs.alloc(std::max(m, 1)));
   VJX_TRY_HIP(w.hist.alloc((size_t)max_hist * n_total));
   const int mo = std::max(m, 1);
-  VJX_TRY_HIP(w.vout.alloc((size_t)cfg->max_steps * n_total));
-  VJX_TRY_HIP(w.iout.alloc((size_t)cfg->max_steps * mo));
+  // output rows on the device: all of them, or a chunk flushed to the host
+  // result each time it fills
+  const int out_rows = cfg->output_chunk > 0 ? std::min(cfg->output_chunk, cfg->max_steps)
+                                             : cfg->max_steps;
+  int out_base = 0;  // step index of the chunk's first row
+  VJX_TRY_HIP(w.vout.alloc((size_t)out_rows * n_total));
+  VJX_TRY_HIP(w.iout.alloc((size_t)out_rows * mo));
   VJX_TRY_HIP(w.st.alloc(1));
   VJX_TRY_HIP(hipHostMalloc((void**)&w.h_st, sizeof(DevStatus), hipHostMallocDefault));
   const int nl = c->n_lim;
@@ -360,7 +365,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     launch_fill(nl, 0.0, w.ls_step.p, s);
   }
   launch_fill(n_unk, 0.0, w.zeros.p, s);
-  launch_fill((int)((size_t)cfg->max_steps * mo), 0.0, w.iout.p, s);
+  launch_fill((int)((size_t)out_rows * mo), 0.0, w.iout.p, s);
   launch_fill(m, 0.0, w.Ivs.p, s);
 
   // ---------------- initial state (full_mna.py:871-957) ----------------
@@ -398,7 +403,31 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   launch_copy(n_total, w.Xa.p, w.hist.p, s);
   // V_max_historic = |X0|: accept kernel on a zero vector computes max(0,|x|)
   launch_fill(n_total, 0.0, w.Vmax.p, s);
-  launch_output(n_total, m, w.Xa.p, w.Ivs.p, 0, w.vout.p, w.iout.p, s);
+  // copy rows [out_base, end) of the chunk to the host result
+  auto flush_rows = [&](int end) -> int {
+    const size_t rows = (size_t)(end - out_base);
+    if (!rows) return VJX_OK;
+    VJX_TRY_HIP(hipStreamSynchronize(s));
+    VJX_TRY_HIP(hipMemcpy(res->V_out + (size_t)out_base * n_total, w.vout.p,
+                          sizeof(double) * rows * n_total, hipMemcpyDeviceToHost));
+    VJX_TRY_HIP(hipMemcpy(res->I_out + (size_t)out_base * mo, w.iout.p,
+                          sizeof(double) * rows * mo, hipMemcpyDeviceToHost));
+    return VJX_OK;
+  };
+  // enqueue the output of step idx (rows are written in increasing order)
+  auto emit = [&](int idx, int zero_current) -> int {
+    if (idx - out_base == out_rows) {
+      int r2 = flush_rows(idx);
+      if (r2 < 0) return r2;
+      out_base = idx;
+    }
+    const size_t row = (size_t)(idx - out_base);
+    launch_output(n_total, m, w.Xa.p, w.Ivs.p, zero_current, w.vout.p + row * n_total,
+                  w.iout.p + row * mo, s);
+    return VJX_OK;
+  };
+  rc = emit(0, 0);
+  if (rc < 0) return rc;
   {
     // Vmax = |X0|
     DBuf<double> tmp;
@@ -463,8 +492,8 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       if (nl && conv) launch_copy(nl, w.ls_iter.p, w.ls_step.p, s);
       times[i + 1] = t_next;
       launch_copy(nx, w.Xw.p, w.Xa.p, s);
-      launch_output(n_total, m, w.Xa.p, w.Ivs.p, 0, w.vout.p + (size_t)(i + 1) * n_total,
-                    w.iout.p + (size_t)(i + 1) * mo, s);
+      rc = emit(i + 1, 0);
+      if (rc < 0) return rc;
       launch_copy(n_unk, w.Qp.p, w.Q2.p, s);
       launch_copy(n_unk, w.Qn.p, w.Qp.p, s);
       launch_copy(n_unk, w.dQn.p, w.dQ.p, s);
@@ -504,8 +533,8 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
         new_dt = std::min(std::max(new_dt, cfg->min_dt), max_dt);
         new_dt = std::min(new_dt, cfg->t_stop - t);
         times[step_idx] = t;
-        launch_output(n_total, m, w.Xa.p, w.Ivs.p, 1, w.vout.p + (size_t)step_idx * n_total,
-                      w.iout.p + (size_t)step_idx * mo, s);
+        rc = emit(step_idx, 1);
+        if (rc < 0) return rc;
         ++step_idx;
         min_dt_used = std::min(min_dt_used, dt_cur);
         max_dt_used = std::max(max_dt_used, dt_cur);
@@ -621,8 +650,8 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       }
       if (accept) {
         times[step_idx] = t_next;
-        launch_output(n_total, m, w.Xa.p, w.Ivs.p, fail_min ? 1 : 0,
-                      w.vout.p + (size_t)step_idx * n_total, w.iout.p + (size_t)step_idx * mo, s);
+        rc = emit(step_idx, fail_min ? 1 : 0);
+        if (rc < 0) return rc;
         ++step_idx;
         min_dt_used = std::min(min_dt_used, dt_cur);
         max_dt_used = std::max(max_dt_used, dt_cur);
@@ -643,10 +672,8 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   res->total_evals = D.total_evals;
   res->repivots = D.repivots;
   std::memcpy(res->times, times.data(), sizeof(double) * step_idx);
-  VJX_TRY_HIP(hipMemcpy(res->V_out, w.vout.p, sizeof(double) * (size_t)step_idx * n_total,
-                        hipMemcpyDeviceToHost));
-  VJX_TRY_HIP(hipMemcpy(res->I_out, w.iout.p, sizeof(double) * (size_t)step_idx * mo,
-                        hipMemcpyDeviceToHost));
+  rc = flush_rows(step_idx);
+  if (rc < 0) return rc;
   return dc_conv ? VJX_OK : VJX_WARN_NOCONV;
 }
 

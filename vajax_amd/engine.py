@@ -56,6 +56,10 @@ class AdaptiveConfig:
     max_consecutive_rejects: int = 5
 
 
+# output rows kept on the device by default before chunked copies to the host
+OUTPUT_RESIDENT_BYTES = 8 << 30
+
+
 @dataclass
 class SimulationOptions:
     """Options used on the transient path (vajax/analysis/options.py:35-192)."""
@@ -424,7 +428,12 @@ class CircuitEngine:
                 refine_steps: Optional[int] = None, max_steps: Optional[int] = None,
                 use_graph: bool = False, band_width: Optional[int] = None,
                 autotune: bool = False, nd_leaf: Optional[int] = None,
-                **_ignored) -> None:
+                checkpoint_interval: Optional[int] = None, **_ignored) -> None:
+        """``checkpoint_interval``: output rows kept on the device before they
+        are copied to the host (full_mna.py:784-1000 checkpoint mode): None
+        keeps all rows resident unless they would take more than
+        ``OUTPUT_RESIDENT_BYTES``, then chunks of that size; <= 0 keeps all
+        rows resident; k > 0 chunks of k rows."""
         if self.topology is None:
             self.parse()
         if nd_leaf is not None:
@@ -459,7 +468,15 @@ class CircuitEngine:
                 cfg = replace(cfg, **over)
         if precision not in ("f64", "f32"):
             raise ValueError("precision must be 'f64' or 'f32'")
-        self._prep = dict(t_stop=float(t_stop), dt=float(dt), cfg=cfg, precision=precision,
+        topo = self.topology
+        row_bytes = 8 * (topo.n_total + max(topo.n_vsources, 1))
+        if checkpoint_interval is None:
+            fits = max_steps * row_bytes <= OUTPUT_RESIDENT_BYTES
+            chunk = 0 if fits else max(1, OUTPUT_RESIDENT_BYTES // row_bytes)
+        else:
+            chunk = max(int(checkpoint_interval), 0)
+        self._prep = dict(output_chunk=int(chunk))
+        self._prep.update(t_stop=float(t_stop), dt=float(dt), cfg=cfg, precision=precision,
                           refine_steps=(1 if precision == "f32" else 0) if refine_steps is None
                           else int(refine_steps), max_steps=int(max_steps), use_graph=use_graph,
                           autotune=bool(autotune))
@@ -482,6 +499,7 @@ class CircuitEngine:
         tc.use_graph = 1 if p["use_graph"] else 0
         tc.autotune = 1 if p.get("autotune") else 0
         tc.repivot_growth = float(o.repivot_growth)
+        tc.output_chunk = p.get("output_chunk", 0)
         tc.reltol = o.reltol
         tc.abstol = o.abstol
         tc.vntol = o.vntol
