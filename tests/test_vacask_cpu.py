@@ -1,0 +1,177 @@
+"""VACASK netlist front end on the CPU: the reference's parsed-circuit
+expectations (tests/test_vacask_sim_parser.py:130-165 of the reference:
+title, ground, loads, models and their parameters, instance masters and raw
+parameter text), expression and scale-suffix values, the flattening rules of
+vajax/analysis/parsing.py:32-140 (path names, ports, globals, parameter
+environments), and an RC ladder written as a netlist that must give the same
+circuit, bit for bit, as the generator that builds it directly -- and the
+same waveforms through the oracle."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mna as omna
+from oracle import transient as otr
+from vajax_amd import generators as G
+from vajax_amd import vacask as V
+
+RESISTOR_SIM = """Resistor
+load "resistor.osdi"
+
+model resistor resistor
+model vsource vsource
+
+v1 (1 0) vsource dc=1
+r1 (1 0) resistor r=2k
+
+control
+  analysis op1 op
+  print stats
+endc
+"""
+
+
+def test_parsed_circuit_like_the_reference():
+    c = V.parse_netlist("", RESISTOR_SIM)
+    assert c.title == "Resistor" and c.ground == "0"
+    assert "resistor.osdi" in c.loads
+    assert set(c.models) == {"resistor", "vsource"}
+    v1 = next(i for i in c.top_instances if i.name == "v1")
+    r1 = next(i for i in c.top_instances if i.name == "r1")
+    assert (v1.model, v1.params["dc"]) == ("vsource", "1")
+    assert (r1.model, r1.params["r"]) == ("resistor", "2k")
+    assert c.control.analyses[0].kind == "op" and c.control.prints == [("stats", [])]
+
+
+@pytest.mark.parametrize("text,want", [
+    ("1u", 1e-6), ("0.1n", 1e-10), ("1.5meg", 1.5e6), ("2k", 2e3), ("10pF", 1e-11),
+    ("3", 3.0), ("2e-3", 2e-3), (".5m", 5e-4), ("1mil", 25.4e-6), ("100fa", 1e-13),
+    ("5ns", 5e-9), ("1g", 1e9), ("-2.5", None),
+])
+def test_numbers_with_scale_suffixes(text, want):
+    assert V.parse_number(text) == (pytest.approx(want, rel=1e-15) if want is not None else None)
+
+
+def test_expressions():
+    env = {"w": 1e-6, "ld": 0.5e-6, "vdd": 1.2, "v1": 1.0}
+    assert V.evaluate("2*(w+ld)", env) == pytest.approx(3e-6, rel=1e-15)
+    assert V.evaluate("v1*vdd", env) == pytest.approx(1.2, rel=1e-15)
+    assert V.evaluate("2*w + 1u", env) == pytest.approx(3e-6, rel=1e-15)
+    assert V.evaluate("-vdd/2", env) == -0.6
+    assert V.evaluate("sqrt(4)*pi", {}) == 2 * math.pi
+    assert V.evaluate("2**3 - max(1, 3)", {}) == 5.0
+    for bad in ("__import__('os')", "w.real", "[1][0]", "unknown*2", "lambda: 1"):
+        with pytest.raises(V.ExpressionError):
+            V.evaluate(bad, env)
+
+
+HIER_SIM = """Hierarchy
+ground gnd
+global vdd
+model r resistor
+model c capacitor
+model v vsource
+parameters vdd_val=1.2 k=2
+
+subckt cell(a b)
+  parameters rr=1k scale=1
+  r1 (a mid) r r=rr*scale
+  c1 (mid gnd) c c=k*1f
+  r2 (mid b) r r=rr
+ends
+
+subckt pair(x y)
+  parameters s=3
+  u1 (x m) cell rr=2k scale=s
+  u2 (m y) cell
+  rv (m vdd) r r=1meg
+ends
+
+vs (vdd gnd) v dc=vdd_val
+vin (in gnd) v type="pulse" val0=0 val1=vdd_val rise=10p fall=10p width=1n period=2n
+top (in out) pair s=5
+rl (out gnd) r r=10k
+
+control
+  options tran_method="gear2" reltol=1e-4
+  analysis t1 tran step=1p stop=2n icmode="op"
+endc
+"""
+
+
+def test_flattening_rules():
+    c = V.parse_netlist("", HIER_SIM)
+    flat = {name: (terms, master, params) for name, terms, master, params, _ in V.flatten(c)}
+    assert set(flat) == {"vs", "vin", "rl", "top.rv", "top.u1.r1", "top.u1.c1", "top.u1.r2",
+                         "top.u2.r1", "top.u2.c1", "top.u2.r2"}
+    # ports map to the caller's nodes, globals and the ground keep their names
+    assert flat["top.u1.r1"][0] == ["in", "top.u1.mid"]
+    assert flat["top.u1.r2"][0] == ["top.u1.mid", "top.m"]
+    assert flat["top.u2.r1"][0] == ["top.m", "top.u2.mid"]
+    assert flat["top.u2.r2"][0] == ["top.u2.mid", "out"]
+    assert flat["top.u1.c1"][0] == ["top.u1.mid", "gnd"]
+    assert flat["top.rv"][0] == ["top.m", "vdd"]
+    # instance parameters evaluated in the caller's environment override defaults:
+    # u1 gets scale = s = 5 (top's instance value), u2 keeps scale = 1
+    assert flat["top.u1.r1"][2]["r"] == pytest.approx(2e3 * 5)
+    assert flat["top.u2.r1"][2]["r"] == pytest.approx(1e3)
+    assert flat["top.u1.c1"][2]["c"] == pytest.approx(2e-15)
+    assert flat["vin"][2]["type"] == "pulse" and flat["vin"][2]["val1"] == 1.2
+
+
+def test_netlist_mapping_and_options():
+    net = V.to_netlist(V.parse_netlist("", HIER_SIM))
+    assert net.ground == "gnd"
+    assert net.groups["resistor"].count == 6 and net.groups["capacitor"].count == 2
+    assert net.vsources.names == ["vs", "vin"]
+    assert list(net.vsources.wave) == [0, 1]
+    assert net.vsources.params["dc"][0] == 1.2 and net.vsources.params["width"][1] == 1e-9
+    assert net.options["tran_method"] == "gear2" and net.options["reltol"] == 1e-4
+    assert net.options["icmode"] == "op"
+    assert net.analysis == {"step": 1e-12, "stop": 2e-9}
+
+
+def _rc_ladder_sim(sections, seed):
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(1e3, 1e4, sections)
+    c = rng.uniform(1e-13, 1e-12, sections)
+    lines = ["RC ladder", "model r resistor", "model c capacitor", "model v vsource",
+             "subckt sec(a b)", "  parameters rs=1k cs=1p", "  r1 (a b) r r=rs",
+             "  c1 (b 0) c c=cs", "ends",
+             'v1 (in 0) v type="pulse" val0=0 val1=1 rise=10p fall=10p width=1n delay=0 period=1']
+    prev = "in"
+    for i in range(sections):
+        node = f"n{i + 1:03d}"
+        lines.append(f"x{i + 1} ({prev} {node}) sec rs={float(r[i])!r} cs={float(c[i])!r}")
+        prev = node
+    lines += ["control", '  options tran_method="gear2"',
+              '  analysis tran1 tran step=1p stop=10n icmode="op"', "endc", ""]
+    return "\n".join(lines)
+
+
+def test_rc_ladder_netlist_matches_the_generator():
+    want = G.rc_ladder(40, seed=3)
+    got = V.load("", _rc_ladder_sim(40, 3))
+    a, b = omna.setup(want), omna.setup(got)
+    assert list(a.indptr) == list(b.indptr) and list(a.indices) == list(b.indices)
+    # same node numbering, same element values
+    assert {k: v for k, v in a.node_names.items()} == {k: v for k, v in b.node_names.items()}
+    assert np.array_equal(np.sort(want.groups["resistor"].params["r"]),
+                          np.sort(got.groups["resistor"].params["r"]))
+    ra = otr.run_fixed(a, 30e-12, 1e-12, "gear2", otr.Options(), tran_fs=1.0)
+    rb = otr.run_fixed(b, 30e-12, 1e-12, "gear2", otr.Options(), tran_fs=1.0)
+    assert np.array_equal(ra.V, rb.V)
+
+
+def test_includes_and_refusals(tmp_path):
+    (tmp_path / "models.inc").write_text("model rr resistor\nparameters rv=3k\n")
+    (tmp_path / "top.sim").write_text('Top\ninclude "models.inc"\nmodel v vsource\n'
+                                      "v1 (a 0) v dc=1\nr1 (a 0) rr r=rv\n")
+    net = V.load(tmp_path / "top.sim")
+    assert net.groups["resistor"].params["r"][0] == 3e3
+    with pytest.raises(V.NetlistSyntaxError):
+        V.parse_netlist("", 'Gen\ngenerate "x.py" <<<FILE\nprint(1)\n>>>FILE\n')
+    with pytest.raises(NotImplementedError):
+        V.load("", "T\nmodel q hicum\nq1 (a b c) q\n")
