@@ -118,6 +118,8 @@ struct DevLU {
   bool analyzed = false;
   int factored_prec = -1;  // precision of the current device factors
   DBuf<int32_t> rowptr, col, diag, src, tgt_pos, tgt_seg, upd_l, upd_u, upd_d, l_slot, l_diag, p, q;
+  DBuf<int32_t> task_scale, task_scale_ptr;  // band programs: L scaled at each task's end
+  DBuf<uint32_t> lds_scale;                  // ... by local LDS index in LDS tasks
   DevSchedule fact, fwd, bwd;
   DevBand fwd_band, bwd_band, fact_band;
   DBuf<int32_t> lds_item, lds_task, lds_slots;  // LDS form of the LU band tasks

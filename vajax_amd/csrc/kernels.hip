@@ -724,33 +724,60 @@ struct LuProg {
   const int32_t* ud;
 };
 
-template <typename T>
+template <typename T, bool SCALED = false>
 struct LuTerm {
   const LuProg& P;
   const T* lu;
   __device__ __forceinline__ T operator()(int u) const {
+    // SCALED: L(i,k) already divided by its pivot (the Schur steps of band
+    // programs), otherwise the raw L(i,k) is divided here
+    if constexpr (SCALED) return lu[P.ul[u]] * lu[P.uu[u]];
     return lu[P.ul[u]] * lu[P.uu[u]] / lu[P.ud[u]];
   }
 };
 
+// The end of a band task: divide the L entries of the task's own pivots'
+// columns (all final now) by their pivots, in HBM, and fold max |L| and the
+// count of non-finite entries into the status (the pivot-growth monitor).
+// Every thread of the block calls it; the caller has synchronised the block
+// after its last writes.
 template <typename T>
+__device__ __forceinline__ void task_scale_l(int e0, int e1, const int2* __restrict__ sc, T* lu,
+                                             DevStatus* st) {
+  double mag = 0.0;
+  unsigned int bad = 0;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += (int)blockDim.x) {
+    const int2 q = sc[e];
+    const T v = lu[q.x] / lu[q.y];
+    lu[q.x] = v;
+    const double m = fabs((double)v);
+    if (m <= 1.7976931348623157e308)
+      mag = fmax(mag, m);
+    else
+      ++bad;
+  }
+  if (st) block_flags(dbits(mag), 0, bad, &st->maxl_bits, nullptr, &st->nonfinite);
+}
+
+template <typename T, bool SCALED = false>
 __device__ __forceinline__ void lu_light(int t, const LuProg& P, T* lu) {
   T acc = (T)0;
-  const LuTerm<T> term{P, lu};
+  const LuTerm<T, SCALED> term{P, lu};
   for (int u = P.tseg[t], e = P.tseg[t + 1]; u < e; ++u) acc += term(u);
   lu[P.tpos[t]] -= acc;
 }
 
 // all kGroupLanes lanes of a group call
-template <typename T>
+template <typename T, bool SCALED = false>
 __device__ __forceinline__ void lu_mid(int t, int lane, const LuProg& P, T* lu) {
-  const T acc = group_sum<kGroupLanes, T>(P.tseg[t], P.tseg[t + 1], lane, LuTerm<T>{P, lu});
+  const T acc =
+      group_sum<kGroupLanes, T>(P.tseg[t], P.tseg[t + 1], lane, LuTerm<T, SCALED>{P, lu});
   if (lane == 0) lu[P.tpos[t]] -= acc;
 }
 
-template <typename T>
+template <typename T, bool SCALED = false>
 __device__ __forceinline__ void lu_heavy(int t, const LuProg& P, T* lu, T* red) {
-  const T acc = block_sum<T>(P.tseg[t], P.tseg[t + 1], LuTerm<T>{P, lu}, red);
+  const T acc = block_sum<T>(P.tseg[t], P.tseg[t + 1], LuTerm<T, SCALED>{P, lu}, red);
   if (threadIdx.x == 0) lu[P.tpos[t]] -= acc;
 }
 
@@ -758,22 +785,23 @@ constexpr int kGroupsPerBlock = 256 / kGroupLanes;
 
 // One wide level: items are target ids in level order (identity): nh heavy
 // (a workgroup each), nm medium (kGroupLanes lanes each), then short ones.
-template <typename T>
+template <typename T, bool SCALED = false>
 __global__ void __launch_bounds__(256) k_lu_wide(int lo, int hi, int nh, int nm, LuProg P, T* lu) {
   __shared__ T red[RB + 1];
   const int b = blockIdx.x;
   const int nmb = (nm + kGroupsPerBlock - 1) / kGroupsPerBlock;
   if (b < nh) {
-    lu_heavy<T>(P.items[lo + b], P, lu, red);
+    lu_heavy<T, SCALED>(P.items[lo + b], P, lu, red);
     return;
   }
   if (b < nh + nmb) {
     const int grp = (b - nh) * kGroupsPerBlock + (int)threadIdx.x / kGroupLanes;
-    if (grp < nm) lu_mid<T>(P.items[lo + nh + grp], (int)threadIdx.x % kGroupLanes, P, lu);
+    if (grp < nm)
+      lu_mid<T, SCALED>(P.items[lo + nh + grp], (int)threadIdx.x % kGroupLanes, P, lu);
     return;
   }
   const int g = lo + nh + nm + (b - nh - nmb) * blockDim.x + threadIdx.x;
-  if (g < hi) lu_light<T>(P.items[g], P, lu);
+  if (g < hi) lu_light<T, SCALED>(P.items[g], P, lu);
 }
 
 // A run of narrow levels in one workgroup; level bounds cached in LDS.
@@ -811,8 +839,9 @@ __global__ void __launch_bounds__(512) k_lu_narrow(int l0, int l1, const int32_t
 // owned by the task (rows or columns of its own pivots).  The band's updates
 // of later pivots' rows and columns (the Schur part) follow as one wide launch.
 template <typename T>
-__global__ void __launch_bounds__(256) k_lu_band(int w, int ptr_off, int hm_off, BandProg B,
-                                                 LuProg P, T* lu) {
+__global__ void __launch_bounds__(256) k_lu_band(int w, int ptr_off, int hm_off, int task0,
+                                                 BandProg B, LuProg P, const int32_t* sc_ptr,
+                                                 const int2* sc, T* lu, DevStatus* st) {
   __shared__ T red[RB + 1];
   __shared__ int32_t s_ptr[kMaxBandWidth + 1], s_h[kMaxBandWidth], s_m[kMaxBandWidth];
   const int t = blockIdx.x;
@@ -837,6 +866,7 @@ __global__ void __launch_bounds__(256) k_lu_band(int w, int ptr_off, int hm_off,
     for (int h = 0; h < nh; ++h) lu_heavy<T>(B.items[lo + h], P, lu, red);
     __syncthreads();
   }
+  task_scale_l<T>(sc_ptr[task0 + t], sc_ptr[task0 + t + 1], sc, lu, st);
 }
 
 // The same band task with its slots in LDS: the task's targets and every
@@ -864,7 +894,10 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
                                                      BandProg B, const int4* __restrict__ recs,
                                                      const uint2* __restrict__ upd,
                                                      const int4* __restrict__ tasks,
-                                                     const int32_t* __restrict__ slots, T* lu) {
+                                                     const int32_t* __restrict__ slots,
+                                                     const int32_t* __restrict__ sc_ptr,
+                                                     const uint32_t* __restrict__ lsc, T* lu,
+                                                     DevStatus* st) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* vals = reinterpret_cast<T*>(lds_raw);
   __shared__ T red[RB + 1];
@@ -974,6 +1007,24 @@ __global__ void __launch_bounds__(256) k_lu_band_lds(int w, int ptr_off, int hm_
 #pragma unroll
         for (int q = 0; q < 3; ++q) U[s][k][q] = U[s + 1][k][q];
     }
+  }
+  __syncthreads();
+  {
+    // the task's pivot columns of L, final now: scaled in LDS before write-back
+    double mag = 0.0;
+    unsigned int bad = 0;
+    for (int e = sc_ptr[task0 + t] + (int)threadIdx.x; e < sc_ptr[task0 + t + 1];
+         e += (int)blockDim.x) {
+      const uint32_t q = lsc[e];
+      const T v = vals[q & 0xffffu] / vals[q >> 16];
+      vals[q & 0xffffu] = v;
+      const double m = fabs((double)v);
+      if (m <= 1.7976931348623157e308)
+        mag = fmax(mag, m);
+      else
+        ++bad;
+    }
+    if (st) block_flags(dbits(mag), 0, bad, &st->maxl_bits, nullptr, &st->nonfinite);
   }
   __syncthreads();
   scatter_slots<T>(tk.z, slots + tk.x, vals, lu);
@@ -1559,8 +1610,11 @@ void launch_residual(const ResidualArgs& a, hipStream_t s) {
 }
 
 template <typename T>
-static void run_fact_bands(DevLU& L, T* lu, hipStream_t s) {
+static void run_fact_bands(DevLU& L, T* lu, DevStatus* st, hipStream_t s) {
   const LuSymbolic& S = L.sym;
+  const int2* sc = reinterpret_cast<const int2*>(L.task_scale.p);
+  const int32_t* scp = L.task_scale_ptr.p;
+  const uint32_t* lsc = L.lds_scale.p;
   const BandSchedule& H = S.fact_band;
   const LuProg P{L.fact_band.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
   const BandProg B{L.fact_band.items.p, L.fact_band.tl_ptr.p, L.fact_band.tl_heavy.p,
@@ -1575,15 +1629,15 @@ static void run_fact_bands(DevLU& L, T* lu, hipStream_t s) {
         k_lu_band_lds<T, 4, 3, 8><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
             H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], B,
             reinterpret_cast<const int4*>(L.lds_item.p), reinterpret_cast<const uint2*>(L.lds_upd.p),
-            reinterpret_cast<const int4*>(L.lds_task.p), L.lds_slots.p, lu);
+            reinterpret_cast<const int4*>(L.lds_task.p), L.lds_slots.p, scp, lsc, lu, st);
       else if (nv > 0 && L.use_lds)
         k_lu_band_lds<T, 1, 2, 2><<<ntask, 256, sizeof(T) * (size_t)nv, s>>>(
             H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], H.band_task[b], B,
             reinterpret_cast<const int4*>(L.lds_item.p), reinterpret_cast<const uint2*>(L.lds_upd.p),
-            reinterpret_cast<const int4*>(L.lds_task.p), L.lds_slots.p, lu);
+            reinterpret_cast<const int4*>(L.lds_task.p), L.lds_slots.p, scp, lsc, lu, st);
       else
-        k_lu_band<T><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b], B,
-                                           P, lu);
+        k_lu_band<T><<<ntask, 256, 0, s>>>(H.band_w[b], H.band_ptr_off[b], H.band_hm_off[b],
+                                           H.band_task[b], B, P, scp, sc, lu, st);
       ktimer_end(KC_LU_TASKS, s);
     }
     const int lo = S.fact_schur_lo[b], hi = S.fact_schur_hi[b];
@@ -1591,16 +1645,16 @@ static void run_fact_bands(DevLU& L, T* lu, hipStream_t s) {
       const int nh = S.fact_schur_heavy[b], nm = S.fact_schur_mid[b];
       const int grid = nh + nblk(nm, kGroupsPerBlock) + nblk(hi - lo - nh - nm, 256);
       ktimer_begin(s);
-      k_lu_wide<T><<<grid, 256, 0, s>>>(lo, hi, nh, nm, P, lu);
+      k_lu_wide<T, true><<<grid, 256, 0, s>>>(lo, hi, nh, nm, P, lu);
       ktimer_end(KC_LU_SCHUR, s);
     }
   }
 }
 
 template <typename T>
-static void run_fact_groups(DevLU& L, T* lu, hipStream_t s) {
+static void run_fact_groups(DevLU& L, T* lu, DevStatus* st, hipStream_t s) {
   if (L.fact_band.on) {
-    run_fact_bands<T>(L, lu, s);
+    run_fact_bands<T>(L, lu, st, s);
     return;
   }
   const LuProg P{L.fact.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
@@ -1627,8 +1681,9 @@ void launch_lu_refactor(DevLU& L, const double* csr, T* lu, DevStatus* st, hipSt
   ktimer_begin(s);
   k_lu_gather<T><<<nblk(nnz, 256), 256, 0, s>>>(nnz, L.src.p, csr, lu);
   ktimer_end(KC_LU_GATHER, s);
-  run_fact_groups<T>(L, lu, s);
-  const int nl = (int)L.l_slot.n;
+  run_fact_groups<T>(L, lu, st, s);
+  // band programs scaled L inside their tasks
+  const int nl = L.fact_band.on ? 0 : (int)L.l_slot.n;
   if (nl > 0) {
     ktimer_begin(s);
     k_lu_scale<T><<<std::min(nblk(nl, 256), 2048), 256, 0, s>>>(nl, L.l_slot.p, L.l_diag.p, lu, st);

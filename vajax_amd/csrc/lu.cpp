@@ -169,6 +169,9 @@ int pattern_upload(vjx_pattern* pat) {
   VJX_TRY_HIP(L.upd_d.upload(s.upd_d));
   VJX_TRY_HIP(L.l_slot.upload(s.l_slot));
   VJX_TRY_HIP(L.l_diag.upload(s.l_diag));
+  VJX_TRY_HIP(L.task_scale.upload(s.task_scale));
+  VJX_TRY_HIP(L.task_scale_ptr.upload(s.task_scale_ptr));
+  VJX_TRY_HIP(L.lds_scale.upload(s.lds_scale));
   VJX_TRY_HIP(L.p.upload(s.p));
   VJX_TRY_HIP(L.q.upload(s.q));
   VJX_TRY_HIP(upload_sched(L.fact, s.fact));

@@ -729,6 +729,22 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
           }
         }
       }
+      // each task's pivot columns of L, scaled at the task's end
+      {
+        std::vector<std::vector<int32_t>> per(ntask);
+        for (int k : piv) {
+          auto& v = per[task_of[comp_id[root(k)]]];
+          for (int d = first_l(k); d < ccnt[k + 1]; ++d) {
+            v.push_back(cslot[d]);
+            v.push_back(s.diag[k]);
+          }
+        }
+        if (s.task_scale_ptr.empty()) s.task_scale_ptr.push_back(0);
+        for (const auto& v : per) {
+          s.task_scale.insert(s.task_scale.end(), v.begin(), v.end());
+          s.task_scale_ptr.push_back((int32_t)(s.task_scale.size() / 2));
+        }
+      }
       for (int k : piv) comp_id[root(k)] = -1;
       // Pivots were visited in ascending order, so each target's updates are
       // already ordered by pivot; stable counting sorts group them by target
@@ -1240,6 +1256,47 @@ bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
         }
       }
   }
+  // L scaled at the ends of the tasks: every L slot exactly once, by the task
+  // that last wrote it in that band (or after all its writes), with its pivot
+  // final; task updates read raw L of their own band's pivots, Schur updates
+  // read L scaled in their band; the LDS encoding maps back to the slots.
+  std::vector<int32_t> scaled_band(nnz, -1);
+  for (size_t b = 0; b + 1 < B.band_task.size(); ++b)
+    for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t)
+      for (int e = s.task_scale_ptr[t]; e < s.task_scale_ptr[t + 1]; ++e) {
+        const int l = s.task_scale[2 * (size_t)e], d = s.task_scale[2 * (size_t)e + 1];
+        const int64_t end = stage((int64_t)b, 1, 0);
+        bool ok = scaled_band[l] < 0 && last_write[l] < end && last_write[d] < end &&
+                  !(task_band_write[l] == end / 128 - 1 && task_writer[l] != t) &&
+                  !(task_band_write[d] == end / 128 - 1 && task_writer[d] != t);
+        if (ok && s.band_lds[b] > 0) {
+          const int32_t* tk = &s.lds_task[(size_t)t * 4];
+          const int32_t* sl = &s.lds_slots[tk[0]];
+          const uint32_t q = s.lds_scale[e];
+          ok = (int)(q & 0xffffu) < tk[2] && (int)(q >> 16) < tk[1] && sl[q & 0xffffu] == l &&
+               sl[q >> 16] == d;
+        }
+        if (!ok) {
+          err = "refactorisation: L scaling entry " + std::to_string(e) + " misplaced";
+          return false;
+        }
+        scaled_band[l] = (int32_t)b;
+      }
+  for (int i = 0; i < s.n; ++i)
+    for (int e = s.rowptr[i]; e < s.diag[i]; ++e)
+      if (scaled_band[e] < 0) {
+        err = "refactorisation: L slot " + std::to_string(e) + " never scaled";
+        return false;
+      }
+  for (size_t k = 0; k < items.size(); ++k) {
+    const int t = items[k].first;
+    const int64_t b = items[k].second / 256;
+    for (int u = s.tgt_seg[t]; u < s.tgt_seg[t + 1]; ++u)
+      if (scaled_band[s.upd_l[u]] != b) {
+        err = "refactorisation: an update reads L scaled in another band";
+        return false;
+      }
+  }
   return true;
 }
 
@@ -1455,6 +1512,7 @@ static void build_fact_lds(LuSymbolic& s) {
   s.lds_task.clear();
   s.lds_slots.clear();
   s.band_lds.clear();
+  s.lds_scale.assign(s.task_scale.size() / 2, 0u);
   std::vector<int32_t> local(s.col.size(), -1), slots;
   for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
     const int w = B.band_w[b];
@@ -1463,14 +1521,18 @@ static void build_fact_lds(LuSymbolic& s) {
       const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
       // written slots first, then the operands only read
       slots.clear();
-      for (int g = tp[0]; g < tp[w]; ++g) {
-        const int pos = s.tgt_pos[B.items[g]];
-        if (local[pos] < 0) {
-          local[pos] = (int32_t)slots.size();
-          slots.push_back(pos);
+      auto add = [&](int32_t sl) {
+        if (local[sl] < 0) {
+          local[sl] = (int32_t)slots.size();
+          slots.push_back(sl);
         }
-      }
+      };
+      for (int g = tp[0]; g < tp[w]; ++g) add(s.tgt_pos[B.items[g]]);
+      // the task's pivot columns of L are scaled in LDS and written back
+      const int e0 = s.task_scale_ptr[t], e1 = s.task_scale_ptr[t + 1];
+      for (int e = e0; e < e1; ++e) add(s.task_scale[2 * (size_t)e]);
       const int nwrite = (int)slots.size();
+      for (int e = e0; e < e1; ++e) add(s.task_scale[2 * (size_t)e + 1]);
       for (int g = tp[0]; g < tp[w]; ++g) {
         const int tg = B.items[g];
         for (int u = s.tgt_seg[tg]; u < s.tgt_seg[tg + 1]; ++u)
@@ -1482,6 +1544,10 @@ static void build_fact_lds(LuSymbolic& s) {
       }
       const int cnt = (int)slots.size();
       const bool fits = cnt <= kLdsSlots;
+      if (fits)
+        for (int e = e0; e < e1; ++e)
+          s.lds_scale[e] = (uint32_t)local[s.task_scale[2 * (size_t)e]] |
+                           ((uint32_t)local[s.task_scale[2 * (size_t)e + 1]] << 16);
       need = (need < 0 || !fits) ? -1 : std::max(need, cnt);
       s.lds_task.insert(s.lds_task.end(), {(int32_t)s.lds_slots.size(), cnt, nwrite, 0});
       s.lds_slots.insert(s.lds_slots.end(), slots.begin(), slots.end());
@@ -1519,9 +1585,11 @@ static void build_fact_lds(LuSymbolic& s) {
 // tasks in one launch (a narrower block band is padded with empty levels),
 // then every block's band-b Schur items, class by class.
 static void merge_fact_bands(const std::vector<std::pair<const LuSymbolic*, int32_t>>& parts,
-                             LuSymbolic& s) {
+                             const std::vector<int32_t>& slot_off, LuSymbolic& s) {
   BandSchedule& M = s.fact_band;
   M = BandSchedule();
+  s.task_scale.clear();
+  s.task_scale_ptr.assign(1, 0);
   s.fact_schur_lo.clear();
   s.fact_schur_hi.clear();
   s.fact_schur_heavy.clear();
@@ -1542,11 +1610,18 @@ static void merge_fact_bands(const std::vector<std::pair<const LuSymbolic*, int3
     M.band_ptr_off.push_back((int32_t)M.tl_ptr.size());
     M.band_hm_off.push_back((int32_t)M.tl_heavy.size());
     int ntask = 0;
-    for (const auto& p : parts) {
+    for (size_t pi = 0; pi < parts.size(); ++pi) {
+      const auto& p = parts[pi];
       const BandSchedule& B = p.first->fact_band;
       if (b >= B.band_w.size()) continue;
       const int bw = B.band_w[b];
       for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t, ++ntask) {
+        const LuSymbolic& P = *p.first;
+        for (int e = P.task_scale_ptr[t]; e < P.task_scale_ptr[t + 1]; ++e) {
+          s.task_scale.push_back(P.task_scale[2 * (size_t)e] + slot_off[pi]);
+          s.task_scale.push_back(P.task_scale[2 * (size_t)e + 1] + slot_off[pi]);
+        }
+        s.task_scale_ptr.push_back((int32_t)(s.task_scale.size() / 2));
         const size_t tl = (size_t)(t - B.band_task[b]);
         const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + tl * (bw + 1)];
         for (int j = 0; j < w; ++j) {
@@ -1669,6 +1744,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
   s.min_pivot_ratio = 1.0;
   std::vector<std::vector<int32_t>> bf[3];
   std::vector<std::pair<const LuSymbolic*, int32_t>> fact_parts;
+  std::vector<int32_t> fact_soff;
   int32_t roff = 0, soff = 0, toff = 0, uoff = 0;
   std::vector<int32_t> gmap;
   for (int c = 0; c < nc; ++c) {
@@ -1707,6 +1783,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     }
     bucket_schedule(b.fact, toff, bf);
     fact_parts.push_back({&b, toff});
+    fact_soff.push_back(soff);
     s.nnz_l += b.nnz_l;
     s.nnz_u += b.nnz_u;
     s.n_updates += b.n_updates;
@@ -1727,7 +1804,7 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     s.pinv[s.p[k]] = k;
   }
   merge_schedule(bf, s.fact);
-  merge_fact_bands(fact_parts, s);
+  merge_fact_bands(fact_parts, fact_soff, s);
   build_fact_lds(s);
   for (const auto& b : sym)
     for (int k = 0; k < 4; ++k) s.analyze_ms[k] += b.analyze_ms[k];
@@ -1794,19 +1871,29 @@ void lu_refactor_host(const LuSymbolic& s, const double* a, T* lu) {
   for (int64_t e = 0; e < nnz; ++e) lu[e] = s.src[e] >= 0 ? (T)a[s.src[e]] : (T)0;
   const int nl = (int)s.fact.lvl_ptr.size() - 1;
   auto term = [&](int u) { return lu[s.upd_l[u]] * lu[s.upd_u[u]] / lu[s.upd_d[u]]; };
+  // Schur updates read L already scaled by its pivot
+  auto term_scaled = [&](int u) { return lu[s.upd_l[u]] * lu[s.upd_u[u]]; };
   auto apply = [&](int t) {
     const int a = s.tgt_seg[t], e = s.tgt_seg[t + 1];
     lu[s.tgt_pos[t]] -= class_sum<T>(reduction_class(e - a), a, e, term);
   };
+  auto apply_scaled = [&](int t) {
+    const int a = s.tgt_seg[t], e = s.tgt_seg[t + 1];
+    lu[s.tgt_pos[t]] -= class_sum<T>(reduction_class(e - a), a, e, term_scaled);
+  };
   const BandSchedule& B = s.fact_band;
+  const bool bands = B.band_task.size() > 1;
   for (size_t b = 0; b + 1 < B.band_task.size(); ++b) {
-    // tasks are independent within a band: task by task, level by level
+    // tasks are independent within a band: task by task, level by level,
+    // each ending with the scaling of its pivots' L columns
     const int w = B.band_w[b];
     for (int t = B.band_task[b]; t < B.band_task[b + 1]; ++t) {
       const int32_t* tp = &B.tl_ptr[B.band_ptr_off[b] + (size_t)(t - B.band_task[b]) * (w + 1)];
       for (int g = tp[0]; g < tp[w]; ++g) apply(B.items[g]);
+      for (int e = s.task_scale_ptr[t]; e < s.task_scale_ptr[t + 1]; ++e)
+        lu[s.task_scale[2 * (size_t)e]] /= lu[s.task_scale[2 * (size_t)e + 1]];
     }
-    for (int g = s.fact_schur_lo[b]; g < s.fact_schur_hi[b]; ++g) apply(B.items[g]);
+    for (int g = s.fact_schur_lo[b]; g < s.fact_schur_hi[b]; ++g) apply_scaled(B.items[g]);
   }
   for (int l = 0; l < nl; ++l)
     for (int it = s.fact.lvl_ptr[l]; it < s.fact.lvl_ptr[l + 1]; ++it) {
@@ -1814,7 +1901,8 @@ void lu_refactor_host(const LuSymbolic& s, const double* a, T* lu) {
       const T acc = class_sum<T>(item_class(s.fact, l, it), s.tgt_seg[t], s.tgt_seg[t + 1], term);
       lu[s.tgt_pos[t]] -= acc;
     }
-  for (size_t e = 0; e < s.l_slot.size(); ++e) lu[s.l_slot[e]] /= lu[s.l_diag[e]];
+  if (!bands)
+    for (size_t e = 0; e < s.l_slot.size(); ++e) lu[s.l_slot[e]] /= lu[s.l_diag[e]];
 }
 
 template <typename T>

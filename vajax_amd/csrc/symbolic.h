@@ -150,6 +150,17 @@ struct LuSymbolic {
   std::vector<int32_t> band_lds;      // per band: LDS values needed (0: band runs from HBM)
   std::vector<int32_t> l_slot;   // every L slot (i,k) ...
   std::vector<int32_t> l_diag;   // ... and the slot of its pivot U(k,k)
+  // Under band programs L is scaled inside the tasks: after its last level a
+  // task divides the L entries of its own pivots' columns by their pivots
+  // ({L slot, pivot slot} pairs, tasks in band-program order), so the Schur
+  // steps read scaled L (a product per update, no division, no pivot index)
+  // and no separate scaling pass runs.  Every L entry belongs to the column
+  // of exactly one pivot, hence to exactly one task.
+  std::vector<int32_t> task_scale;      // 2 ints per entry
+  std::vector<int32_t> task_scale_ptr;  // per task: entries [ptr[t], ptr[t + 1])
+  // the same pairs as local LDS indices (L | pivot << 16) for tasks of bands
+  // that run from LDS, whose slot sets include them (L entries written back)
+  std::vector<uint32_t> lds_scale;
   LevelSchedule fwd;             // items = rows, L solve
   LevelSchedule bwd;             // items = rows, U solve
   BandSchedule fwd_band;         // band programs of the two solves (empty: level launches)
