@@ -81,16 +81,19 @@ def kernel_bytes(topo, info, precision: str):
     # residual: rf/rq + segments + history vectors + f/Q/mrc writes
     stamp = 8 * n_jc + 4 * (nse + 1) + 12 * nse + 8 * nse + 16 * n_rc + 4 * n + 8 * 8 * n
     nnz_lu = info["nnz_l"] + info["nnz_u"]
-    # LU update programs: the three slot indices of every contribution are
-    # streamed once; per target the item, slot, segment pair and the value's
-    # read-modify-write.  Operand values (L, U, pivot) are gathered many times
-    # but are counted once, by k_lu_gather / k_lu_scale (the minimum traffic).
-    upd = 12
-    tgt = 4 + 4 + 8 + 2 * vb
-    lu_tasks = upd * info["fact_task_updates"] + tgt * info["fact_task_targets"]
-    lu_schur = upd * info["fact_schur_updates"] + tgt * info["fact_schur_targets"]
+    # LU band tasks (run from LDS): each item's 16-byte record and each
+    # update's 8-byte record (packed local L / U indices, pivot index) are
+    # streamed once, and the task's slots -- targets, operands and its
+    # pivots' L columns, scaled in LDS -- are gathered (index + value) and
+    # the written ones stored back: at most the LU's entries once each.
+    # Schur steps: the L and U slots of each update (L already scaled) once,
+    # per target the item, slot, segment pair and the value's
+    # read-modify-write.  Operand values gathered many times are counted once.
+    lu_tasks = (16 * info["fact_task_targets"] + 8 * info["fact_task_updates"]
+                + (4 + 2 * vb) * nnz_lu)
+    lu_schur = 8 * info["fact_schur_updates"] + (4 + 4 + 8 + 2 * vb) * info["fact_schur_targets"]
     lu_gather = (4 + 8 + vb) * nnz_lu
-    lu_scale = (8 + 3 * vb) * info["nnz_l"]
+    lu_scale = 0  # folded into the tasks
     offd = nnz_lu - n            # off-diagonal entries, each read once per solve
     # solves: each off-diagonal entry's column index and value once (the y
     # operands it gathers are counted once per row), per row the item, row
@@ -105,7 +108,7 @@ def kernel_bytes(topo, info, precision: str):
 KERNEL_NAMES = {"eval": "k_eval<MOSFET> (+R/C)", "stamp": "k_chunk_sum/k_reduce_jac/k_chunk_res/k_residual",
                 "lu_gather": "k_lu_gather", "lu_tasks": "k_lu_band_lds (band tasks)",
                 "lu_schur": "k_lu_wide (Schur step)",
-                "lu_scale": "k_lu_scale", "solve_tasks": "k_tri_band_lde (forward + backward)",
+                "lu_scale": "k_lu_scale (level programs only)", "solve_tasks": "k_tri_band_lde (forward + backward)",
                 "solve_hub": "k_tri_hub_part/k_tri_hub_finish"}
 
 
@@ -114,7 +117,7 @@ TRAFFIC_KERNELS = {
     "eval": ["k_eval<2>"], "stamp": ["k_reduce_jac", "k_residual", "k_chunk_sum", "k_chunk_res"],
     "lu_gather": ["k_lu_gather<double>"], "lu_tasks": ["k_lu_band_lds<double, 4, 3, 8>", "k_lu_band_lds<double, 1, 2, 2>",
                  "k_lu_band<double>"],
-    "lu_schur": ["k_lu_wide<double>"], "lu_scale": ["k_lu_scale<double>"],
+    "lu_schur": ["k_lu_wide<double, true>"], "lu_scale": ["k_lu_scale<double>"],
     "solve_tasks": ["k_tri_band_lde<double, true>", "k_tri_band_lde<double, false>",
                     "k_tri_band<double, true>", "k_tri_band<double, false>"]}
 
