@@ -249,6 +249,22 @@ int vjx_build_system(vjx_circuit* c, const double* X_dev, const double* vsrc_dev
                      double* I_vsource_dev, double* max_res_contrib_dev,
                      const double* limit_state_in_dev, double* limit_state_out_dev);
 
+/* Fused eval + stamp of one model group of a circuit (the per-group device
+ * loop of build_system, mna_builder.py:927-1010, through the precomputed
+ * slot map): the devices of group model_index (position in the circuit's
+ * model list; -1 = every group and the current sources, i.e. build_system
+ * with no charge history) are evaluated at X and their Jacobian (resistive +
+ * c0 x reactive), residual, charge and max-contribution entries stamped;
+ * the other groups and the current sources contribute nothing, while the
+ * structural stamps remain (gmin on node diagonals, the voltage sources'
+ * B / B^T entries and branch rows).  limit_state_out receives the group's
+ * entries (others untouched).  csr_data_dev may be NULL. */
+int vjx_eval_stamp(vjx_circuit* c, int32_t model_index, const double* X_dev,
+                   const double* vsrc_dev, const double* isrc_dev, double c0, double gmin,
+                   double* csr_data_dev, double* f_dev, double* Q_dev,
+                   double* max_res_contrib_dev, const double* limit_state_in_dev,
+                   double* limit_state_out_dev);
+
 /* Vectorised device eval for one model group (the openvaf_jax eval_fn
  * contract, openvaf_jax/codegen/function_builder.py:967-996, restricted to
  * these models): branch voltages in, residuals and Jacobians out, row-major

@@ -175,6 +175,8 @@ _PROTOS = {
                                    C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp,
                                    vp, vp, vp, vp]),
     "vjx_model_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "vjx_eval_stamp": (C.c_int, [vp, C.c_int32, vp, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp,
+                                 vp, vp]),
     "vjx_limit_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, C.c_double, C.c_double, vp]),
     "vjx_eval_sources": (C.c_int, [vp, C.c_double, vp, vp]),
     "vjx_transient_run": (C.c_int, [vp, C.POINTER(TranConfig), C.POINTER(TranResult)]),

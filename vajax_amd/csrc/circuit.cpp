@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "circuit.h"
+#include "transient.h"
 
 vjx_circuit::~vjx_circuit() {
   if (h_sc) (void)hipHostFree(h_sc);
@@ -16,7 +17,7 @@ void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, co
                          const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
                          const double* res_floor, double* csr, double* f, double* Q, double* mrc,
                          double* I_vs, DevStatus* st, const double* ls_in, double* ls_out,
-                         hipStream_t s) {
+                         hipStream_t s, int only_model) {
   EvalOut out{csr ? c->jac_contrib.p : nullptr, c->rf.p, c->rq.p};
   if (c->n_lim > 0) {
     out.lr = c->lr.p;
@@ -24,8 +25,19 @@ void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, co
     out.ls_in = ls_in;
     out.ls_out = ls_out ? ls_out : c->ls_scratch.p;
   }
-  for (auto& m : c->models) launch_eval(*m, X, c->d_sc.p, out, s);
-  launch_isrc_contrib(c->k, isrc, c->is_dst.p, c->rf.p, c->rq.p, s);
+  if (only_model >= 0) {
+    // one group alone: every contribution slot starts from zero
+    if (csr) launch_fill(c->n_jac_contrib, 0.0, c->jac_contrib.p, s);
+    launch_fill(c->n_res_contrib, 0.0, c->rf.p, s);
+    launch_fill(c->n_res_contrib, 0.0, c->rq.p, s);
+    if (c->n_lim > 0) {
+      launch_fill(c->n_res_contrib, 0.0, c->lr.p, s);
+      launch_fill(c->n_res_contrib, 0.0, c->lq.p, s);
+    }
+  }
+  for (size_t g = 0; g < c->models.size(); ++g)
+    if (only_model < 0 || (size_t)only_model == g) launch_eval(*c->models[g], X, c->d_sc.p, out, s);
+  if (only_model < 0) launch_isrc_contrib(c->k, isrc, c->is_dst.p, c->rf.p, c->rq.p, s);
   if (csr)
     launch_reduce_jac(c->nse, c->jac_seg.p, c->jac_contrib.p, c->slot_diag.p, c->slot_b.p,
                       c->d_sc.p, csr, c->hub_slots.view(), s);
@@ -267,6 +279,40 @@ int vjx_build_system(vjx_circuit* c, const double* X, const double* vsrc, const 
   VJX_TRY_HIP(hipMemcpyAsync(c->d_sc.p, h, sizeof(DevScalars), hipMemcpyHostToDevice, s));
   circuit_build_async(c, X, vsrc, isrc, Q_prev, dQdt_prev, Q_prev2, nullptr, csr, f, Q, mrc, I_vs,
                       nullptr, ls_in, ls_out, s);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipStreamSynchronize(s));
+  return VJX_OK;
+}
+
+int vjx_eval_stamp(vjx_circuit* c, int32_t model_index, const double* X, const double* vsrc,
+                   const double* isrc, double c0, double gmin, double* csr, double* f, double* Q,
+                   double* mrc, const double* ls_in, double* ls_out) {
+  if (!c || !X || !f || !Q || !mrc || (c->m && !vsrc) || (c->k && model_index < 0 && !isrc)) {
+    set_error("vjx_eval_stamp: null argument");
+    return VJX_ERR_ARG;
+  }
+  if (model_index < -1 || model_index >= (int32_t)c->models.size()) {
+    set_error("vjx_eval_stamp: model_index out of range");
+    return VJX_ERR_ARG;
+  }
+  hipStream_t s = c->pat->stream;
+  DevScalars* h = c->h_sc;
+  std::memset(h, 0, sizeof(DevScalars));
+  h->c0 = c0;
+  h->gmin = gmin;
+  h->gmin_dev = gmin;
+  h->reltol = 1e-3;
+  h->vntol = 1e-6;
+  h->abstol = 1e-12;
+  h->nr_damping = 1.0;
+  h->max_step = 1e30;
+  VJX_TRY_HIP(hipMemcpyAsync(c->d_sc.p, h, sizeof(DevScalars), hipMemcpyHostToDevice, s));
+  DBuf<double> zeros, ivs;
+  VJX_TRY_HIP(zeros.alloc(std::max(c->n_unk, 1)));
+  VJX_TRY_HIP(ivs.alloc(std::max(c->m, 1)));
+  launch_fill(c->n_unk, 0.0, zeros.p, s);
+  circuit_build_async(c, X, vsrc, isrc, zeros.p, zeros.p, zeros.p, nullptr, csr, f, Q, mrc, ivs.p,
+                      nullptr, ls_in, ls_out, s, model_index);
   VJX_TRY_HIP(hipGetLastError());
   VJX_TRY_HIP(hipStreamSynchronize(s));
   return VJX_OK;

@@ -81,10 +81,12 @@ void launch_status_reset(vjx::DevStatus* st, hipStream_t s);
 
 // Enqueue one system build: eval, stamping reductions, residual.
 // ls_in (NULL: zeros) / ls_out: flat limit states of the limited devices.
+// only_model >= 0 evaluates that model group alone (the others and the
+// current sources contribute nothing; the structural stamps remain).
 void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
                          const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,
                          const double* res_floor, double* csr, double* f, double* Q, double* mrc,
                          double* I_vs, vjx::DevStatus* st, const double* ls_in, double* ls_out,
-                         hipStream_t s);
+                         hipStream_t s, int only_model = -1);
 
 }  // namespace vjx

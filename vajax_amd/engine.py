@@ -290,6 +290,37 @@ class DeviceCircuit:
                 "vjx_build_system")
         return out
 
+    def eval_stamp(self, model, X, vsrc, isrc=None, c0=0.0, gmin=1e-12, limit_state=None,
+                   want_jacobian=True):
+        """One model group's fused eval + stamp (vjx_eval_stamp): ``model`` is a
+        group name of the topology or its index, or None for every group.
+        Returns a dict of CUDA tensors (csr_data, f, Q, max_res_contrib,
+        limit_state_out)."""
+        import torch
+
+        t = self.topo
+        names = list(t.models)
+        idx = -1 if model is None else (names.index(model) if isinstance(model, str) else int(model))
+        dev = X.device
+        nl = self.n_limit_states
+        out = {
+            "csr_data": torch.empty(t.nse, dtype=torch.float64, device=dev) if want_jacobian else None,
+            "f": torch.empty(t.n_augmented, dtype=torch.float64, device=dev),
+            "Q": torch.empty(t.n_unknowns, dtype=torch.float64, device=dev),
+            "max_res_contrib": torch.empty(t.n_unknowns, dtype=torch.float64, device=dev),
+            "limit_state_out": torch.zeros(nl, dtype=torch.float64, device=dev) if nl else None,
+        }
+
+        def p(x):
+            return C.c_void_p(x.data_ptr()) if x is not None and x.numel() else None
+
+        torch.cuda.synchronize()
+        L.check(L.lib.vjx_eval_stamp(self.h, idx, p(X), p(vsrc), p(isrc), float(c0), float(gmin),
+                                     p(out["csr_data"]), p(out["f"]), p(out["Q"]),
+                                     p(out["max_res_contrib"]), p(limit_state),
+                                     p(out["limit_state_out"])), "vjx_eval_stamp")
+        return out
+
     def nr_solve(self, X_init, vsource_vals, isource_vals, Q_prev, integ_c0, gmin=1e-12,
                  gshunt=0.0, integ_c1=0.0, integ_d1=0.0, dQdt_prev=None, integ_c2=0.0,
                  Q_prev2=None, limit_state_in=None, res_tol_floor=None,
