@@ -265,7 +265,7 @@ int pattern_autotune_solves(vjx_pattern* pat, const double* data_dev) {
     else
       launch_lu_solve<double>(L, L.lu64.p, L.r64.p, 1.0, L.x64.p, L.y64.p, L.z64.p, s);
   };
-  const int candidates[] = {8, 12, 16, 24, 32, 48};
+  const int candidates[] = {8, 12, 16, 24, 32, 48, 64};
   int best = 0;
   float best_ms = 0.f;
   rc = VJX_OK;

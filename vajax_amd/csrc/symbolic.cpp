@@ -905,7 +905,7 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
 // LU arrays; deps(i, f) calls f(j) for every row j that row i reads.
 template <typename Seg, typename Deps>
 static void build_bands(const std::vector<int32_t>& level, Seg seg, Deps deps, int width,
-                        int max_tasks, BandSchedule& B) {
+                        int max_tasks, BandSchedule& B, const std::vector<char>* cut = nullptr) {
   B = BandSchedule();
   const int n = (int)level.size();
   if (n == 0 || width <= 0) return;
@@ -942,7 +942,7 @@ static void build_bands(const std::vector<int32_t>& level, Seg seg, Deps deps, i
   for (int b0 = 0; b0 < nl;) {
     int b1 = std::min(nl, b0 + width);
     for (int l = b0; l < b1; ++l)
-      if (has_hub[l]) {
+      if (has_hub[l] || (cut && l < (int)cut->size() && (*cut)[l])) {
         b1 = l + 1;
         break;
       }
@@ -1382,22 +1382,47 @@ static void solve_schedules(LuSymbolic& s, const AnalyzeOptions& opt) {
   }
   rows_by_level(lf, [&](int i) { return s.diag[i] - s.rowptr[i]; }, s.fwd);
   rows_by_level(lb, [&](int i) { return s.rowptr[i + 1] - s.diag[i] - 1; }, s.bwd);
-  build_bands(
-      lf, [&](int i) { return std::make_pair((int)s.rowptr[i], (int)s.diag[i]); },
-      [&](int i, auto f) {
-        for (int e = s.rowptr[i]; e < s.diag[i]; ++e) f(s.col[e]);
-      },
-      sw, opt.max_band_tasks, s.fwd_band);
-  build_bands(
-      lb, [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); },
-      [&](int i, auto f) {
-        for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) f(s.col[e]);
-      },
-      sw, opt.max_band_tasks, s.bwd_band);
-  build_tri_lds(s, [&](int i) { return std::make_pair((int)s.rowptr[i], (int)s.diag[i]); },
-                s.fwd_band);
-  build_tri_lds(s, [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); },
-                s.bwd_band);
+  auto seg_f = [&](int i) { return std::make_pair((int)s.rowptr[i], (int)s.diag[i]); };
+  auto seg_b = [&](int i) { return std::make_pair((int)s.diag[i] + 1, (int)s.rowptr[i + 1]); };
+  auto deps_f = [&](int i, auto f) {
+    for (int e = s.rowptr[i]; e < s.diag[i]; ++e) f(s.col[e]);
+  };
+  auto deps_b = [&](int i, auto f) {
+    for (int e = s.diag[i] + 1; e < s.rowptr[i + 1]; ++e) f(s.col[e]);
+  };
+  // A band whose tasks do not fit a workgroup's LDS (FP64 values, entries,
+  // records; kSolveLdsBytes) runs from HBM, several times slower: such a
+  // band is split at its middle level and the program rebuilt, a few rounds.
+  auto lde_fits = [&](const BandSchedule& B, size_t b, bool fwd) {
+    if (B.band_lds.empty() || B.band_lds[b] <= 0) return false;
+    const size_t nv = (size_t)B.band_lds[b], ne = (size_t)B.band_ent[b],
+                 nr = (size_t)B.band_rows[b];
+    const size_t rec = ((nv + ne) * 8 + 15) & ~(size_t)15;
+    return rec + nr * 16 + (fwd ? 0 : 2 * nr * 8) + 2 * ne <= (size_t)kSolveLdsBytes;
+  };
+  auto build_split = [&](const std::vector<int32_t>& lv, auto seg, auto deps, bool fwd,
+                         BandSchedule& B) {
+    int nl = 0;
+    for (int v : lv) nl = std::max(nl, v + 1);
+    std::vector<char> cut(nl, 0);
+    for (int round = 0; round < 6; ++round) {
+      build_bands(lv, seg, deps, sw, opt.max_band_tasks, B, &cut);
+      build_tri_lds(s, seg, B);
+      bool changed = false;
+      int l0 = 0;
+      for (size_t b = 0; b < B.band_w.size(); ++b) {
+        const int w = B.band_w[b];
+        if (w > 1 && B.band_task[b + 1] > B.band_task[b] && !lde_fits(B, b, fwd)) {
+          cut[l0 + w / 2 - 1] = 1;
+          changed = true;
+        }
+        l0 += w;
+      }
+      if (!changed) break;
+    }
+  };
+  build_split(lf, seg_f, deps_f, true, s.fwd_band);
+  build_split(lb, seg_b, deps_b, false, s.bwd_band);
 }
 
 void lu_rebuild_solve_programs(LuSymbolic& s, const AnalyzeOptions& opt) {

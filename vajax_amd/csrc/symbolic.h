@@ -77,6 +77,9 @@ constexpr int kLdsSlots = 7680;
 // Static LDS of the entries-in-LDS solve kernel (reduction tree and level
 // tables), kept out of the dynamic budget with margin.
 constexpr int kTriStaticLds = 4096;
+// LDS a solve band task may use for its values, entries and records: the
+// 160 KiB per workgroup of gfx950 less the kernel's static arrays
+constexpr int kSolveLdsBytes = 160 * 1024 - kTriStaticLds;
 
 // Band program of a triangular solve.  The levels are cut into bands of at
 // most `width` levels; inside a band the rows form connected components
