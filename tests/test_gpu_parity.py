@@ -390,7 +390,7 @@ def test_autotuned_solves_give_the_same_waveforms():
     eng_a, a = run_gpu(net, 0.05e-9, 1e-12, autotune=True, use_graph=True)
     eng_b, b = run_gpu(net, 0.05e-9, 1e-12, autotune=False, use_graph=True)
     ia = eng_a.device.lu.info()
-    assert ia["tuned_solve_ms"] > 0 and ia["solve_band_width"] in (8, 12, 16, 24, 32, 48)
+    assert ia["tuned_solve_ms"] > 0 and ia["solve_band_width"] in (8, 12, 16, 24, 32, 48, 64)
     assert np.array_equal(a.times, b.times)
     assert np.array_equal(a.stats["V_out"], b.stats["V_out"])
 
