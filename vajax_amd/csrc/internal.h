@@ -119,6 +119,7 @@ struct DevLU {
   int factored_prec = -1;  // precision of the current device factors
   DBuf<int32_t> rowptr, col, diag, src, tgt_pos, tgt_seg, upd_l, upd_u, upd_d, l_slot, l_diag, p, q;
   DBuf<int32_t> task_scale, task_scale_ptr;  // band programs: L scaled at each task's end
+  DBuf<int32_t> upd_lu;                      // band programs: {upd_l, upd_u} interleaved
   DBuf<uint32_t> lds_scale;                  // ... by local LDS index in LDS tasks
   DevSchedule fact, fwd, bwd;
   DevBand fwd_band, bwd_band, fact_band;

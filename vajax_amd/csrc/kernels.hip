@@ -722,6 +722,7 @@ struct LuProg {
   const int32_t* ul;
   const int32_t* uu;
   const int32_t* ud;
+  const int2* lu2;  // {ul, uu} interleaved: one 8-byte load per scaled update
 };
 
 template <typename T, bool SCALED = false>
@@ -731,7 +732,10 @@ struct LuTerm {
   __device__ __forceinline__ T operator()(int u) const {
     // SCALED: L(i,k) already divided by its pivot (the Schur steps of band
     // programs), otherwise the raw L(i,k) is divided here
-    if constexpr (SCALED) return lu[P.ul[u]] * lu[P.uu[u]];
+    if constexpr (SCALED) {
+      const int2 q = P.lu2[u];
+      return lu[q.x] * lu[q.y];
+    }
     return lu[P.ul[u]] * lu[P.uu[u]] / lu[P.ud[u]];
   }
 };
@@ -1616,7 +1620,8 @@ static void run_fact_bands(DevLU& L, T* lu, DevStatus* st, hipStream_t s) {
   const int32_t* scp = L.task_scale_ptr.p;
   const uint32_t* lsc = L.lds_scale.p;
   const BandSchedule& H = S.fact_band;
-  const LuProg P{L.fact_band.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
+  const LuProg P{L.fact_band.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p,
+                 reinterpret_cast<const int2*>(L.upd_lu.p)};
   const BandProg B{L.fact_band.items.p, L.fact_band.tl_ptr.p, L.fact_band.tl_heavy.p,
                    L.fact_band.tl_mid.p};
   for (size_t b = 0; b + 1 < H.band_task.size(); ++b) {
@@ -1657,7 +1662,8 @@ static void run_fact_groups(DevLU& L, T* lu, DevStatus* st, hipStream_t s) {
     run_fact_bands<T>(L, lu, st, s);
     return;
   }
-  const LuProg P{L.fact.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p};
+  const LuProg P{L.fact.items.p, L.tgt_pos.p, L.tgt_seg.p, L.upd_l.p, L.upd_u.p, L.upd_d.p,
+                 nullptr};
   for (const auto& g : L.fact.groups) {
     ktimer_begin(s);
     if (g.wide) {

@@ -172,6 +172,17 @@ int pattern_upload(vjx_pattern* pat) {
   VJX_TRY_HIP(L.task_scale.upload(s.task_scale));
   VJX_TRY_HIP(L.task_scale_ptr.upload(s.task_scale_ptr));
   VJX_TRY_HIP(L.lds_scale.upload(s.lds_scale));
+  if (s.fact_band.band_w.empty()) {
+    L.upd_lu.release();
+  } else {
+    // the Schur steps read both slots of an update with one 8-byte load
+    std::vector<int32_t> lu2(2 * s.upd_l.size());
+    for (size_t u = 0; u < s.upd_l.size(); ++u) {
+      lu2[2 * u] = s.upd_l[u];
+      lu2[2 * u + 1] = s.upd_u[u];
+    }
+    VJX_TRY_HIP(L.upd_lu.upload(lu2));
+  }
   VJX_TRY_HIP(L.p.upload(s.p));
   VJX_TRY_HIP(L.q.upload(s.q));
   VJX_TRY_HIP(upload_sched(L.fact, s.fact));
