@@ -1,0 +1,68 @@
+"""Writes the golden fixtures of this directory: waveforms and Newton
+statistics of small seeded circuits computed by the oracle (the numpy
+restatement, itself pinned by the closed forms and known-answer tests of
+tests/test_oracle_units.py and tests/test_limiting_cpu.py), and the DC point
+the homotopy chain finds on the diode clamp.  The GPU tests compare the HIP
+path with these files directly; tests/test_golden_cpu.py checks that the
+oracle still reproduces them.
+
+    python -m tests.golden.make_golden
+"""
+
+import os
+
+import numpy as np
+
+from oracle import mna as omna
+from vajax_amd import dc as D
+from vajax_amd import generators as G
+from vajax_amd.engine import SimulationOptions
+from vajax_amd.topology import build_topology
+
+from ..helpers import run_oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# name -> (circuit, t_stop, dt, fixed)
+CASES = {
+    "rc_ladder_20_s1_fixed": (lambda: G.rc_ladder(20, seed=1), 50e-12, 1e-12, True),
+    "diode_ladder_20_s2_adaptive": (lambda: G.diode_ladder(20, seed=2), 0.8e-9, 1e-12, False),
+    "gate_dag_40_s3_fixed": (lambda: G.gate_dag(40, seed=3), 20e-12, 1e-12, True),
+}
+
+
+def transient_case(name):
+    make, t_stop, dt, fixed = CASES[name]
+    net = make()
+    _, out = run_oracle(net, t_stop, dt, fixed=fixed, tran_fs=1.0 if fixed else None)
+    return {"times": np.asarray(out.times), "V": np.asarray(out.V),
+            "total_nr_iterations": np.array([out.stats["total_nr_iterations"]])}
+
+
+def dc_clamp_case(nr_solve_factory):
+    net = G.diode_clamp_chain()
+    cir = omna.setup(net)
+    topo = build_topology(net)
+    V0, _ = D.initialize_dc_voltages(topo, net, D.get_vdd_value(net))
+    X0 = np.zeros(cir.n_total + cir.m)
+    X0[:cir.n_total] = V0
+    vs, isv = D.get_dc_source_values(net)
+    X, r = D.compute_dc_operating_point(nr_solve_factory(cir, 12), X0, vs, isv,
+                                        np.zeros(cir.n_unk), SimulationOptions())
+    X = X.cpu().numpy() if hasattr(X, "cpu") else np.asarray(X)
+    return {"X": X[:cir.n_total], "iterations": np.array([r.iterations]),
+            "homotopy_steps": np.array([r.homotopy_steps]),
+            "method": np.frombuffer(r.method.encode(), dtype=np.uint8)}
+
+
+def main():
+    from ..test_homotopy_cpu import oracle_nr_solve
+
+    for name in CASES:
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **transient_case(name))
+    np.savez_compressed(os.path.join(HERE, "dc_clamp_homotopy.npz"),
+                        **dc_clamp_case(oracle_nr_solve))
+
+
+if __name__ == "__main__":
+    main()
