@@ -300,8 +300,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes_each_way": a.numel() * 8, "copies": 5}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                   "note only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "runs on one GPU only: instances and circuits are not "
+                                   "sharded across GPUs", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
