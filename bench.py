@@ -57,6 +57,29 @@ def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None, a
 N_PAR = {"resistor": 1, "capacitor": 1, "mosfet": 11}
 
 
+def mixed_error(eng, steps: int):
+    """Largest node-voltage difference between the mixed (FP32 factor + FP64
+    refinement) and the plain FP64 runs of the same circuit over ``steps``
+    fixed 1 ps BDF2 steps (one time grid, so the waveforms compare point by
+    point)."""
+    import numpy as np
+
+    from vajax_amd.engine import AdaptiveConfig
+
+    V = {}
+    eng.options.fixed_step_transient = True
+    try:
+        for prec in ("f64", "f32"):
+            eng.prepare(t_stop=steps * 1e-12, dt=1e-12, max_steps=steps + 1, precision=prec,
+                        adaptive_config=AdaptiveConfig(integration_method="gear2", tran_fs=1.0),
+                        use_graph=True)
+            V[prec] = eng.run_transient().stats["V_out"]
+    finally:
+        eng.options.fixed_step_transient = False
+    return {"max_abs_dV_vs_fp64": float(np.max(np.abs(V["f32"] - V["f64"]))), "steps": steps,
+            "grid": "fixed 1 ps, gear2"}
+
+
 def kernel_bytes(topo, info, precision: str):
     """Algorithmic HBM bytes per Newton iteration of each kernel class (the
     data each kernel must move at least once: indices, gathered operands,
@@ -340,6 +363,10 @@ def main():
     log(f"timed: {steps_done} steps, {nr} Newton iterations, loop {elapsed:.3f}s, "
         f"call {call_s:.3f}s")
 
+    mixed = mixed_error(eng, max(args.steps, 200)) if args.precision == "f64" else None
+    if mixed:
+        log(f"mixed precision: max |dV| vs FP64 {mixed['max_abs_dV_vs_fp64']:.2e} V")
+
     # eager pass with phase events on the library stream
     L.lib.vjx_profile_enable(1)
     L.lib.vjx_profile_reset()
@@ -426,6 +453,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "mixed_precision": mixed,
     }
     if cpu:
         line["speedup_vs_cpu_ms_per_step"] = round(cpu["ms_per_step"] / ms_per_step, 2)
