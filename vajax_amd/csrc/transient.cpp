@@ -53,7 +53,16 @@ struct Work {
   // Newton iterate, written by the last build, saved for a re-pivot redo
   DBuf<double> ls_step, ls_iter, ls_out, ls_save;
   DBuf<DevStatus> st;
+  // [0] Newton / LTE status, [1] LU status: mapped pinned host memory the
+  // iteration's last kernel writes (d_hst: its device address)
   DevStatus* h_st = nullptr;
+  DevStatus* d_hst = nullptr;
+  hipError_t alloc_status() {
+    hipError_t e = hipHostMalloc((void**)&h_st, 2 * sizeof(DevStatus), hipHostMallocMapped);
+    if (e != hipSuccess) return e;
+    std::memset(h_st, 0, 2 * sizeof(DevStatus));
+    return hipHostGetDevicePointer((void**)&d_hst, h_st, 0);
+  }
   ~Work() {
     if (h_st) (void)hipHostFree(h_st);
   }
@@ -101,7 +110,8 @@ struct Driver {
   }
 
   int read_status() {
-    VJX_TRY_HIP(hipMemcpyAsync(w.h_st, w.st.p, sizeof(DevStatus), hipMemcpyDeviceToHost, s));
+    launch_status_export(w.st.p, nullptr, w.d_hst, s);
+    VJX_TRY_HIP(hipGetLastError());
     VJX_TRY_HIP(hipStreamSynchronize(s));
     return VJX_OK;
   }
@@ -201,6 +211,7 @@ struct Driver {
     launch_delta(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
     launch_update(n_unk, m, n_total, w.delta.p, w.Xw.p, c->d_sc.p, w.st.p, s);
     launch_noi_clamp(c->n_noi, c->noi_rows.p, w.Xw.p, s);
+    launch_status_export(w.st.p, pat->d_status.p, w.d_hst, s);
     return VJX_OK;
   }
 
@@ -262,16 +273,15 @@ struct Driver {
       if (rc < 0) return rc;
       rc = nr_iteration(Qp, dQ, Q2, floor);
       if (rc < 0) return rc;
-      VJX_TRY_HIP(hipMemcpyAsync(pat->h_status, pat->d_status.p, sizeof(DevStatus),
-                                 hipMemcpyDeviceToHost, s));
-      rc = read_status();
-      if (rc < 0) return rc;
+      // the iteration exported both status records as its last kernel
+      VJX_TRY_HIP(hipStreamSynchronize(s));
       collect_phase_times();
       // pivot monitor: re-pivot on this matrix and redo the iteration from
       // the saved iterate
-      const double maxl = bits_to_double(pat->h_status->maxl_bits);
+      const DevStatus& lu_st = w.h_st[1];
+      const double maxl = bits_to_double(lu_st.maxl_bits);
       const double growth = cfg->repivot_growth > 0.0 ? cfg->repivot_growth : 1e12;
-      if ((pat->h_status->nonfinite > 0 || maxl > growth) && repivots < 64) {
+      if ((lu_st.nonfinite > 0 || maxl > growth) && repivots < 64) {
         std::vector<double> h(c->nse);
         VJX_TRY_HIP(hipMemcpy(h.data(), w.csr.p, sizeof(double) * c->nse, hipMemcpyDeviceToHost));
         rc = vjx_lu_analyze(pat, h.data());
@@ -366,7 +376,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   VJX_TRY_HIP(w.vout.alloc((size_t)out_rows * n_total));
   VJX_TRY_HIP(w.iout.alloc((size_t)out_rows * mo));
   VJX_TRY_HIP(w.st.alloc(1));
-  VJX_TRY_HIP(hipHostMalloc((void**)&w.h_st, sizeof(DevStatus), hipHostMallocDefault));
+  VJX_TRY_HIP(w.alloc_status());
   const int nl = c->n_lim;
   D.n_lim = nl;
   if (nl) {
@@ -732,7 +742,7 @@ extern "C" int vjx_newton_solve(vjx_circuit* c, const vjx_tran_config* cfg,
   VJX_TRY_HIP(w.isrc.alloc(std::max(k, 1)));
   VJX_TRY_HIP(w.Ivs.alloc(std::max(m, 1)));
   VJX_TRY_HIP(w.st.alloc(1));
-  VJX_TRY_HIP(hipHostMalloc((void**)&w.h_st, sizeof(DevStatus), hipHostMallocDefault));
+  VJX_TRY_HIP(w.alloc_status());
   const int nl = c->n_lim;
   D.n_lim = nl;
   if (nl) {

@@ -99,6 +99,16 @@ __global__ void k_copy(int n, const double* __restrict__ a, double* b) {
   if (i < n) b[i] = a[i];
 }
 
+// The Newton and LU status records written by one thread straight into
+// mapped pinned host memory at the end of an iteration (no copy launches on
+// the critical path).
+__global__ void k_status_export(const DevStatus* __restrict__ a, const DevStatus* __restrict__ b,
+                                DevStatus* out) {
+  if (threadIdx.x != 0) return;
+  out[0] = *a;
+  if (b) out[1] = *b;
+}
+
 __global__ void k_gather(int n, const int32_t* __restrict__ idx, const double* __restrict__ a,
                          double* b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -218,6 +228,9 @@ void launch_fill(int n, double v, double* x, hipStream_t s) {
 }
 void launch_copy(int n, const double* a, double* b, hipStream_t s) {
   if (n) k_copy<<<nb(n), 256, 0, s>>>(n, a, b);
+}
+void launch_status_export(const DevStatus* a, const DevStatus* b, DevStatus* out, hipStream_t s) {
+  k_status_export<<<1, 64, 0, s>>>(a, b, out);
 }
 void launch_gather(int n, const int32_t* idx, const double* a, double* b, hipStream_t s) {
   if (n) k_gather<<<nb(n), 256, 0, s>>>(n, idx, a, b);
