@@ -305,6 +305,8 @@ def main():
     ap.add_argument("--cpu-numpy", action="store_true",
                     help="also time the numpy oracle (slow) as a secondary CPU figure")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision waveform-error runs (profiling passes)")
     ap.add_argument("--batch", type=int, default=256, help="Monte Carlo instances (0: skip)")
     ap.add_argument("--batch-gates", type=int, default=2000)
     ap.add_argument("--batch-ragged", default="500,1000,2000,4000")
@@ -363,7 +365,8 @@ def main():
     log(f"timed: {steps_done} steps, {nr} Newton iterations, loop {elapsed:.3f}s, "
         f"call {call_s:.3f}s")
 
-    mixed = mixed_error(eng, max(args.steps, 200)) if args.precision == "f64" else None
+    mixed = (mixed_error(eng, max(args.steps, 200))
+             if args.precision == "f64" and not args.no_mixed else None)
     if mixed:
         log(f"mixed precision: max |dV| vs FP64 {mixed['max_abs_dV_vs_fp64']:.2e} V")
 
