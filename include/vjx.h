@@ -172,7 +172,29 @@ int vjx_lu_download(vjx_pattern* pat, double* lu_host);
 #define VJX_MODEL_CAPACITOR 1
 #define VJX_MODEL_MOSFET 2
 #define VJX_MODEL_DIODE 3 /* junction diode with $limit pnjlim: one limit state */
+#define VJX_MODEL_COUNT 4
 #define VJX_MAX_MODELS 8
+
+/* Independent-source waveforms (vajax/analysis/sources.py:25-294).  Each
+ * source has VJX_SRC_NPAR parameters, stored column-major [VJX_SRC_NPAR][m]:
+ *   column 0 the dc value, then by waveform
+ *   pulse: 1 val0, 2 val1, 3 rise, 4 fall, 5 width, 6 period, 7 delay
+ *   sine:  1 sinedc, 2 ampl, 3 freq, 4 phase
+ *   exp:   1 val0, 2 val1, 3 delay, 4 td2, 5 tau1, 6 tau2
+ *   am/fm: 1 sinedc, 2 ampl, 3 freq, 4 phase (deg), 5 delay, 6 modfreq,
+ *          7 modphase (deg), 8 modindex
+ *   pwl:   1 pwlperiod; its points are pwl_t/pwl_v[pwl_ptr[i] .. pwl_ptr[i+1]),
+ *          times already multiplied by stretch and values by scale plus
+ *          offset (sources.py:81-106). */
+#define VJX_WAVE_DC 0
+#define VJX_WAVE_PULSE 1
+#define VJX_WAVE_SINE 2
+#define VJX_WAVE_PWL 3
+#define VJX_WAVE_EXP 4
+#define VJX_WAVE_AM 5
+#define VJX_WAVE_FM 6
+#define VJX_WAVE_COUNT 7
+#define VJX_SRC_NPAR 10
 
 /* One model group.  All arrays are column-major [k][count] (structure of
  * arrays): column k of v_node1 is the first node of branch voltage k, etc. */
@@ -199,10 +221,18 @@ typedef struct {
   const double* slot_b;     /* [nse] +-1 for B/B^T slots else 0 */
   const int32_t* vs_node_p; /* [m] */
   const int32_t* vs_node_n; /* [m] */
-  const int32_t* vs_wave;   /* [m] 0 dc, 1 pulse */
-  const double* vs_params;  /* [8][m] dc, val0, val1, rise, fall, width, period, delay */
+  const int32_t* vs_wave;   /* [m] VJX_WAVE_* */
+  const double* vs_params;  /* [VJX_SRC_NPAR][m] */
   const int32_t* is_wave;   /* [k] */
-  const double* is_params;  /* [8][k] */
+  const double* is_params;  /* [VJX_SRC_NPAR][k] */
+  /* pwl points of the voltage / current sources ([m+1] / [k+1] pointers;
+   * all three NULL when no source of the kind is a pwl) */
+  const int32_t* vs_pwl_ptr;
+  const double* vs_pwl_t;
+  const double* vs_pwl_v;
+  const int32_t* is_pwl_ptr;
+  const double* is_pwl_t;
+  const double* is_pwl_v;
   const int32_t* is_dst;    /* [k][2] residual contribution slots of +I at p, -I at n
                                (interleaved: is_dst[2i], is_dst[2i+1]; -1 = ground) */
   int32_t n_jac_contrib;

@@ -174,6 +174,8 @@ class CTran:
         d.b_val = _p(arr([v for _, v in cir.b_pos] or [0.0], np.float64))
 
         def sources(grp, p, n):
+            if grp.count and np.any(np.asarray(grp.wave) > 1):
+                raise ValueError("the C++ restatement evaluates dc and pulse sources only")
             s = _Sources()
             s.count = grp.count
             s.p = _p(arr(p if grp.count else [0], np.int64), i64p)

@@ -16,6 +16,10 @@ LIB_PATH = Path(os.environ.get("VJX_LIB", Path(__file__).resolve().parent / "lib
 VJX_OK = 0
 VJX_WARN_PIVOT = 1
 VJX_WARN_NOCONV = 2
+VJX_ERR_ARG = -1
+VJX_ERR_HIP = -2
+VJX_ERR_SINGULAR = -3
+VJX_ERR_STATE = -4
 VJX_PREC_F64 = 0
 VJX_PREC_F32 = 1
 VJX_METHOD = {"be": 0, "trap": 1, "gear2": 2}
@@ -66,7 +70,10 @@ class CircuitDesc(C.Structure):
         ("nse", C.c_int32),
         ("indptr", i32p), ("indices", i32p), ("slot_diag", i32p), ("slot_b", f64p),
         ("vs_node_p", i32p), ("vs_node_n", i32p), ("vs_wave", i32p), ("vs_params", f64p),
-        ("is_wave", i32p), ("is_params", f64p), ("is_dst", i32p),
+        ("is_wave", i32p), ("is_params", f64p),
+        ("vs_pwl_ptr", i32p), ("vs_pwl_t", f64p), ("vs_pwl_v", f64p),
+        ("is_pwl_ptr", i32p), ("is_pwl_t", f64p), ("is_pwl_v", f64p),
+        ("is_dst", i32p),
         ("n_jac_contrib", C.c_int32), ("n_res_contrib", C.c_int32),
         ("jac_seg", i32p), ("res_seg", i32p), ("br_ptr", i32p), ("br_ent", i32p),
         ("conv_mask", i32p),

@@ -71,12 +71,15 @@ int vjx_lu_analyze(vjx_pattern* pat, const double* data_host) {
   }
   std::string err;
   auto t0 = std::chrono::steady_clock::now();
-  if (!lu_analyze(pat->n, pat->indptr.data(), pat->indices.data(), data_host, pat->opts,
-                  pat->lu.sym, err)) {
+  // analysed into a fresh record: a failed re-analysis leaves the previous
+  // analysis (and the device factors built from it) in effect
+  LuSymbolic fresh;
+  if (!lu_analyze(pat->n, pat->indptr.data(), pat->indices.data(), data_host, pat->opts, fresh,
+                  err)) {
     set_error("vjx_lu_analyze: " + err);
-    pat->lu.analyzed = false;
     return err.find("singular") != std::string::npos ? VJX_ERR_SINGULAR : VJX_ERR_ARG;
   }
+  pat->lu.sym = std::move(fresh);
   pat->analyze_seconds =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   pat->lu.analyzed = true;

@@ -77,16 +77,119 @@ using namespace vjx;
 
 extern "C" {
 
+namespace {
+
+// A segment pointer array seg[0..n] that the kernels follow: starts at 0,
+// never decreases, ends at total.
+bool valid_segments(const int32_t* seg, int n, int64_t total) {
+  if (!seg || seg[0] != 0 || (int64_t)seg[n] != total) return false;
+  for (int i = 0; i < n; ++i)
+    if (seg[i + 1] < seg[i]) return false;
+  return true;
+}
+
+// Every pointer, size and index of the descriptor that a kernel will follow,
+// checked on the host before anything is uploaded.
+int validate_desc(const vjx_circuit_desc* d) {
+  auto fail = [](const char* what) {
+    set_error(std::string("vjx_circuit_create: ") + what);
+    return VJX_ERR_ARG;
+  };
+  if (d->n_total < 1 || d->n_vsources < 0 || d->n_isources < 0 || d->nse < 0 ||
+      d->n_models < 0 || d->n_models > VJX_MAX_MODELS || d->n_jac_contrib < 0 ||
+      d->n_res_contrib < 0 || d->n_noi < 0)
+    return fail("invalid sizes");
+  const int nu = d->n_total - 1, m = d->n_vsources, k = d->n_isources;
+  if (!d->indptr || (d->nse && !d->indices) || (d->nse && (!d->slot_diag || !d->slot_b)))
+    return fail("null CSR pattern or slot array");
+  if (!valid_segments(d->jac_seg, d->nse, d->n_jac_contrib))
+    return fail("jac_seg must start at 0, not decrease and end at n_jac_contrib");
+  if (!valid_segments(d->res_seg, nu, d->n_res_contrib))
+    return fail("res_seg must start at 0, not decrease and end at n_res_contrib");
+  if (!d->br_ptr || d->br_ptr[0] != 0) return fail("br_ptr must start at 0");
+  for (int i = 0; i < nu; ++i)
+    if (d->br_ptr[i + 1] < d->br_ptr[i]) return fail("br_ptr decreases");
+  if (d->br_ptr[nu] > 2 * m) return fail("br_ptr[n_unknowns] exceeds two entries per source");
+  if (d->br_ptr[nu] > 0 && !d->br_ent) return fail("null br_ent");
+  for (int e = 0; e < d->br_ptr[nu]; ++e) {
+    const int kk = d->br_ent[e] >= 0 ? d->br_ent[e] : ~d->br_ent[e];
+    if (kk >= m) return fail("branch entry out of range");
+  }
+  if (m > 0 && (!d->vs_node_p || !d->vs_node_n || !d->vs_wave || !d->vs_params))
+    return fail("null voltage-source arrays");
+  if (k > 0 && (!d->is_wave || !d->is_params || !d->is_dst))
+    return fail("null current-source arrays");
+  for (int i = 0; i < m; ++i)
+    if (d->vs_node_p[i] < 0 || d->vs_node_p[i] >= d->n_total || d->vs_node_n[i] < 0 ||
+        d->vs_node_n[i] >= d->n_total)
+      return fail("voltage-source node out of range");
+  // waveform codes; a pwl source needs its points: pointers from 0, not
+  // decreasing, at least one point per pwl source, times not decreasing
+  auto check_sources = [&](int n, const int32_t* wave, const int32_t* ptr, const double* tt,
+                           const double* vv) -> int {
+    bool any_pwl = false;
+    for (int i = 0; i < n; ++i) {
+      if (wave[i] < 0 || wave[i] >= VJX_WAVE_COUNT) return fail("unknown waveform");
+      any_pwl = any_pwl || wave[i] == VJX_WAVE_PWL;
+    }
+    if (!any_pwl) return VJX_OK;
+    if (!valid_segments(ptr, n, ptr ? ptr[n] : 0) || (ptr[n] > 0 && (!tt || !vv)))
+      return fail("pwl point arrays missing or malformed");
+    for (int i = 0; i < n; ++i) {
+      if (wave[i] != VJX_WAVE_PWL) continue;
+      if (ptr[i + 1] <= ptr[i]) return fail("pwl source without points");
+      for (int e = ptr[i] + 1; e < ptr[i + 1]; ++e)
+        if (!(tt[e] >= tt[e - 1])) return fail("pwl times decrease");
+    }
+    return VJX_OK;
+  };
+  if (m > 0) {
+    int r = check_sources(m, d->vs_wave, d->vs_pwl_ptr, d->vs_pwl_t, d->vs_pwl_v);
+    if (r < 0) return r;
+  }
+  if (k > 0) {
+    int r = check_sources(k, d->is_wave, d->is_pwl_ptr, d->is_pwl_t, d->is_pwl_v);
+    if (r < 0) return r;
+  }
+  for (int e = 0; e < 2 * k; ++e)
+    if (d->is_dst[e] < -1 || d->is_dst[e] >= d->n_res_contrib)
+      return fail("current-source slot out of range");
+  for (int i = 0; i < d->n_models; ++i) {
+    const vjx_model_desc& md = d->models[i];
+    if (md.model_id < 0 || md.model_id >= VJX_MODEL_COUNT || md.count < 0)
+      return fail("unknown model id");
+    if (md.count == 0) continue;
+    if (!md.v_node1 || !md.v_node2 || !md.jac_dst || !md.res_dst || !md.params)
+      return fail("null model arrays");
+    const size_t N = (size_t)md.count;
+    const int n_v = model_n_v(md.model_id), n_jac = model_n_jac(md.model_id),
+              n_res = model_n_res(md.model_id);
+    for (size_t e = 0; e < N * n_v; ++e)
+      if (md.v_node1[e] < 0 || md.v_node1[e] >= d->n_total || md.v_node2[e] < 0 ||
+          md.v_node2[e] >= d->n_total)
+        return fail("branch-voltage node index out of range");
+    for (size_t e = 0; e < N * n_jac; ++e)
+      if (md.jac_dst[e] < -1 || md.jac_dst[e] >= d->n_jac_contrib)
+        return fail("Jacobian contribution slot out of range");
+    for (size_t e = 0; e < N * n_res; ++e)
+      if (md.res_dst[e] < -1 || md.res_dst[e] >= d->n_res_contrib)
+        return fail("residual contribution slot out of range");
+  }
+  if (d->n_noi > 0 && !d->noi_rows) return fail("bad NOI list");
+  for (int i = 0; i < d->n_noi; ++i)
+    if (d->noi_rows[i] < 0 || d->noi_rows[i] >= nu) return fail("NOI row out of range");
+  return VJX_OK;
+}
+
+}  // namespace
+
 int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
   if (!d || !out) {
     set_error("vjx_circuit_create: null argument");
     return VJX_ERR_ARG;
   }
-  if (d->n_total < 1 || d->n_vsources < 0 || d->n_isources < 0 || d->nse < 0 ||
-      d->n_models < 0 || d->n_models > VJX_MAX_MODELS) {
-    set_error("vjx_circuit_create: invalid sizes");
-    return VJX_ERR_ARG;
-  }
+  int vrc = validate_desc(d);
+  if (vrc < 0) return vrc;
   std::unique_ptr<vjx_circuit> c(new vjx_circuit());
   c->n_total = d->n_total;
   c->n_unk = d->n_total - 1;
@@ -99,17 +202,12 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
   rc = pattern_upload_csr(c->pat);
   if (rc < 0) return rc;
   const int nu = c->n_unk, m = c->m, k = c->k;
-  if (d->jac_seg[0] != 0 || d->jac_seg[c->nse] != d->n_jac_contrib || d->res_seg[0] != 0 ||
-      d->res_seg[nu] != d->n_res_contrib) {
-    set_error("vjx_circuit_create: reduction segments do not match contribution counts");
-    return VJX_ERR_ARG;
-  }
   VJX_TRY_HIP(c->slot_diag.upload(d->slot_diag, c->nse));
   VJX_TRY_HIP(c->slot_b.upload(d->slot_b, c->nse));
   VJX_TRY_HIP(c->jac_seg.upload(d->jac_seg, c->nse + 1));
   VJX_TRY_HIP(c->res_seg.upload(d->res_seg, nu + 1));
   for (int i = 0; i < d->n_models; ++i)
-    if (d->models[i].model_id >= 0 && d->models[i].model_id <= VJX_MODEL_DIODE &&
+    if (d->models[i].model_id >= 0 && d->models[i].model_id < VJX_MODEL_COUNT &&
         d->models[i].count > 0)
       c->n_lim += model_n_lim(d->models[i].model_id) * d->models[i].count;
   VJX_TRY_HIP(c->hub_slots.build(d->jac_seg, c->nse, 1));
@@ -119,14 +217,20 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
   VJX_TRY_HIP(c->vs_p.upload(d->vs_node_p, m));
   VJX_TRY_HIP(c->vs_n.upload(d->vs_node_n, m));
   VJX_TRY_HIP(c->vs_wave.upload(d->vs_wave, m));
-  VJX_TRY_HIP(c->vs_par.upload(d->vs_params, (size_t)8 * m));
+  VJX_TRY_HIP(c->vs_par.upload(d->vs_params, (size_t)VJX_SRC_NPAR * m));
   VJX_TRY_HIP(c->is_wave.upload(d->is_wave, k));
-  VJX_TRY_HIP(c->is_par.upload(d->is_params, (size_t)8 * k));
+  VJX_TRY_HIP(c->is_par.upload(d->is_params, (size_t)VJX_SRC_NPAR * k));
   VJX_TRY_HIP(c->is_dst.upload(d->is_dst, (size_t)2 * k));
-  c->h_vs_wave.assign(d->vs_wave, d->vs_wave + m);
-  c->h_is_wave.assign(d->is_wave, d->is_wave + k);
-  c->h_vs_par.assign(d->vs_params, d->vs_params + (size_t)8 * m);
-  c->h_is_par.assign(d->is_params, d->is_params + (size_t)8 * k);
+  if (d->vs_pwl_ptr) {
+    VJX_TRY_HIP(c->vs_pwl_ptr.upload(d->vs_pwl_ptr, m + 1));
+    VJX_TRY_HIP(c->vs_pwl_t.upload(d->vs_pwl_t, d->vs_pwl_ptr[m]));
+    VJX_TRY_HIP(c->vs_pwl_v.upload(d->vs_pwl_v, d->vs_pwl_ptr[m]));
+  }
+  if (d->is_pwl_ptr) {
+    VJX_TRY_HIP(c->is_pwl_ptr.upload(d->is_pwl_ptr, k + 1));
+    VJX_TRY_HIP(c->is_pwl_t.upload(d->is_pwl_t, d->is_pwl_ptr[k]));
+    VJX_TRY_HIP(c->is_pwl_v.upload(d->is_pwl_v, d->is_pwl_ptr[k]));
+  }
   if (d->conv_mask) {
     VJX_TRY_HIP(c->conv_mask.upload(d->conv_mask, nu));
     c->has_conv_mask = true;
@@ -147,7 +251,7 @@ int vjx_circuit_create(const vjx_circuit_desc* d, vjx_circuit** out) {
   int64_t lim_off = 0;
   for (int i = 0; i < d->n_models; ++i) {
     const vjx_model_desc& md = d->models[i];
-    if (md.model_id < 0 || md.model_id > VJX_MODEL_DIODE || md.count < 0) {
+    if (md.model_id < 0 || md.model_id >= VJX_MODEL_COUNT || md.count < 0) {
       set_error("vjx_circuit_create: unknown model id");
       return VJX_ERR_ARG;
     }
@@ -321,7 +425,7 @@ int vjx_eval_stamp(vjx_circuit* c, int32_t model_index, const double* X, const d
 int vjx_model_eval(int32_t model_id, int32_t count, const double* vb, const double* par,
                    const double* li, double* rr, double* rc, double* jr, double* jc, double* lr,
                    double* lc, double* lo) {
-  if (model_id < 0 || model_id > VJX_MODEL_DIODE || count < 0) {
+  if (model_id < 0 || model_id >= VJX_MODEL_COUNT || count < 0) {
     set_error("vjx_model_eval: bad model id or count");
     return VJX_ERR_ARG;
   }
@@ -353,8 +457,8 @@ int vjx_eval_sources(vjx_circuit* c, double t, double* vsrc, double* isrc) {
     return VJX_ERR_ARG;
   }
   hipStream_t s = c->pat->stream;
-  if (c->m && vsrc) launch_sources(c->m, c->vs_wave.p, c->vs_par.p, t, vsrc, s);
-  if (c->k && isrc) launch_sources(c->k, c->is_wave.p, c->is_par.p, t, isrc, s);
+  if (c->m && vsrc) launch_sources(c->vsrc_table(), t, vsrc, s);
+  if (c->k && isrc) launch_sources(c->isrc_table(), t, isrc, s);
   VJX_TRY_HIP(hipGetLastError());
   VJX_TRY_HIP(hipStreamSynchronize(s));
   return VJX_OK;

@@ -39,8 +39,15 @@ struct vjx_circuit {
       is_dst, conv_mask;
   vjx::HubPlan hub_slots, hub_rows;  // long stamping / residual segments
   vjx::DBuf<double> slot_b, vs_par, is_par;
-  std::vector<int32_t> h_vs_wave, h_is_wave;
-  std::vector<double> h_vs_par, h_is_par;
+  // pwl points (empty when no source of the kind is a pwl)
+  vjx::DBuf<int32_t> vs_pwl_ptr, is_pwl_ptr;
+  vjx::DBuf<double> vs_pwl_t, vs_pwl_v, is_pwl_t, is_pwl_v;
+  vjx::SourceTable vsrc_table() const {
+    return {m, vs_wave.p, vs_par.p, vs_pwl_ptr.p, vs_pwl_t.p, vs_pwl_v.p};
+  }
+  vjx::SourceTable isrc_table() const {
+    return {k, is_wave.p, is_par.p, is_pwl_ptr.p, is_pwl_t.p, is_pwl_v.p};
+  }
   int n_jac_contrib = 0, n_res_contrib = 0;
   vjx::DBuf<double> jac_contrib, rf, rq;
   // limited devices: flat limit-state count, lim_rhs contributions (same

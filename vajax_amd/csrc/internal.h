@@ -75,8 +75,8 @@ struct DevStatus {
   unsigned long long limr_bits;  // max |limit_state_in|
   unsigned int res_fail;
   unsigned int delta_fail;
-  unsigned int nonfinite;
-  unsigned int pad;
+  unsigned int nonfinite;     // non-finite LU factor entries (pivot monitor)
+  unsigned int nonfinite_in;  // non-finite entries of the matrix being factorised
 };
 
 struct DevModel {
@@ -246,9 +246,11 @@ void launch_residual(const ResidualArgs& a, hipStream_t s);
 
 template <typename T>
 void launch_lu_refactor(DevLU& lu, const double* csr, T* lu_vals, DevStatus* st, hipStream_t s);
+// hub_parts (may be null): the forward and backward solves' hub chunk
+// partials, in place of the band programs' own buffers
 template <typename T>
 void launch_lu_solve(DevLU& lu, const T* lu_vals, const double* rhs, double rhs_scale, double* x,
-                     T* y, T* z, hipStream_t s);
+                     T* y, T* z, hipStream_t s, void* const* hub_parts = nullptr);
 void launch_axpy(int n, double* x, const double* d, hipStream_t s);
 
 void launch_model_eval_raw(int model_id, int count, const double* vb, const double* par,
@@ -258,8 +260,17 @@ void launch_limit(int kind, int n, const double* vnew, const double* vold, doubl
                   double* out, hipStream_t s);
 void launch_lim_stats(int n, const double* ls_in, const double* ls_out, DevStatus* st,
                       hipStream_t s);
-void launch_sources(int m, const int32_t* wave, const double* params, double t, double* out,
-                    hipStream_t s);
+// Independent sources of one kind on the device: waveform codes, the
+// [VJX_SRC_NPAR][n] parameter block and the pwl points (may be null).
+struct SourceTable {
+  int n;
+  const int32_t* wave;
+  const double* par;
+  const int32_t* pwl_ptr;
+  const double* pwl_t;
+  const double* pwl_v;
+};
+void launch_sources(const SourceTable& src, double t, double* out, hipStream_t s);
 
 int model_n_v(int model_id);
 int model_n_lim(int model_id);

@@ -661,13 +661,21 @@ __global__ void __launch_bounds__(256) k_residual(ResidualArgs a) {
 
 // ---- LU refactor ------------------------------------------------------------
 
+// LU values <- matrix values; counts non-finite matrix entries into
+// st->nonfinite_in (a non-finite Jacobian is not a pivoting problem).
 template <typename T>
 __global__ void __launch_bounds__(256) k_lu_gather(int nnz, const int32_t* __restrict__ src,
-                                                   const double* __restrict__ csr, T* lu) {
+                                                   const double* __restrict__ csr, T* lu,
+                                                   DevStatus* st) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nnz) return;
-  const int s = src[e];
-  lu[e] = s >= 0 ? (T)csr[s] : (T)0;
+  unsigned int bad = 0;
+  if (e < nnz) {
+    const int s = src[e];
+    const double v = s >= 0 ? csr[s] : 0.0;
+    lu[e] = (T)v;
+    bad = fabs(v) <= 1.7976931348623157e308 ? 0u : 1u;
+  }
+  if (st) block_flags(0ull, 0ull, bad, nullptr, nullptr, &st->nonfinite_in);
 }
 
 // v[i] = src[slots[i]] for i < n by the whole block, 8 per thread and round:
@@ -1441,27 +1449,85 @@ __global__ void k_axpy(int n, double* x, const double* d) {
   if (i < n) x[i] = x[i] + d[i];
 }
 
-// vajax/analysis/sources.py:31-63 (pulse) and :25-28 (dc)
-__device__ __forceinline__ double wave_value(int w, const double* p, int i, int m, double t) {
-  const double dc = p[0 * (size_t)m + i];
-  if (w == 0) return dc;
-  const double v0 = p[1 * (size_t)m + i], v1 = p[2 * (size_t)m + i];
-  const double r = p[3 * (size_t)m + i], f = p[4 * (size_t)m + i];
-  const double wd = p[5 * (size_t)m + i], per = p[6 * (size_t)m + i];
-  const double d = p[7 * (size_t)m + i];
-  if (t < d) return v0;
-  double tin = fmod(t - d, per);
-  if (tin < 0.0) tin += per;
-  if (tin < r) return v0 + (v1 - v0) * tin / r;
-  if (tin < r + wd) return v1;
-  if (tin < r + wd + f) return v1 - (v1 - v0) * (tin - r - wd) / f;
-  return v0;
+// Independent-source waveforms, vajax/analysis/sources.py: dc :25-28,
+// pulse :31-63, sine :66-78, pwl :81-106 (jnp.interp: clamped ends, the
+// segment found by a right-sided search), exp :109-125, am :128-156,
+// fm :159-186.  Parameter columns as documented in vjx.h.
+__device__ double wave_value(const SourceTable& S, int i, double t) {
+  const size_t m = (size_t)S.n;
+  const double* p = S.par;
+  auto P = [&](int c) { return p[c * m + i]; };
+  const double kPi = 3.141592653589793;
+  switch (S.wave[i]) {
+    case VJX_WAVE_PULSE: {
+      const double v0 = P(1), v1 = P(2), r = P(3), f = P(4), wd = P(5), per = P(6), d = P(7);
+      if (t < d) return v0;
+      double tin = fmod(t - d, per);
+      if (tin < 0.0) tin += per;
+      if (tin < r) return v0 + (v1 - v0) * tin / r;
+      if (tin < r + wd) return v1;
+      if (tin < r + wd + f) return v1 - (v1 - v0) * (tin - r - wd) / f;
+      return v0;
+    }
+    case VJX_WAVE_SINE:
+      return P(1) + P(2) * sin(2.0 * kPi * P(3) * t + P(4));
+    case VJX_WAVE_PWL: {
+      const double per = P(1);
+      if (per > 0.0) {
+        t = fmod(t, per);
+        if (t < 0.0) t += per;
+      }
+      const int a = S.pwl_ptr[i], b = S.pwl_ptr[i + 1];
+      const double* xp = S.pwl_t;
+      const double* fp = S.pwl_v;
+      if (t < xp[a]) return fp[a];
+      if (t > xp[b - 1]) return fp[b - 1];
+      if (b - a == 1) return fp[a];
+      // first index in (a, b) with xp > t, clamped to [a + 1, b - 1]
+      int lo = a, hi = b;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (xp[mid] <= t)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const int j = lo < a + 1 ? a + 1 : (lo > b - 1 ? b - 1 : lo);
+      const double dx = xp[j] - xp[j - 1];
+      if (fabs(dx) <= 4.930380657631324e-32) return fp[j - 1];  // spacing(DBL_EPSILON)
+      return fp[j - 1] + ((t - xp[j - 1]) / dx) * (fp[j] - fp[j - 1]);
+    }
+    case VJX_WAVE_EXP: {
+      const double v0 = P(1), v1 = P(2), d = P(3), td = P(4), t1 = P(5), t2 = P(6);
+      const double tf = d + td;
+      if (t < d) return v0;
+      if (t < tf) return v0 + (v1 - v0) * (1.0 - exp(-(t - d) / t1));
+      const double risen = v0 + (v1 - v0) * (1.0 - exp(-td / t1));
+      return risen + (v0 - risen) * (1.0 - exp(-(t - tf) / t2));
+    }
+    case VJX_WAVE_AM:
+    case VJX_WAVE_FM: {
+      const double dc = P(1), a = P(2), f = P(3), ph = P(4), d = P(5), mf = P(6), mph = P(7),
+                   mi = P(8);
+      if (S.wave[i] == VJX_WAVE_AM) {
+        if (t < d) return dc + a * sin(ph * kPi / 180) * (1 + mi * sin(mph * kPi / 180));
+        const double carrier = sin(2 * kPi * f * (t - d) + ph * kPi / 180);
+        const double modulation = 1 + mi * sin(2 * kPi * mf * (t - d) + mph * kPi / 180);
+        return dc + a * carrier * modulation;
+      }
+      if (t < d) return dc + a * sin(ph * kPi / 180 + mi * sin(mph * kPi / 180));
+      const double cph = 2 * kPi * f * (t - d) + ph * kPi / 180;
+      const double mod = mi * sin(2 * kPi * mf * (t - d) + mph * kPi / 180);
+      return dc + a * sin(cph + mod);
+    }
+    default:
+      return P(0);
+  }
 }
 
-__global__ void k_sources(int m, const int32_t* __restrict__ wave, const double* __restrict__ par,
-                          double t, double* out) {
+__global__ void k_sources(SourceTable src, double t, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) out[i] = wave_value(wave[i], par, i, m, t);
+  if (i < src.n) out[i] = wave_value(src, i, t);
 }
 
 __global__ void k_status_reset(DevStatus* st) {
@@ -1475,7 +1541,7 @@ __global__ void k_status_reset(DevStatus* st) {
     st->res_fail = 0;
     st->delta_fail = 0;
     st->nonfinite = 0;
-    st->pad = 0;
+    st->nonfinite_in = 0;
   }
 }
 
@@ -1685,7 +1751,7 @@ template <typename T>
 void launch_lu_refactor(DevLU& L, const double* csr, T* lu, DevStatus* st, hipStream_t s) {
   const int nnz = (int)L.col.n;
   ktimer_begin(s);
-  k_lu_gather<T><<<nblk(nnz, 256), 256, 0, s>>>(nnz, L.src.p, csr, lu);
+  k_lu_gather<T><<<nblk(nnz, 256), 256, 0, s>>>(nnz, L.src.p, csr, lu, st);
   ktimer_end(KC_LU_GATHER, s);
   run_fact_groups<T>(L, lu, st, s);
   // band programs scaled L inside their tasks
@@ -1719,10 +1785,10 @@ static void run_tri_groups(DevLU& L, const DevSchedule& S, const T* lu, T* y, T*
 
 template <typename T, bool FWD>
 static void run_tri_bands(DevLU& L, const BandSchedule& H, DevBand& D, const T* lu, T* y, T* z,
-                          hipStream_t s) {
+                          void* part_override, hipStream_t s) {
   const TriProg P{nullptr, L.rowptr.p, L.col.p, L.diag.p};
   const BandProg B{D.items.p, D.tl_ptr.p, D.tl_heavy.p, D.tl_mid.p};
-  T* part = reinterpret_cast<T*>(D.part.p);
+  T* part = reinterpret_cast<T*>(part_override ? part_override : (void*)D.part.p);
   for (size_t b = 0; b + 1 < H.band_task.size(); ++b) {
     const int ntask = H.band_task[b + 1] - H.band_task[b];
     if (ntask > 0) {
@@ -1756,16 +1822,18 @@ static void run_tri_bands(DevLU& L, const BandSchedule& H, DevBand& D, const T* 
 
 template <typename T>
 void launch_lu_solve(DevLU& L, const T* lu, const double* rhs, double scale, double* x, T* y, T* z,
-                     hipStream_t s) {
+                     hipStream_t s, void* const* hub_parts) {
   const int n = L.sym.n;
   if (n == 0) return;
   k_solve_init<T><<<nblk(n, 256), 256, 0, s>>>(n, L.p.p, rhs, scale, y);
   if (L.fwd_band.on)
-    run_tri_bands<T, true>(L, L.sym.fwd_band, L.fwd_band, lu, y, z, s);
+    run_tri_bands<T, true>(L, L.sym.fwd_band, L.fwd_band, lu, y, z,
+                           hub_parts ? hub_parts[0] : nullptr, s);
   else
     run_tri_groups<T, true>(L, L.fwd, lu, y, z, s);
   if (L.bwd_band.on)
-    run_tri_bands<T, false>(L, L.sym.bwd_band, L.bwd_band, lu, y, z, s);
+    run_tri_bands<T, false>(L, L.sym.bwd_band, L.bwd_band, lu, y, z,
+                            hub_parts ? hub_parts[1] : nullptr, s);
   else
     run_tri_groups<T, false>(L, L.bwd, lu, y, z, s);
   k_solve_finish<T><<<nblk(n, 256), 256, 0, s>>>(n, L.q.p, z, x);
@@ -1774,9 +1842,9 @@ void launch_lu_solve(DevLU& L, const T* lu, const double* rhs, double scale, dou
 template void launch_lu_refactor<double>(DevLU&, const double*, double*, DevStatus*, hipStream_t);
 template void launch_lu_refactor<float>(DevLU&, const double*, float*, DevStatus*, hipStream_t);
 template void launch_lu_solve<double>(DevLU&, const double*, const double*, double, double*,
-                                      double*, double*, hipStream_t);
+                                      double*, double*, hipStream_t, void* const*);
 template void launch_lu_solve<float>(DevLU&, const float*, const double*, double, double*, float*,
-                                     float*, hipStream_t);
+                                     float*, hipStream_t, void* const*);
 
 void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const double* data,
                  const double* x, const double* base, double sgn, double* out,
@@ -1795,10 +1863,9 @@ void launch_axpy(int n, double* x, const double* d, hipStream_t s) {
   k_axpy<<<nblk(n, 256), 256, 0, s>>>(n, x, d);
 }
 
-void launch_sources(int m, const int32_t* wave, const double* params, double t, double* out,
-                    hipStream_t s) {
-  if (m == 0) return;
-  k_sources<<<nblk(m, 64), 64, 0, s>>>(m, wave, params, t, out);
+void launch_sources(const SourceTable& src, double t, double* out, hipStream_t s) {
+  if (src.n == 0) return;
+  k_sources<<<nblk(src.n, 64), 64, 0, s>>>(src, t, out);
 }
 
 }  // namespace vjx

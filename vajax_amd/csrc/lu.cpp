@@ -322,6 +322,11 @@ int pattern_check_pivots(vjx_pattern* pat) {
   VJX_TRY_HIP(hipMemcpy(pat->h_status, pat->d_status.p, sizeof(DevStatus), hipMemcpyDeviceToHost));
   double maxl;
   std::memcpy(&maxl, &pat->h_status->maxl_bits, sizeof(double));
+  if (pat->h_status->nonfinite_in > 0) {
+    set_error("vjx_lu_refactor: " + std::to_string(pat->h_status->nonfinite_in) +
+              " non-finite matrix entries");
+    return VJX_WARN_PIVOT;
+  }
   if (pat->h_status->nonfinite > 0 || maxl > 1e12) {
     set_error("vjx_lu_refactor: pivot growth (max |L| = " + std::to_string(maxl) +
               ", non-finite = " + std::to_string(pat->h_status->nonfinite) +

@@ -29,23 +29,6 @@ double bits_to_double(unsigned long long b) {
   return d;
 }
 
-// sources.py:31-63 (pulse) / :25-28 (dc)
-double wave_value(int w, const std::vector<double>& p, int i, int m, double t) {
-  const double dc = p[0 * (size_t)m + i];
-  if (w == 0) return dc;
-  const double v0 = p[1 * (size_t)m + i], v1 = p[2 * (size_t)m + i];
-  const double r = p[3 * (size_t)m + i], f = p[4 * (size_t)m + i];
-  const double wd = p[5 * (size_t)m + i], per = p[6 * (size_t)m + i];
-  const double d = p[7 * (size_t)m + i];
-  if (t < d) return v0;
-  double tin = std::fmod(t - d, per);
-  if (tin < 0.0) tin += per;
-  if (tin < r) return v0 + (v1 - v0) * tin / r;
-  if (tin < r + wd) return v1;
-  if (tin < r + wd + f) return v1 - (v1 - v0) * (tin - r - wd) / f;
-  return v0;
-}
-
 struct Work {
   DBuf<double> Xa, Xw, Xsave, Vp, Qp, dQ, Q2, Qn, dQn, mrc, hmrc, floor, Vmax, f, csr, delta, vsrc, isrc,
       Ivs, hist, vout, iout, zeros;
@@ -98,14 +81,11 @@ struct Driver {
     return VJX_OK;
   }
 
+  // source values at t, evaluated on the device (no host round trip)
   int set_sources(double t) {
-    std::vector<double> v(std::max(m, 1)), i(std::max(k, 1));
-    for (int j = 0; j < m; ++j) v[j] = wave_value(c->h_vs_wave[j], c->h_vs_par, j, m, t);
-    for (int j = 0; j < k; ++j) i[j] = wave_value(c->h_is_wave[j], c->h_is_par, j, k, t);
-    if (m) VJX_TRY_HIP(hipMemcpyAsync(w.vsrc.p, v.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
-    if (k) VJX_TRY_HIP(hipMemcpyAsync(w.isrc.p, i.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-    // the host vectors die at return: make the copies complete first
-    VJX_TRY_HIP(hipStreamSynchronize(s));
+    launch_sources(c->vsrc_table(), t, w.vsrc.p, s);
+    launch_sources(c->isrc_table(), t, w.isrc.p, s);
+    VJX_TRY_HIP(hipGetLastError());
     return VJX_OK;
   }
 
@@ -268,6 +248,7 @@ struct Driver {
     conv = false;
     max_f = std::numeric_limits<double>::infinity();
     vjx_pattern* pat = c->pat;
+    bool redone = false;  // this iteration already re-pivoted once
     while (!conv && iters < cfg->tran_itl) {
       int rc = upload_scalars(c0, c1, d1, c2, gshunt, iters);
       if (rc < 0) return rc;
@@ -276,25 +257,36 @@ struct Driver {
       // the iteration exported both status records as its last kernel
       VJX_TRY_HIP(hipStreamSynchronize(s));
       collect_phase_times();
-      // pivot monitor: re-pivot on this matrix and redo the iteration from
-      // the saved iterate
+      // pivot monitor: a finite matrix whose factor is non-finite or grows
+      // beyond the threshold is re-pivoted (at most once per iteration) and
+      // the iteration redone from the saved iterate.  A non-finite matrix
+      // (device eval overflow) is not a pivoting problem: the iteration
+      // stands, its non-finite update fails the convergence tests, as the
+      // reference's solve would (solver_factories.py:1225-1297).  A matrix
+      // the analysis finds singular keeps the previous pivots (UMFPACK's
+      // singular-matrix warning is tolerated, umfpack_ffi.cpp:297, 320).
       const DevStatus& lu_st = w.h_st[1];
       const double maxl = bits_to_double(lu_st.maxl_bits);
       const double growth = cfg->repivot_growth > 0.0 ? cfg->repivot_growth : 1e12;
-      if ((lu_st.nonfinite > 0 || maxl > growth) && repivots < 64) {
+      if (lu_st.nonfinite_in == 0 && !redone && (lu_st.nonfinite > 0 || maxl > growth) &&
+          repivots < 64) {
+        redone = true;
         std::vector<double> h(c->nse);
         VJX_TRY_HIP(hipMemcpy(h.data(), w.csr.p, sizeof(double) * c->nse, hipMemcpyDeviceToHost));
         rc = vjx_lu_analyze(pat, h.data());
-        if (rc < 0) return rc;
-        rc = pattern_upload(pat);
-        if (rc < 0) return rc;
-        drop_graph();  // factor buffers were reallocated
-        ++repivots;
-        launch_copy(n_total + m, w.Xsave.p, w.Xw.p, s);
-        if (n_lim) launch_copy(n_lim, w.ls_save.p, w.ls_iter.p, s);
-        --total_evals;
-        continue;
+        if (rc != VJX_ERR_SINGULAR) {
+          if (rc < 0) return rc;
+          rc = pattern_upload(pat);
+          if (rc < 0) return rc;
+          drop_graph();  // factor buffers were reallocated
+          ++repivots;
+          launch_copy(n_total + m, w.Xsave.p, w.Xw.p, s);
+          if (n_lim) launch_copy(n_lim, w.ls_save.p, w.ls_iter.p, s);
+          --total_evals;
+          continue;
+        }
       }
+      redone = false;
       const bool res_ok = w.h_st->res_fail == 0;
       const bool delta_ok = (iters == 0) || w.h_st->delta_fail == 0;
       max_f = bits_to_double(w.h_st->maxf_bits);

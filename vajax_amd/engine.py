@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib as L
 from .models import param_block
-from .netlist import Netlist
+from .netlist import Netlist, pack_sources
 from .sparse import SparseLU
 from .topology import Topology, build_reduction_map, build_topology
 
@@ -145,99 +145,105 @@ def _soa(a: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.int64).T, dtype=np.int32)
 
 
+def circuit_desc(topo: Topology, net: Netlist, red=None):
+    """The C ABI's vjx_circuit_desc for a topology, and the numpy buffers it
+    points into (keep them alive until vjx_circuit_create returns)."""
+    red = build_reduction_map(topo) if red is None else red
+    n_unk, m, k = topo.n_unknowns, topo.n_vsources, net.isources.count
+    nse = topo.nse
+    keep = []  # keep numpy buffers alive while the C side copies them
+
+    def hold(a):
+        keep.append(a)
+        return a
+
+    slot_diag = np.zeros(nse, np.int32)
+    slot_diag[topo.diag_pos] = 1
+    slot_b = np.zeros(nse, np.float64)
+    for key, val in (("b_p", 1.0), ("b_n", -1.0), ("bt_p", 1.0), ("bt_n", -1.0)):
+        pos = topo.vs_pos.get(key, np.zeros(0, np.int32))
+        pos = pos[pos >= 0]
+        if np.any(slot_b[pos] != 0.0):
+            raise ValueError("voltage source with both terminals on one node")
+        slot_b[pos] += val
+    br_ptr = np.zeros(n_unk + 1, np.int32)
+    ents = [[] for _ in range(n_unk)]
+    for kk in range(m):
+        p = int(topo.vs_node_p[kk])
+        if p > 0:
+            ents[p - 1].append(kk)
+    for kk in range(m):
+        nn = int(topo.vs_node_n[kk])
+        if nn > 0:
+            ents[nn - 1].append(~kk)
+    br_ptr[1:] = np.cumsum([len(e) for e in ents])
+    br_ent = np.array([x for e in ents for x in e], dtype=np.int32)
+
+    def src_table(src):
+        w, par, ptr, tt, vv = pack_sources(src)
+        return (L.ptr(hold(w)), L.ptr(hold(par), C.c_double),
+                L.ptr(hold(ptr)) if ptr is not None else None,
+                L.ptr(hold(tt), C.c_double) if ptr is not None else None,
+                L.ptr(hold(vv), C.c_double) if ptr is not None else None)
+
+    d = L.CircuitDesc()
+    d.n_total = topo.n_total
+    d.n_vsources = m
+    d.n_isources = k
+    d.nse = nse
+    d.indptr = L.ptr(hold(np.ascontiguousarray(topo.indptr, np.int32)))
+    d.indices = L.ptr(hold(np.ascontiguousarray(topo.indices, np.int32)))
+    d.slot_diag = L.ptr(hold(slot_diag))
+    d.slot_b = L.ptr(hold(slot_b), C.c_double)
+    d.vs_node_p = L.ptr(hold(np.ascontiguousarray(topo.vs_node_p, np.int32)))
+    d.vs_node_n = L.ptr(hold(np.ascontiguousarray(topo.vs_node_n, np.int32)))
+    d.vs_wave, d.vs_params, d.vs_pwl_ptr, d.vs_pwl_t, d.vs_pwl_v = src_table(net.vsources)
+    d.is_wave, d.is_params, d.is_pwl_ptr, d.is_pwl_t, d.is_pwl_v = src_table(net.isources)
+    d.is_dst = L.ptr(hold(np.ascontiguousarray(red.is_dst, np.int32)))
+    d.n_jac_contrib = red.n_jac_contrib
+    d.n_res_contrib = red.n_res_contrib
+    d.jac_seg = L.ptr(hold(np.ascontiguousarray(red.jac_seg, np.int32)))
+    d.res_seg = L.ptr(hold(np.ascontiguousarray(red.res_seg, np.int32)))
+    d.br_ptr = L.ptr(hold(br_ptr))
+    d.br_ent = L.ptr(hold(br_ent))
+    # residual test mask: internal device nodes and NOI nodes are left out
+    # (solver_factories.py:86-92); NOI rows for the solve
+    skip = [topo.node_names[n] - 1 for n in list(net.internal_nodes) + list(net.noi_nodes)]
+    if skip:
+        mask = np.ones(topo.n_unknowns, dtype=np.int32)
+        mask[skip] = 0
+        d.conv_mask = L.ptr(hold(mask))
+    else:
+        d.conv_mask = None
+    noi = sorted({topo.node_names[n] - 1 for n in net.noi_nodes})
+    d.n_noi = len(noi)
+    d.noi_rows = L.ptr(hold(np.asarray(noi, dtype=np.int32))) if noi else None
+    models = list(topo.models.items())
+    if len(models) > L.MAX_MODELS:
+        raise ValueError("too many model groups")
+    d.n_models = len(models)
+    for i, (name, ms) in enumerate(models):
+        md = d.models[i]
+        md.model_id = L.MODEL_IDS[name]
+        md.count = ms.count
+        n_jac = ms.csr_pos.shape[1]
+        n_res = ms.res_idx.shape[1]
+        md.v_node1 = L.ptr(hold(_soa(ms.v_node1)))
+        md.v_node2 = L.ptr(hold(_soa(ms.v_node2)))
+        md.jac_dst = L.ptr(hold(_soa(red.jac_dst[name].reshape(ms.count, n_jac))))
+        md.res_dst = L.ptr(hold(_soa(red.res_dst[name].reshape(ms.count, n_res))))
+        md.params = L.ptr(hold(param_block(name, ms.params)), C.c_double)
+    return d, keep
+
+
 class DeviceCircuit:
     """A circuit resident on the GPU (vjx_circuit handle)."""
 
     def __init__(self, topo: Topology, net: Netlist):
         self.topo = topo
         self.net = net
-        red = build_reduction_map(topo)
-        self.red = red
-        n_unk, m, k = topo.n_unknowns, topo.n_vsources, net.isources.count
-        nse = topo.nse
-        keep = []  # keep numpy buffers alive while the C side copies them
-
-        def hold(a):
-            keep.append(a)
-            return a
-
-        slot_diag = np.zeros(nse, np.int32)
-        slot_diag[topo.diag_pos] = 1
-        slot_b = np.zeros(nse, np.float64)
-        for key, val in (("b_p", 1.0), ("b_n", -1.0), ("bt_p", 1.0), ("bt_n", -1.0)):
-            pos = topo.vs_pos.get(key, np.zeros(0, np.int32))
-            pos = pos[pos >= 0]
-            if np.any(slot_b[pos] != 0.0):
-                raise ValueError("voltage source with both terminals on one node")
-            slot_b[pos] += val
-        br_ptr = np.zeros(n_unk + 1, np.int32)
-        ents = [[] for _ in range(n_unk)]
-        for kk in range(m):
-            p = int(topo.vs_node_p[kk])
-            if p > 0:
-                ents[p - 1].append(kk)
-        for kk in range(m):
-            nn = int(topo.vs_node_n[kk])
-            if nn > 0:
-                ents[nn - 1].append(~kk)
-        br_ptr[1:] = np.cumsum([len(e) for e in ents])
-        br_ent = np.array([x for e in ents for x in e], dtype=np.int32)
-
-        def src_params(src):
-            cols = ("dc", "val0", "val1", "rise", "fall", "width", "period", "delay")
-            if src.count == 0:
-                return np.zeros((8, 0))
-            return np.ascontiguousarray(np.stack([src.params[c] for c in cols]), dtype=np.float64)
-
-        d = L.CircuitDesc()
-        d.n_total = topo.n_total
-        d.n_vsources = m
-        d.n_isources = k
-        d.nse = nse
-        d.indptr = L.ptr(hold(np.ascontiguousarray(topo.indptr, np.int32)))
-        d.indices = L.ptr(hold(np.ascontiguousarray(topo.indices, np.int32)))
-        d.slot_diag = L.ptr(hold(slot_diag))
-        d.slot_b = L.ptr(hold(slot_b), C.c_double)
-        d.vs_node_p = L.ptr(hold(np.ascontiguousarray(topo.vs_node_p, np.int32)))
-        d.vs_node_n = L.ptr(hold(np.ascontiguousarray(topo.vs_node_n, np.int32)))
-        d.vs_wave = L.ptr(hold(np.ascontiguousarray(net.vsources.wave, np.int32)))
-        d.vs_params = L.ptr(hold(src_params(net.vsources)), C.c_double)
-        d.is_wave = L.ptr(hold(np.ascontiguousarray(net.isources.wave, np.int32)))
-        d.is_params = L.ptr(hold(src_params(net.isources)), C.c_double)
-        d.is_dst = L.ptr(hold(np.ascontiguousarray(red.is_dst, np.int32)))
-        d.n_jac_contrib = red.n_jac_contrib
-        d.n_res_contrib = red.n_res_contrib
-        d.jac_seg = L.ptr(hold(np.ascontiguousarray(red.jac_seg, np.int32)))
-        d.res_seg = L.ptr(hold(np.ascontiguousarray(red.res_seg, np.int32)))
-        d.br_ptr = L.ptr(hold(br_ptr))
-        d.br_ent = L.ptr(hold(br_ent))
-        # residual test mask: internal device nodes and NOI nodes are left out
-        # (solver_factories.py:86-92); NOI rows for the solve
-        skip = [topo.node_names[n] - 1 for n in list(net.internal_nodes) + list(net.noi_nodes)]
-        if skip:
-            mask = np.ones(topo.n_unknowns, dtype=np.int32)
-            mask[skip] = 0
-            d.conv_mask = L.ptr(hold(mask))
-        else:
-            d.conv_mask = None
-        noi = sorted({topo.node_names[n] - 1 for n in net.noi_nodes})
-        d.n_noi = len(noi)
-        d.noi_rows = L.ptr(hold(np.asarray(noi, dtype=np.int32))) if noi else None
-        models = list(topo.models.items())
-        if len(models) > L.MAX_MODELS:
-            raise ValueError("too many model groups")
-        d.n_models = len(models)
-        for i, (name, ms) in enumerate(models):
-            md = d.models[i]
-            md.model_id = L.MODEL_IDS[name]
-            md.count = ms.count
-            n_jac = ms.csr_pos.shape[1]
-            n_res = ms.res_idx.shape[1]
-            md.v_node1 = L.ptr(hold(_soa(ms.v_node1)))
-            md.v_node2 = L.ptr(hold(_soa(ms.v_node2)))
-            md.jac_dst = L.ptr(hold(_soa(red.jac_dst[name].reshape(ms.count, n_jac))))
-            md.res_dst = L.ptr(hold(_soa(red.res_dst[name].reshape(ms.count, n_res))))
-            md.params = L.ptr(hold(param_block(name, ms.params)), C.c_double)
+        self.red = build_reduction_map(topo)
+        d, keep = circuit_desc(topo, net, self.red)
         h = C.c_void_p()
         L.check(L.lib.vjx_circuit_create(C.byref(d), C.byref(h)), "vjx_circuit_create")
         self.h = h
@@ -257,6 +263,19 @@ class DeviceCircuit:
     @property
     def n_limit_states(self) -> int:
         return int(L.lib.vjx_circuit_limit_states(self.h))
+
+    def source_values(self, t: float, device="cuda"):
+        """Source values at time t evaluated on the device (vjx_eval_sources):
+        (vsource_vals, isource_vals) as torch tensors."""
+        import torch
+
+        m, k = self.topo.n_vsources, self.net.isources.count
+        v = torch.zeros(max(m, 1), dtype=torch.float64, device=device)
+        i = torch.zeros(max(k, 1), dtype=torch.float64, device=device)
+        torch.cuda.synchronize()
+        L.check(L.lib.vjx_eval_sources(self.h, float(t), C.c_void_p(v.data_ptr()),
+                                       C.c_void_p(i.data_ptr())), "vjx_eval_sources")
+        return v[:m], i[:k]
 
     def build_system(self, X, vsrc, isrc, Q_prev=None, dQdt_prev=None, Q_prev2=None, c0=0.0,
                      c1=0.0, d1=0.0, c2=0.0, gmin=1e-12, gshunt=0.0, want_jacobian=True,

@@ -374,6 +374,8 @@ def pack(nets, title: str = "batch") -> Netlist:
             terminals=np.concatenate([t for _, t, _ in acc]),
             wave=np.concatenate([s.wave for _, _, s in acc]).astype(np.int32),
             params={k: np.concatenate([s.params[k] for _, _, s in acc]) for k in SOURCE_DEFAULTS},
+            pwl=([s.pwl_points(i) for _, _, s in acc for i in range(s.count)]
+                 if any(s.pwl is not None for _, _, s in acc) else None),
         )
 
     out_groups = {}

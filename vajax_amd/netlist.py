@@ -11,8 +11,10 @@ sorted order from 1) is applied by :mod:`vajax_amd.topology`, following
 vajax/analysis/engine.py:459-468.
 
 Voltage and current sources carry a waveform record.  Supported waveform
-types are ``dc`` and ``pulse`` with the reference's parameter names and
-defaults (vajax/analysis/sources.py:204-227).
+types are the reference's ``dc``, ``pulse``, ``sine``, ``pwl``, ``exp``,
+``am`` and ``fm`` with its parameter names and defaults
+(vajax/analysis/sources.py:204-294); a pwl source's ``wave`` list of
+time/value pairs is kept per source in :attr:`SourceGroup.pwl`.
 """
 
 from __future__ import annotations
@@ -72,11 +74,34 @@ SOURCE_DEFAULTS: Dict[str, float] = {
     "width": 1e-6,
     "period": 2e-6,
     "delay": 0.0,
+    # sine / am / fm (sources.py:231-236, 264-285)
+    "sinedc": 0.0,
+    "ampl": 1.0,
+    "freq": 1e3,
+    "phase": 0.0,
+    "modfreq": 1e3,
+    "modphase": 0.0,
+    "modindex": 0.5,
+    # exp (sources.py:254-262; val0, val1 and delay shared with pulse)
+    "td2": 1e-9,
+    "tau1": 1e-10,
+    "tau2": 1e-10,
+    # pwl (sources.py:238-252)
+    "offset": 0.0,
+    "scale": 1.0,
+    "stretch": 1.0,
+    "pwlperiod": 0.0,
 }
 
 WAVE_DC = 0
 WAVE_PULSE = 1
-WAVE_CODES = {"dc": WAVE_DC, "pulse": WAVE_PULSE}
+WAVE_SINE = 2
+WAVE_PWL = 3
+WAVE_EXP = 4
+WAVE_AM = 5
+WAVE_FM = 6
+WAVE_CODES = {"dc": WAVE_DC, "pulse": WAVE_PULSE, "sine": WAVE_SINE, "pwl": WAVE_PWL,
+              "exp": WAVE_EXP, "am": WAVE_AM, "fm": WAVE_FM}
 
 
 @dataclass
@@ -102,10 +127,18 @@ class SourceGroup:
     terminals: np.ndarray  # int64 [m, 2] (p, n)
     wave: np.ndarray  # int32 [m] waveform code
     params: Dict[str, np.ndarray]  # SOURCE_DEFAULTS keys -> float64 [m]
+    # per source, the pwl ``wave`` list [t0, v0, t1, v1, ...] (empty for the
+    # other waveforms); None when no source is a pwl
+    pwl: Optional[List[np.ndarray]] = None
 
     @property
     def count(self) -> int:
         return len(self.names)
+
+    def pwl_points(self, i: int) -> np.ndarray:
+        if self.pwl is None:
+            return np.zeros(0)
+        return np.asarray(self.pwl[i], dtype=np.float64)
 
 
 def empty_sources(kind: str) -> SourceGroup:
@@ -153,7 +186,7 @@ class NetlistBuilder:
         self._dev_params: Dict[str, Dict[str, List[float]]] = {}
         self._dev_names: Dict[str, List[str]] = {}
         self._src: Dict[str, Dict[str, list]] = {
-            k: {"names": [], "terms": [], "wave": [], **{p: [] for p in SOURCE_DEFAULTS}}
+            k: {"names": [], "terms": [], "wave": [], "pwl": [], **{p: [] for p in SOURCE_DEFAULTS}}
             for k in ("vsource", "isource")
         }
         self.node(ground)
@@ -184,18 +217,34 @@ class NetlistBuilder:
             self._dev_params[model][p].append(float(params.get(p, d)))
         self._dev_names[model].append(name)
 
-    def source(self, kind: str, name: str, p: str, n: str, type: str = "dc", **params) -> None:
+    def source(self, kind: str, name: str, p: str, n: str, type: str = "dc", wave=None,
+               **params) -> None:
+        """A vsource / isource; ``wave`` is a pwl source's [t0, v0, t1, v1, ...]
+        (sources.py:238-252, default [0, 0])."""
         if kind not in self._src:
             raise ValueError(f"unknown source kind {kind!r}")
         if type not in WAVE_CODES:
             raise ValueError(f"unsupported waveform {type!r}")
+        if "sinephase" in params and "phase" not in params:
+            # am / fm accept sinephase for phase (sources.py:268, 280)
+            params["phase"] = params.pop("sinephase")
         unknown = set(params) - set(SOURCE_DEFAULTS)
         if unknown:
             raise ValueError(f"unknown source parameters {sorted(unknown)}")
+        pts = np.zeros(0)
+        if type == "pwl":
+            pts = np.asarray([0.0, 0.0] if wave is None else wave, dtype=np.float64).ravel()
+            if pts.size < 2 or pts.size % 2:
+                raise ValueError(f"pwl source {name}: wave needs time/value pairs")
+            if np.any(np.diff(pts[0::2]) < 0):
+                raise ValueError(f"pwl source {name}: times must not decrease")
+        elif wave is not None:
+            raise ValueError(f"source {name}: wave is a pwl parameter")
         s = self._src[kind]
         s["names"].append(name)
         s["terms"].append([self.node(p), self.node(n)])
         s["wave"].append(WAVE_CODES[type])
+        s["pwl"].append(pts)
         for k, d in SOURCE_DEFAULTS.items():
             s[k].append(float(params.get(k, d)))
 
@@ -216,6 +265,7 @@ class NetlistBuilder:
                 terminals=np.asarray(s["terms"], dtype=np.int64).reshape(-1, 2),
                 wave=np.asarray(s["wave"], dtype=np.int32),
                 params={k: np.asarray(s[k], dtype=np.float64) for k in SOURCE_DEFAULTS},
+                pwl=list(s["pwl"]) if any(w == WAVE_PWL for w in s["wave"]) else None,
             )
         return Netlist(
             title=self.title,
@@ -229,3 +279,47 @@ class NetlistBuilder:
             internal_nodes=[n for n in internal_nodes],
             noi_nodes=[n for n in noi_nodes],
         )
+
+
+# Parameter columns of each waveform in the C ABI's [VJX_SRC_NPAR][m] block
+# (include/vjx.h); column 0 is always dc.
+SRC_NPAR = 10
+_WAVE_COLUMNS = {
+    WAVE_DC: (),
+    WAVE_PULSE: ("val0", "val1", "rise", "fall", "width", "period", "delay"),
+    WAVE_SINE: ("sinedc", "ampl", "freq", "phase"),
+    WAVE_EXP: ("val0", "val1", "delay", "td2", "tau1", "tau2"),
+    WAVE_AM: ("sinedc", "ampl", "freq", "phase", "delay", "modfreq", "modphase", "modindex"),
+    WAVE_FM: ("sinedc", "ampl", "freq", "phase", "delay", "modfreq", "modphase", "modindex"),
+    WAVE_PWL: ("pwlperiod",),
+}
+
+
+def pack_sources(src: SourceGroup):
+    """(wave int32[m], params float64[SRC_NPAR][m], pwl_ptr, pwl_t, pwl_v) for
+    the C ABI.  pwl points are stored as the reference evaluates them:
+    times * stretch, values * scale + offset (vajax/analysis/sources.py:100-101);
+    the three pwl arrays are None when no source is a pwl."""
+    m = src.count
+    wave = np.ascontiguousarray(src.wave, dtype=np.int32)
+    par = np.zeros((SRC_NPAR, m), dtype=np.float64)
+    if m:
+        par[0] = src.params["dc"]
+    for i in range(m):
+        for c, name in enumerate(_WAVE_COLUMNS[int(wave[i])], start=1):
+            par[c, i] = src.params[name][i]
+    if not np.any(wave == WAVE_PWL):
+        return wave, par, None, None, None
+    ptr = np.zeros(m + 1, dtype=np.int32)
+    ts, vs = [], []
+    for i in range(m):
+        if int(wave[i]) == WAVE_PWL:
+            w = src.pwl_points(i)
+            ts.append(w[0::2] * src.params["stretch"][i])
+            vs.append(w[1::2] * src.params["scale"][i] + src.params["offset"][i])
+            ptr[i + 1] = ptr[i] + len(ts[-1])
+        else:
+            ptr[i + 1] = ptr[i]
+    tt = np.ascontiguousarray(np.concatenate(ts), dtype=np.float64)
+    vv = np.ascontiguousarray(np.concatenate(vs), dtype=np.float64)
+    return wave, par, ptr, tt, vv
