@@ -71,6 +71,18 @@ int vjx_lu_refactor(vjx_pattern* pat, const double* data_dev, int32_t precision)
 int vjx_lu_solve(vjx_pattern* pat, const double* data_dev, const double* b_dev, double* x_dev,
                  int32_t refine_steps);
 
+/* vjx_lu_solve with no allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  Same arguments, kernels and results as vjx_lu_solve;
+ * the intermediate vectors and hub partials live in a fixed device array of
+ * about 16.4 MiB per GPU (held until the process ends), so systems of more
+ * than 524288 unknowns or hub plans of more than 16384 chunks are rejected
+ * with VJX_ERR_ARG.  The pattern must have been factorised by
+ * vjx_lu_refactor (VJX_ERR_STATE otherwise).  Calls that overlap on one GPU
+ * share that array and race. */
+int vjx_lu_solve_noalloc(vjx_pattern* pat, const double* data_dev, const double* b_dev,
+                         double* x_dev, int32_t refine_steps);
+
 /* y = A x (umfpack_ffi.cpp:345-370, UmfpackDotF64Impl). */
 int vjx_spmv(vjx_pattern* pat, const double* data_dev, const double* x_dev, double* y_dev);
 

@@ -20,6 +20,10 @@ VJX_ERR_ARG = -1
 VJX_ERR_HIP = -2
 VJX_ERR_SINGULAR = -3
 VJX_ERR_STATE = -4
+VJX_ERR_ARG = -1
+VJX_ERR_HIP = -2
+VJX_ERR_SINGULAR = -3
+VJX_ERR_STATE = -4
 VJX_PREC_F64 = 0
 VJX_PREC_F32 = 1
 VJX_METHOD = {"be": 0, "trap": 1, "gear2": 2}
@@ -158,6 +162,7 @@ _PROTOS = {
     "vjx_lu_analyze": (C.c_int, [vp, f64p]),
     "vjx_lu_refactor": (C.c_int, [vp, vp, C.c_int32]),
     "vjx_lu_solve": (C.c_int, [vp, vp, vp, vp, C.c_int32]),
+    "vjx_lu_solve_noalloc": (C.c_int, [vp, vp, vp, vp, C.c_int32]),
     "vjx_spmv": (C.c_int, [vp, vp, vp, vp]),
     "vjx_solve_host": (C.c_int, [C.c_int32, C.c_int32, i32p, i32p, f64p, f64p, f64p]),
     "vjx_lu_solve_transpose": (C.c_int, [vp, vp, vp, vp, C.c_int32, C.c_int32]),
