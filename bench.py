@@ -4,16 +4,26 @@ Workload (one "step" = one accepted transient time step, each running the full
 Newton loop: device eval, stamping, sparse LU refactorisation, triangular
 solves): the seeded gate DAG with 65 000 two-input NAND/NOR gates (260 000
 MOSFETs, ~130 000 unknowns), adaptive BDF2 from the power-up initial state.
-``--steps K`` steps are timed after ``--warmup W`` untimed steps (the warm-up
-run also performs the one-time symbolic analysis, which is excluded like the
-reference excludes JIT and setup).
+One run takes ``--warmup W`` accepted steps untimed -- through the 100 ps
+supply ramp into the switching of the primary inputs -- and the next
+``--steps K`` accepted steps are timed (the driver's measurement window,
+between stream synchronisations).  A short run before it performs the
+one-time symbolic analysis and the launch-plan autotuner, which are excluded
+like the reference excludes JIT and setup and reported beside the result.
 
-Prints one JSON line with ms/step and Newton iterations/s (``value``), a
-roofline estimate for the dominant phase from HIP events recorded on the
-library's own stream in a second, eager (non-graph) pass over the same steps,
-and a CPU baseline: the C++ restatement of the reference algorithm
-(oracle/ctran.cpp) timed on the same circuit's first steps on all host threads
-and on one.
+Prints one JSON line with ms/step and Newton iterations/s (``value``) over the
+timed window, the same window in mixed precision with its waveform error
+against the FP64 run on the FP64 run's time grid, a roofline estimate for the
+dominant kernel class from HIP events recorded on the library's own stream in
+a second, eager (non-graph) pass over the same steps, and a CPU baseline: the
+C++ restatement of the reference algorithm (oracle/ctran.cpp) timed on the
+same circuit's first steps on all host threads and on one.  Per step the two
+sides run different windows (different Newton iteration counts), so the
+speed-up is stated per Newton iteration as well.
+
+``--summarize-trace kernel_trace.csv`` instead prints a markdown table of the
+kernels a rocprofv3 kernel trace holds between the two window markers
+(k_window_mark) of the timed run.
 """
 
 from __future__ import annotations
@@ -44,40 +54,48 @@ def build_circuit(gates: int, seed: int):
 
 
 def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None, autotune=True,
-            nd_leaf=None):
+            nd_leaf=None, window_from=0, replay=None):
     from vajax_amd.engine import AdaptiveConfig
 
     eng.prepare(t_stop=1e-9, dt=1e-12, adaptive_config=AdaptiveConfig(integration_method="gear2"),
                 precision=precision, max_steps=steps + 1, use_graph=use_graph,
-                band_width=band_width, autotune=autotune, nd_leaf=nd_leaf)
+                band_width=band_width, autotune=autotune, nd_leaf=nd_leaf,
+                window_from_step=window_from, replay_times=replay)
     return eng.run_transient()
+
+
+def window_stats(res):
+    st = res.stats
+    k = max(st["window_steps"], 1)
+    el = st["window_time"]
+    return {"accepted_steps": st["window_steps"], "rejected_attempts": st["window_rejected"],
+            "newton_iterations": st["window_nr_iterations"],
+            "newton_iters_per_step": round(st["window_nr_iterations"] / k, 3),
+            "ms_per_step": round(el / k * 1e3, 4),
+            "ms_per_newton_iter": round(el / max(st["window_nr_iterations"], 1) * 1e3, 4),
+            "newton_iters_per_s": round(st["window_nr_iterations"] / el, 3) if el > 0 else None,
+            "loop_s": round(el, 6),
+            "t_window_ps": [round(float(res.times[-1 - st["window_steps"]]) * 1e12, 3),
+                            round(float(res.times[-1]) * 1e12, 3)]}
 
 
 # model parameter columns read by k_eval (the MOSFET's include its init cache)
 N_PAR = {"resistor": 1, "capacitor": 1, "mosfet": 11}
 
 
-def mixed_error(eng, steps: int):
+def mixed_error(eng, ref_res, use_graph):
     """Largest node-voltage difference between the mixed (FP32 factor + FP64
-    refinement) and the plain FP64 runs of the same circuit over ``steps``
-    fixed 1 ps BDF2 steps (one time grid, so the waveforms compare point by
-    point)."""
+    refinement) solution and the plain FP64 run over every step of the FP64
+    run's time grid (the mixed run replays that grid, so the waveforms
+    compare point by point)."""
     import numpy as np
 
-    from vajax_amd.engine import AdaptiveConfig
-
-    V = {}
-    eng.options.fixed_step_transient = True
-    try:
-        for prec in ("f64", "f32"):
-            eng.prepare(t_stop=steps * 1e-12, dt=1e-12, max_steps=steps + 1, precision=prec,
-                        adaptive_config=AdaptiveConfig(integration_method="gear2", tran_fs=1.0),
-                        use_graph=True)
-            V[prec] = eng.run_transient().stats["V_out"]
-    finally:
-        eng.options.fixed_step_transient = False
-    return {"max_abs_dV_vs_fp64": float(np.max(np.abs(V["f32"] - V["f64"]))), "steps": steps,
-            "grid": "fixed 1 ps, gear2"}
+    times = ref_res.times
+    res = run_gpu(eng, len(times) - 1, "f32", use_graph, replay=times)
+    dv = np.abs(res.stats["V_out"] - ref_res.stats["V_out"])
+    return {"max_abs_dV_vs_fp64": float(np.max(dv)), "steps": len(times) - 1,
+            "nonconverged_steps": int(res.stats["rejected_steps"]),
+            "grid": "the FP64 run's accepted grid (gear2 adaptive), replayed"}
 
 
 def kernel_bytes(topo, info, precision: str):
@@ -187,6 +205,7 @@ def cpu_baseline(args):
         "value": round(full["newton_iters_per_s"], 3),
         "unit": "Newton iters/s",
         "ms_per_step": round(full["ms_per_step"], 3),
+        "ms_per_newton_iter": round(full["wall_s"] / max(full["newton_iterations"], 1) * 1e3, 4),
         "cores": full["threads"],
         "kind": "port",
         "sample": (f"C++ restatement of the reference algorithm (oracle/ctran.cpp: OpenMP device "
@@ -288,11 +307,42 @@ def run_batched(args):
     return out
 
 
+def summarize_trace(path):
+    """Markdown table of the kernels in a rocprofv3 kernel-trace CSV between
+    the first and the last k_window_mark dispatch (the timed window)."""
+    import csv
+
+    with open(path, newline="") as fh:
+        rows = list(csv.DictReader(fh))
+    key = "Start_Timestamp" if "Start_Timestamp" in rows[0] else "Start_Time"
+    endk = "End_Timestamp" if "End_Timestamp" in rows[0] else "End_Time"
+    rows.sort(key=lambda r: int(r[key]))
+    marks = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_window_mark")]
+    if len(marks) < 2:
+        raise SystemExit("no window markers in the trace")
+    win = rows[marks[0] + 1:marks[-1]]
+    t0, t1 = int(rows[marks[0]][endk]), int(rows[marks[-1]][key])
+    agg = {}
+    for r in win:
+        name = r["Kernel_Name"].split("(")[0].replace("vjx::(anonymous namespace)::", "")
+        d = int(r[endk]) - int(r[key])
+        a = agg.setdefault(name, [0, 0])
+        a[0] += 1
+        a[1] += d
+    total = sum(v[1] for v in agg.values())
+    print(f"Timed window: {len(win)} dispatches, {(t1 - t0) / 1e6:.3f} ms between the markers, "
+          f"{total / 1e6:.3f} ms of kernel time\n")
+    print("| kernel | calls | total ms | avg us | % of kernel time |")
+    print("|---|---:|---:|---:|---:|")
+    for name, (n, d) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{name}` | {n} | {d / 1e6:.3f} | {d / n / 1e3:.2f} | {100.0 * d / total:.1f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--gates", type=int, default=65000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=("f64", "f32"), default="f64")
@@ -306,14 +356,21 @@ def main():
                     help="also time the numpy oracle (slow) as a secondary CPU figure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
-                    help="skip the mixed-precision waveform-error runs (profiling passes)")
+                    help="skip the mixed-precision runs (profiling passes)")
+    ap.add_argument("--no-eager", action="store_true",
+                    help="skip the eager per-kernel timing pass")
     ap.add_argument("--batch", type=int, default=256, help="Monte Carlo instances (0: skip)")
     ap.add_argument("--batch-gates", type=int, default=2000)
     ap.add_argument("--batch-ragged", default="500,1000,2000,4000")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers (reference kernel for HBM counters)")
+    ap.add_argument("--summarize-trace", default=None,
+                    help="print the timed window of a rocprofv3 kernel_trace.csv and exit")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
     if args.copy_calibration:
         import torch
 
@@ -340,50 +397,60 @@ def main():
     eng = CircuitEngine(net)
     eng.parse()
     topo = eng.topology
-    log(f"setup {time.time() - t0:.1f}s: n_unknowns={topo.n_augmented} nse={topo.nse}")
+    setup_s = time.time() - t0
+    log(f"setup {setup_s:.1f}s: n_unknowns={topo.n_augmented} nse={topo.nse}")
 
     use_graph = not args.no_graph
     t0 = time.time()
-    run_gpu(eng, args.warmup, args.precision, use_graph, args.band_width, nd_leaf=args.nd_leaf)
-    log(f"warm-up ({args.warmup} steps incl. symbolic analysis) {time.time() - t0:.1f}s")
+    run_gpu(eng, 3, args.precision, use_graph, args.band_width, nd_leaf=args.nd_leaf)
+    analysis_s = time.time() - t0
+    log(f"analysis + autotune run {analysis_s:.1f}s")
     lu_info = eng.device.lu.info()
 
+    total = args.warmup + args.steps
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = run_gpu(eng, args.steps, args.precision, use_graph, args.band_width,
-                  nd_leaf=args.nd_leaf)
+    res = run_gpu(eng, total, args.precision, use_graph, args.band_width, nd_leaf=args.nd_leaf,
+                  window_from=args.warmup + 1)
     torch.cuda.synchronize()
     call_s = time.perf_counter() - t0
     st = res.stats
-    steps_done = st["n_steps"] - 1
-    nr = st["total_nr_iterations"]
-    # the step loop's own clock (between stream synchronisations around the K
-    # steps): per-call setup -- buffers, initial state, graph instantiation,
-    # output copies -- is reported beside it, as the CPU baseline's loop clock
-    elapsed = st["wall_time"]
-    ms_per_step = elapsed / max(steps_done, 1) * 1e3
-    log(f"timed: {steps_done} steps, {nr} Newton iterations, loop {elapsed:.3f}s, "
-        f"call {call_s:.3f}s")
+    win = window_stats(res)
+    steps_done = win["accepted_steps"]
+    nr = win["newton_iterations"]
+    elapsed = st["window_time"]
+    ms_per_step = win["ms_per_step"]
+    log(f"timed window: {steps_done} steps, {nr} Newton iterations, {elapsed:.3f}s "
+        f"(whole run {st['n_steps'] - 1} steps in {st['wall_time']:.3f}s, call {call_s:.3f}s)")
 
-    mixed = (mixed_error(eng, max(args.steps, 200))
-             if args.precision == "f64" and not args.no_mixed else None)
-    if mixed:
-        log(f"mixed precision: max |dV| vs FP64 {mixed['max_abs_dV_vs_fp64']:.2e} V")
+    mixed = None
+    if args.precision == "f64" and not args.no_mixed:
+        r32 = run_gpu(eng, total, "f32", use_graph, args.band_width, nd_leaf=args.nd_leaf,
+                      window_from=args.warmup + 1)
+        mixed = {"window": window_stats(r32)}
+        mixed.update(mixed_error(eng, res, use_graph))
+        mixed["mixed_vs_fp64_ms_per_step"] = round(ms_per_step / mixed["window"]["ms_per_step"], 3)
+        mixed["mixed_vs_fp64_ms_per_newton_iter"] = round(
+            win["ms_per_newton_iter"] / mixed["window"]["ms_per_newton_iter"], 3)
+        log(f"mixed: {mixed['window']['ms_per_step']} ms/step, max |dV| vs FP64 "
+            f"{mixed['max_abs_dV_vs_fp64']:.2e} V")
 
-    # eager pass with phase events on the library stream
-    L.lib.vjx_profile_enable(1)
-    L.lib.vjx_profile_reset()
-    run_gpu(eng, args.steps, args.precision, False, args.band_width, nd_leaf=args.nd_leaf)
-    pt = L.PhaseTimes()
-    L.lib.vjx_profile_read(C.byref(pt))
-    L.lib.vjx_profile_enable(0)
-    ph = pt.as_dict()
-    iters = max(ph["n_iterations"], 1)
-    per_iter = {"build": ph["build_ms"] / iters, "lu": ph["lu_ms"] / iters,
-                "solve": ph["solve_ms"] / iters}
-    kb = kernel_bytes(topo, lu_info, args.precision)
+    ph = None
     kernels = {}
-    for name, kt in ph["kernels"].items():
+    if not args.no_eager:
+        # eager pass with per-launch events on the library stream
+        L.lib.vjx_profile_enable(1)
+        L.lib.vjx_profile_reset()
+        run_gpu(eng, total, args.precision, False, args.band_width, nd_leaf=args.nd_leaf)
+        pt = L.PhaseTimes()
+        L.lib.vjx_profile_read(C.byref(pt))
+        L.lib.vjx_profile_enable(0)
+        ph = pt.as_dict()
+    iters = max(ph["n_iterations"], 1) if ph else 1
+    per_iter = ({"build": ph["build_ms"] / iters, "lu": ph["lu_ms"] / iters,
+                 "solve": ph["solve_ms"] / iters} if ph else {})
+    kb = kernel_bytes(topo, lu_info, args.precision)
+    for name, kt in (ph["kernels"].items() if ph else ()):
         if kt["launches"] == 0:
             continue
         avg_ms = kt["ms"] / kt["launches"]
@@ -393,22 +460,34 @@ def main():
                          "avg_launch_us": round(avg_ms * 1e3, 2),
                          "alg_bytes_per_launch": int(per_launch),
                          "achieved_GBs": round(per_launch / (avg_ms * 1e-3) / 1e9, 2)}
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_newton_iter"])
-    d = kernels[dom]
-    traffic = measured_traffic(args, dom)
-    roofline = {
-        "bound": "hbm",
-        "kernel": d["kernel"],
-        "achieved": d["achieved_GBs"],
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6),
-        "traffic": traffic,
-        "avg_launch_us": d["avg_launch_us"],
-        "alg_bytes_per_launch": d["alg_bytes_per_launch"],
-        "phase_ms_per_newton_iter": {k: round(v, 4) for k, v in per_iter.items()},
-        "kernels": kernels,
-    }
+    roofline = None
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["ms_per_newton_iter"])
+        d = kernels[dom]
+        traffic = measured_traffic(args, dom)
+        roofline = {
+            "bound": "hbm",
+            "kernel": d["kernel"],
+            "achieved": d["achieved_GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 6),
+            "traffic": traffic,
+            "avg_launch_us": d["avg_launch_us"],
+            "alg_bytes_per_launch": d["alg_bytes_per_launch"],
+            "phase_ms_per_newton_iter": {k: round(v, 4) for k, v in per_iter.items()},
+            "kernels": kernels,
+        }
+        lt = kernels.get("lu_tasks")
+        if lt:
+            # the band tasks are bound by their chain of dependent pivot
+            # levels, not by bandwidth: time per level of the elimination
+            levels = lu_info["fact_levels"]
+            roofline["level_latency"] = {
+                "kernel": lt["kernel"], "pivot_levels_per_refactor": levels,
+                "launches_per_refactor": lt["launches_per_newton_iter"],
+                "us_per_level": round(lt["ms_per_newton_iter"] * 1e3 / max(levels, 1), 3),
+                "note": "each level waits on one LDS round trip and a workgroup barrier"}
 
     cpu = None
     if not args.no_cpu:
@@ -423,13 +502,13 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": round(nr / elapsed, 3),
+        "value": win["newton_iters_per_s"],
         "unit": "Newton iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "call_ms_per_step": round(call_s / max(steps_done, 1) * 1e3, 4),
+        "ms_per_step": ms_per_step,
+        "call_ms_per_step": round(call_s / max(total, 1) * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -449,8 +528,13 @@ def main():
             "integration": "gear2 adaptive",
             "precision": args.precision,
             "hip_graph": use_graph,
-            "accepted_steps": steps_done,
-            "newton_iterations": nr,
+            "window": win,
+            "whole_run": {"accepted_steps": st["n_steps"] - 1, "rejected": st["rejected_steps"],
+                          "newton_iterations": st["total_nr_iterations"],
+                          "loop_s": round(st["wall_time"], 4)},
+            "setup_s": round(setup_s, 2),
+            "analysis_s": round(analysis_s, 2),
+            "analysis_ms": [round(x, 1) for x in lu_info["analyze_ms"]],
             "parallelism": "single-gpu",
         },
         "roofline": roofline,
@@ -459,6 +543,8 @@ def main():
         "mixed_precision": mixed,
     }
     if cpu:
+        line["speedup_vs_cpu_ms_per_newton_iter"] = round(
+            cpu["ms_per_newton_iter"] / win["ms_per_newton_iter"], 2)
         line["speedup_vs_cpu_ms_per_step"] = round(cpu["ms_per_step"] / ms_per_step, 2)
     print(json.dumps(line))
 

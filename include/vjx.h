@@ -138,6 +138,8 @@ typedef struct {
   double analyze_ms[5];
   int32_t band_width, solve_band_width; /* launch plan in use (solve 0: band_width) */
   double tuned_solve_ms;                /* solve time the autotuner measured (0: not run) */
+  double tuned_factor_ms;               /* refactorisation time it measured (0: not run) */
+  int32_t lu_pipeline;                  /* LU band tasks: 0 by task count, 1 shallow, 2 deep */
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
@@ -160,10 +162,12 @@ int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width);
  * caches one symbolic analysis per pattern); a new value drops the analysis
  * like a new band width. */
 int vjx_pattern_set_ordering(vjx_pattern* pat, int32_t ordering, int32_t nd_leaf);
-/* Autotuner of the solves' launch plan: with the matrix data_dev (device),
- * factorise, then time the triangular solves for band widths 8..48 on the
- * device and keep the fastest (results do not depend on the width).  No
- * counterpart in the reference (a launch-plan choice). */
+/* Autotuner of the launch plan: with the matrix data_dev (device), time the
+ * numeric refactorisation for refactorisation bands of 12, 16 and 24 pivot
+ * levels (a fresh analysis each: same ordering and pivots) and each LU
+ * band-task pipeline, keep the fastest, then time the triangular solves for
+ * band widths 8..64 and keep the fastest (solve results do not depend on
+ * the width).  No counterpart in the reference (a launch-plan choice). */
 int vjx_lu_autotune(vjx_pattern* pat, const double* data_dev);
 /* Factorise / solve on the CPU with the same numeric program as the GPU path. */
 int vjx_lu_refactor_cpu(vjx_pattern* pat, const double* data_host, int32_t precision,
@@ -322,6 +326,40 @@ int vjx_model_eval(int32_t model_id, int32_t count, const double* vbranch_dev /*
                    double* jac_react_dev, double* lim_rhs_resist_dev, double* lim_rhs_react_dev,
                    double* limit_state_out_dev);
 
+/* The openvaf_jax eval_fn contract in full (openvaf_jax/codegen/
+ * function_builder.py:967-996, vmapped with in_axes = (None, 0, None, 0,
+ * None, 0) at vajax/analysis/openvaf_models.py:996-997): one model group's
+ * inputs come split into arrays shared by every device and per-device rows,
+ * as vajax/analysis/openvaf_models.py:668-1056 prepares them.  The layout
+ * says where the eval finds what it reads: each branch voltage is a column of
+ * device_params (the reference's voltage_positions) and each value of the
+ * model's eval input block (the rows of vjx_model_desc.params: the init
+ * cache) comes from shared_cache (src >= 0) or a device_cache column
+ * (src = ~column).  simparams[mfactor_index] is $mfactor, which scales every
+ * residual, Jacobian and lim_rhs output (-1: 1); the models here read no
+ * other simparam and no model parameter outside their cache, so
+ * shared_params may be NULL.  Host layout arrays; device data arrays. */
+typedef struct {
+  int32_t n_device_params;          /* columns of device_params per device */
+  int32_t n_device_cache;           /* columns of device_cache per device */
+  const int32_t* voltage_positions; /* [n_v] device_params column of each branch voltage */
+  const int32_t* cache_src;         /* [n_par] */
+  int32_t mfactor_index;
+} vjx_eval_split;
+
+/* Outputs row-major per device ([count][n_res], [count][n_jac],
+ * [count][n_lim]); all nine are written: lim_rhs zero for models without
+ * $limit, small-signal residuals zero (these models have no small-signal-only
+ * contributions), limit_state_out empty for models without limit states. */
+int vjx_model_eval_split(int32_t model_id, int32_t count, const vjx_eval_split* split,
+                         const double* shared_params_dev, const double* device_params_dev,
+                         const double* shared_cache_dev, const double* device_cache_dev,
+                         const double* simparams_dev, const double* limit_state_in_dev,
+                         double* res_resist_dev, double* res_react_dev, double* jac_resist_dev,
+                         double* jac_react_dev, double* lim_rhs_resist_dev,
+                         double* lim_rhs_react_dev, double* ss_resist_dev, double* ss_react_dev,
+                         double* limit_state_out_dev);
+
 /* The limiting functions, elementwise on device arrays (vajax/analysis/
  * limiting.py:22-107): kind 0 = pnjlim(vnew, vold, vt = p1, vcrit = p2),
  * kind 1 = fetlim(vnew, vold, vto = p1). */
@@ -379,6 +417,18 @@ typedef struct {
    * k > 0 keeps k rows and copies them to the host result buffers each time
    * they fill (the checkpoint mode of full_mna.py:1443-1699) */
   int32_t output_chunk;
+  /* replay (adaptive loop only; n_replay > 1): take exactly the steps to
+   * replay_times_host[1 .. n_replay-1] (accepted times of another run, [0] =
+   * 0), with the adaptive loop's coefficients, predictor and history but no
+   * accept/reject decisions; rejected_steps then counts replayed steps whose
+   * Newton solve did not converge.  A test harness for comparing waveforms
+   * on a common time grid; no counterpart in the reference. */
+  const double* replay_times_host;
+  int32_t n_replay;
+  /* measurement window: statistics of the window below are gathered from
+   * the first step attempt at which the accepted-step count reaches
+   * window_from_step (0: the whole loop) */
+  int32_t window_from_step;
 } vjx_tran_config;
 
 typedef struct {
@@ -397,6 +447,13 @@ typedef struct {
   double wall_time;  /* seconds, time loop only (setup and DC excluded) */
   double dc_time;
   int32_t repivots;
+  /* the measurement window (window_from_step): accepted steps, rejected
+   * attempts, Newton iterations and seconds between stream synchronisations
+   * at its start and at the end of the loop */
+  int32_t window_steps;
+  int32_t window_rejected;
+  int64_t window_nr_iterations;
+  double window_time;
 } vjx_tran_result;
 
 int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx_tran_result* res);

@@ -342,6 +342,7 @@ def run_adaptive(cir, t_stop: float, dt: float, cfg: AdaptiveConfig, opts: Optio
     V_max_hist = np.abs(X[:n_total])
     lim = np.zeros(cir.n_lim)
     method = method_of(cfg.integration_method)
+    trace = []  # (t_next, dt, Newton iterations) of every step attempt
     t0 = time.perf_counter()
     iters = 0
     while t < t_stop and step_idx < max_steps:
@@ -385,6 +386,7 @@ def run_adaptive(cir, t_stop: float, dt: float, cfg: AdaptiveConfig, opts: Optio
         r = nr_solve(cir, solver, opts, X_init, vs, isv, Q_prev, c0, gshunt, c1, d1, dQdt_prev, c2,
                      Q_prev2, res_tol_floor=floor, lim_in=lim)
         total_nr += r.iterations
+        trace.append((t_next, dt_cur, r.iterations))
         nr_failed = not r.converged
         at_min = dt_cur <= cfg.min_dt
         fail_min = nr_failed and at_min
@@ -445,6 +447,7 @@ def run_adaptive(cir, t_stop: float, dt: float, cfg: AdaptiveConfig, opts: Optio
         "min_dt_used": min_dt_used,
         "max_dt_used": max_dt_used,
         "body_iterations": iters,
+        "trace": trace,
     }
     return TransientOut(np.array(times), np.array(Vout), np.array(Iout), stats)
 

@@ -36,7 +36,8 @@ def run_oracle(net, t_stop, dt, fixed=False, tran_fs=None, dense=False, max_step
 
 def run_gpu(net, t_stop, dt, fixed=False, tran_fs=None, precision="f64", refine_steps=None,
             use_graph=False, band_width=None, autotune=False, max_steps=None, tran_itl=None,
-            repivot_growth=None, checkpoint_interval=None, **over):
+            repivot_growth=None, checkpoint_interval=None, replay_times=None,
+            window_from_step=0, **over):
     from vajax_amd.engine import AdaptiveConfig, CircuitEngine
 
     eng = CircuitEngine(net)
@@ -53,7 +54,8 @@ def run_gpu(net, t_stop, dt, fixed=False, tran_fs=None, precision="f64", refine_
         setattr(cfg, k, v)
     eng.prepare(t_stop=t_stop, dt=dt, adaptive_config=cfg, precision=precision,
                 refine_steps=refine_steps, use_graph=use_graph, band_width=band_width,
-                autotune=autotune, max_steps=max_steps, checkpoint_interval=checkpoint_interval)
+                autotune=autotune, max_steps=max_steps, checkpoint_interval=checkpoint_interval,
+                replay_times=replay_times, window_from_step=window_from_step)
     return eng, eng.run_transient()
 
 

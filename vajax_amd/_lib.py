@@ -20,10 +20,6 @@ VJX_ERR_ARG = -1
 VJX_ERR_HIP = -2
 VJX_ERR_SINGULAR = -3
 VJX_ERR_STATE = -4
-VJX_ERR_ARG = -1
-VJX_ERR_HIP = -2
-VJX_ERR_SINGULAR = -3
-VJX_ERR_STATE = -4
 VJX_PREC_F64 = 0
 VJX_PREC_F32 = 1
 VJX_METHOD = {"be": 0, "trap": 1, "gear2": 2}
@@ -51,7 +47,8 @@ class LuInfo(C.Structure):
         ("fact_task_targets", C.c_int64), ("fact_schur_targets", C.c_int64),
         ("analyze_ms", C.c_double * 5),
         ("band_width", C.c_int32), ("solve_band_width", C.c_int32),
-        ("tuned_solve_ms", C.c_double),
+        ("tuned_solve_ms", C.c_double), ("tuned_factor_ms", C.c_double),
+        ("lu_pipeline", C.c_int32),
     ]
 
     def as_dict(self):
@@ -88,6 +85,11 @@ class CircuitDesc(C.Structure):
     ]
 
 
+class EvalSplit(C.Structure):
+    _fields_ = [("n_device_params", C.c_int32), ("n_device_cache", C.c_int32),
+                ("voltage_positions", i32p), ("cache_src", i32p), ("mfactor_index", C.c_int32)]
+
+
 class TranConfig(C.Structure):
     _fields_ = [
         ("t_stop", C.c_double), ("dt", C.c_double),
@@ -106,6 +108,8 @@ class TranConfig(C.Structure):
         ("autotune", C.c_int32),
         ("repivot_growth", C.c_double),
         ("output_chunk", C.c_int32),
+        ("replay_times_host", f64p), ("n_replay", C.c_int32),
+        ("window_from_step", C.c_int32),
     ]
 
 
@@ -118,6 +122,8 @@ class TranResult(C.Structure):
         ("min_dt_used", C.c_double), ("max_dt_used", C.c_double),
         ("wall_time", C.c_double), ("dc_time", C.c_double),
         ("repivots", C.c_int32),
+        ("window_steps", C.c_int32), ("window_rejected", C.c_int32),
+        ("window_nr_iterations", C.c_int64), ("window_time", C.c_double),
     ]
 
 
@@ -186,6 +192,7 @@ _PROTOS = {
     "vjx_build_system": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp,
                                    vp, vp, vp, vp]),
+    "vjx_model_eval_split": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(EvalSplit)] + [vp] * 15),
     "vjx_model_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "vjx_eval_stamp": (C.c_int, [vp, C.c_int32, vp, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp,
                                  vp, vp]),

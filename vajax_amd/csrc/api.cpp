@@ -132,6 +132,8 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->band_width = pat->opts.band_width;
   o->solve_band_width = pat->opts.solve_band_width;
   o->tuned_solve_ms = pat->tuned_solve_ms;
+  o->tuned_factor_ms = pat->tuned_factor_ms;
+  o->lu_pipeline = pat->lu.lu_deep;
   o->fact_task_updates = o->fact_schur_updates = o->fact_task_targets = o->fact_schur_targets = 0;
   if (!s.fact_band.band_w.empty()) {
     std::vector<char> schur(s.fact_band.items.size(), 0);
@@ -184,6 +186,8 @@ int vjx_lu_autotune(vjx_pattern* pat, const double* data_dev) {
     return VJX_ERR_ARG;
   }
   int rc = pattern_ensure_analyzed(pat, data_dev);
+  if (rc < 0) return rc;
+  rc = pattern_autotune_factor(pat, data_dev);
   if (rc < 0) return rc;
   rc = pattern_autotune_solves(pat, data_dev);
   if (rc < 0) return rc;

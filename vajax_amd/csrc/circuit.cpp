@@ -439,6 +439,61 @@ int vjx_model_eval(int32_t model_id, int32_t count, const double* vb, const doub
   return VJX_OK;
 }
 
+int vjx_model_eval_split(int32_t model_id, int32_t count, const vjx_eval_split* sp,
+                         const double* shared_params, const double* dp, const double* sc,
+                         const double* dc, const double* simparams, const double* li, double* rr,
+                         double* rc, double* jr, double* jc, double* lr, double* lc, double* sr,
+                         double* sq, double* lo) {
+  (void)shared_params;  // no model here reads a parameter outside its cache
+  if (model_id < 0 || model_id >= VJX_MODEL_COUNT || count < 0 || !sp) {
+    set_error("vjx_model_eval_split: bad model id, count or layout");
+    return VJX_ERR_ARG;
+  }
+  const int n_v = model_n_v(model_id), n_par = model_n_par(model_id);
+  const int n_res = model_n_res(model_id), n_jac = model_n_jac(model_id);
+  const int n_lim = model_n_lim(model_id);
+  if (count == 0) return VJX_OK;
+  if (!dp || !rr || !rc || !jr || !jc || !lr || !lc || !sr || !sq || (n_lim && !lo) ||
+      !sp->voltage_positions || !sp->cache_src || (sp->mfactor_index >= 0 && !simparams)) {
+    set_error("vjx_model_eval_split: null argument");
+    return VJX_ERR_ARG;
+  }
+  bool need_sc = false, need_dc = false;
+  for (int k = 0; k < n_v; ++k)
+    if (sp->voltage_positions[k] < 0 || sp->voltage_positions[k] >= sp->n_device_params) {
+      set_error("vjx_model_eval_split: voltage position outside device_params");
+      return VJX_ERR_ARG;
+    }
+  for (int j = 0; j < n_par; ++j) {
+    const int c = sp->cache_src[j];
+    if (c < 0 && ~c >= sp->n_device_cache) {
+      set_error("vjx_model_eval_split: cache column outside device_cache");
+      return VJX_ERR_ARG;
+    }
+    need_sc = need_sc || c >= 0;
+    need_dc = need_dc || c < 0;
+  }
+  if ((need_sc && !sc) || (need_dc && !dc)) {
+    set_error("vjx_model_eval_split: null cache array");
+    return VJX_ERR_ARG;
+  }
+  DBuf<int32_t> vpos, csrc;
+  DBuf<double> vb, par;
+  VJX_TRY_HIP(vpos.upload(sp->voltage_positions, n_v));
+  VJX_TRY_HIP(csrc.upload(sp->cache_src, n_par));
+  VJX_TRY_HIP(vb.alloc((size_t)n_v * count));
+  VJX_TRY_HIP(par.alloc((size_t)n_par * count));
+  launch_split_gather(count, n_v, n_par, sp->n_device_params, sp->n_device_cache, vpos.p, csrc.p,
+                      sc, dp, dc, vb.p, par.p, 0);
+  launch_model_eval_raw(model_id, count, vb.p, par.p, li, rr, rc, jr, jc, lr, lc, lo, 0);
+  launch_split_finish(count, n_res, n_jac, n_lim > 0 ? 1 : 0,
+                      sp->mfactor_index >= 0 ? simparams + sp->mfactor_index : nullptr, rr, rc,
+                      jr, jc, lr, lc, sr, sq, 0);
+  VJX_TRY_HIP(hipGetLastError());
+  VJX_TRY_HIP(hipDeviceSynchronize());
+  return VJX_OK;
+}
+
 int vjx_limit_eval(int32_t kind, int32_t n, const double* vnew, const double* vold, double p1,
                    double p2, double* out) {
   if ((kind != 0 && kind != 1) || n < 0 || (n && (!vnew || !vold || !out))) {

@@ -21,7 +21,8 @@ struct vjx_pattern {
   vjx::DevStatus* h_status = nullptr;  // pinned
   double analyze_seconds = 0.0;
   int n_analyses = 0;
-  double tuned_solve_ms = 0.0;  // solve time at the width the autotuner kept
+  double tuned_solve_ms = 0.0;   // solve time at the width the autotuner kept
+  double tuned_factor_ms = 0.0;  // refactorisation time of the plan it kept
   // transposed solves: a second handle for the pattern of A^T and, for each
   // of its entries, the position of the same value in A's data
   vjx_pattern* trans = nullptr;
@@ -76,6 +77,8 @@ int pattern_refactor_async(vjx_pattern* pat, const double* data_dev, int prec, b
 int pattern_check_pivots(vjx_pattern* pat);
 // Pick the solves' band width by timing candidates on the device.
 int pattern_autotune_solves(vjx_pattern* pat, const double* data_dev);
+// Pick the refactorisation's band width and band-task pipeline by timing them.
+int pattern_autotune_factor(vjx_pattern* pat, const double* data_dev);
 // Solve A x = scale*rhs (scale = +1 or -1) with the current factors; for FP32
 // factors run refine_steps FP64 residual corrections.
 int pattern_solve_async(vjx_pattern* pat, const double* data_dev, const double* rhs, double scale,

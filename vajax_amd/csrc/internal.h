@@ -256,6 +256,12 @@ void launch_axpy(int n, double* x, const double* d, hipStream_t s);
 void launch_model_eval_raw(int model_id, int count, const double* vb, const double* par,
                            const double* li, double* rr, double* rc, double* jr, double* jc,
                            double* lr, double* lc, double* lo, hipStream_t s);
+void launch_split_gather(int N, int n_v, int n_par, int n_dp, int n_dc, const int32_t* vpos,
+                         const int32_t* csrc, const double* sc, const double* dp, const double* dc,
+                         double* vb, double* par, hipStream_t s);
+void launch_split_finish(int N, int n_res, int n_jac, int limited, const double* mf, double* rr,
+                         double* rc, double* jr, double* jc, double* lr, double* lc, double* sr,
+                         double* sq, hipStream_t s);
 void launch_limit(int kind, int n, const double* vnew, const double* vold, double p1, double p2,
                   double* out, hipStream_t s);
 void launch_lim_stats(int n, const double* ls_in, const double* ls_out, DevStatus* st,

@@ -329,6 +329,47 @@ __global__ void __launch_bounds__(256) k_eval(int N, const int32_t* __restrict__
   }
 }
 
+// Inputs of the split eval contract gathered into the raw eval's layout:
+// branch voltages [n_v][N] from device_params columns, the eval input block
+// [n_par][N] from shared_cache or device_cache columns.
+__global__ void k_split_gather(int N, int n_v, int n_par, int n_dp, int n_dc,
+                               const int32_t* __restrict__ vpos, const int32_t* __restrict__ csrc,
+                               const double* __restrict__ sc, const double* __restrict__ dp,
+                               const double* __restrict__ dc, double* vb, double* par) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int k = 0; k < n_v; ++k) vb[(size_t)k * N + i] = dp[(size_t)i * n_dp + vpos[k]];
+  for (int j = 0; j < n_par; ++j) {
+    const int c = csrc[j];
+    par[(size_t)j * N + i] = c >= 0 ? sc[c] : dc[(size_t)i * n_dc + ~c];
+  }
+}
+
+// $mfactor on every residual, Jacobian and lim_rhs output; lim_rhs zero for
+// models without $limit (the raw eval does not write them); small-signal
+// residuals zero.
+__global__ void k_split_finish(int N, int n_res, int n_jac, int limited, const double* mf,
+                               double* rr, double* rc, double* jr, double* jc, double* lr,
+                               double* lc, double* sr, double* sq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double m = mf ? *mf : 1.0;
+  for (int r = 0; r < n_res; ++r) {
+    const size_t e = (size_t)i * n_res + r;
+    rr[e] *= m;
+    rc[e] *= m;
+    lr[e] = limited ? lr[e] * m : 0.0;
+    lc[e] = limited ? lc[e] * m : 0.0;
+    sr[e] = 0.0;
+    sq[e] = 0.0;
+  }
+  for (int j = 0; j < n_jac; ++j) {
+    const size_t e = (size_t)i * n_jac + j;
+    jr[e] *= m;
+    jc[e] *= m;
+  }
+}
+
 // Raw eval (openvaf_jax contract), row-major outputs per instance.
 template <int MID>
 __global__ void k_model_eval(int N, const double* __restrict__ vbr, const double* __restrict__ par,
@@ -1623,6 +1664,19 @@ __global__ void k_limit(int kind, int n, const double* __restrict__ vnew,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   out[i] = kind == 0 ? pnjlim(vnew[i], vold[i], p1, p2) : fetlim(vnew[i], vold[i], p1);
+}
+
+void launch_split_gather(int N, int n_v, int n_par, int n_dp, int n_dc, const int32_t* vpos,
+                         const int32_t* csrc, const double* sc, const double* dp, const double* dc,
+                         double* vb, double* par, hipStream_t s) {
+  if (N > 0) k_split_gather<<<nblk(N, 256), 256, 0, s>>>(N, n_v, n_par, n_dp, n_dc, vpos, csrc, sc,
+                                                         dp, dc, vb, par);
+}
+void launch_split_finish(int N, int n_res, int n_jac, int limited, const double* mf, double* rr,
+                         double* rc, double* jr, double* jc, double* lr, double* lc, double* sr,
+                         double* sq, hipStream_t s) {
+  if (N > 0) k_split_finish<<<nblk(N, 256), 256, 0, s>>>(N, n_res, n_jac, limited, mf, rr, rc, jr,
+                                                         jc, lr, lc, sr, sq);
 }
 
 void launch_limit(int kind, int n, const double* vnew, const double* vold, double p1, double p2,

@@ -201,8 +201,9 @@ int pattern_upload(vjx_pattern* pat) {
   // VJX_LU_DEEP=1/2 forces the shallow / deep LU band pipeline (A/B
   // measurements); by default bands with more than two tasks per CU take the
   // shallow one
+  // (when unset the pipeline stays what the autotuner or the default chose)
   const char* deep_env = std::getenv("VJX_LU_DEEP");
-  L.lu_deep = deep_env && (deep_env[0] == '1' || deep_env[0] == '2') ? deep_env[0] - '0' : 0;
+  if (deep_env && deep_env[0] >= '0' && deep_env[0] <= '2') L.lu_deep = deep_env[0] - '0';
   // Solve band tasks run from LDS when everything they read fits the
   // device's LDS per workgroup (less the kernel's static arrays), else from
   // HBM; VJX_SOLVE_LDS=0 keeps every task in HBM (A/B measurements)
@@ -315,6 +316,84 @@ int pattern_autotune_solves(vjx_pattern* pat, const double* data_dev) {
   VJX_TRY_HIP(upload_band(L.fwd_band, L.sym.fwd_band));
   VJX_TRY_HIP(upload_band(L.bwd_band, L.sym.bwd_band));
   pat->tuned_solve_ms = best_ms / 5.0;
+  return VJX_OK;
+}
+
+// Autotuner of the refactorisation's launch plan: times the numeric
+// refactorisation on the device for each LU band-task pipeline (chosen per
+// band by its task count, shallow and low-register, or deep prefetch: the same
+// sums in the same order) and for refactorisation bands of 12, 16 and 24
+// pivot levels (each a fresh analysis of the same matrix: the same ordering
+// and pivots, a different split of the updates between band tasks and Schur
+// steps, which changes the factors at rounding level), and keeps the fastest
+// pair.
+int pattern_autotune_factor(vjx_pattern* pat, const double* data_dev) {
+  DevLU& L = pat->lu;
+  int rc = pattern_ensure_analyzed(pat, data_dev);
+  if (rc < 0) return rc;
+  if (pat->opts.band_width <= 0) return VJX_OK;
+  hipStream_t s = pat->stream;
+  const int prec = L.factored_prec < 0 ? VJX_PREC_F64 : L.factored_prec;
+  hipEvent_t ev[2];
+  VJX_TRY_HIP(hipEventCreate(&ev[0]));
+  VJX_TRY_HIP(hipEventCreate(&ev[1]));
+  auto time_refactor = [&](float& ms) -> int {
+    int r = pattern_refactor_async(pat, data_dev, prec, true);  // warm-up
+    if (r < 0) return r;
+    VJX_TRY_HIP(hipEventRecord(ev[0], s));
+    for (int k = 0; k < 5; ++k) {
+      r = pattern_refactor_async(pat, data_dev, prec, true);
+      if (r < 0) return r;
+    }
+    VJX_TRY_HIP(hipEventRecord(ev[1], s));
+    VJX_TRY_HIP(hipEventSynchronize(ev[1]));
+    VJX_TRY_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    ms /= 5.0f;
+    return VJX_OK;
+  };
+  std::vector<double> host(pat->nnz);
+  if (pat->nnz)
+    VJX_TRY_HIP(hipMemcpy(host.data(), data_dev, sizeof(double) * pat->nnz, hipMemcpyDeviceToHost));
+  const int width0 = pat->opts.band_width;
+  int best_w = width0, best_deep = L.lu_deep;
+  float best_ms = -1.f;
+  const int widths[] = {width0, 12, 16, 24};
+  for (int wi = 0; wi < 4 && rc >= 0; ++wi) {
+    const int w = widths[wi];
+    if (wi > 0 && w == width0) continue;
+    if (w != pat->opts.band_width) {
+      pat->opts.band_width = w;
+      rc = vjx_lu_analyze(pat, host.data());
+      if (rc < 0) break;
+      rc = pattern_upload(pat);
+      if (rc < 0) break;
+    }
+    for (int deep = 0; deep <= 2 && rc >= 0; ++deep) {
+      L.lu_deep = deep;
+      float ms = 0.f;
+      rc = time_refactor(ms);
+      // another band width changes the order of the Schur sums (rounding of
+      // the factors), so it must beat the current one by 3 % to be kept
+      if (w != best_w) ms *= 1.03f;
+      if (rc >= 0 && (best_ms < 0.f || ms < best_ms)) {
+        best_ms = ms;
+        best_w = w;
+        best_deep = deep;
+      }
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc < 0) return rc;
+  if (pat->opts.band_width != best_w) {
+    pat->opts.band_width = best_w;
+    rc = vjx_lu_analyze(pat, host.data());
+    if (rc < 0) return rc;
+    rc = pattern_upload(pat);
+    if (rc < 0) return rc;
+  }
+  L.lu_deep = best_deep;
+  pat->tuned_factor_ms = best_ms;
   return VJX_OK;
 }
 

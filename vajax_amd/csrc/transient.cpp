@@ -143,6 +143,8 @@ struct Driver {
     VJX_TRY_HIP(hipStreamSynchronize(s));
     int rc = pattern_ensure_analyzed(pat, w.csr.p);
     if (rc < 0 || !cfg->autotune) return rc;
+    rc = pattern_autotune_factor(pat, w.csr.p);
+    if (rc < 0) return rc;
     return pattern_autotune_solves(pat, w.csr.p);
   }
 
@@ -324,6 +326,15 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     return VJX_ERR_ARG;
   }
   ktimer_enable(false);
+  if (cfg->n_replay > 1) {
+    bool ok = cfg->replay_times_host != nullptr && !cfg->fixed_step;
+    for (int i = 1; ok && i < cfg->n_replay; ++i)
+      ok = cfg->replay_times_host[i] >= cfg->replay_times_host[i - 1];
+    if (!ok) {
+      set_error("vjx_transient_run: replay needs a non-decreasing time grid and the adaptive loop");
+      return VJX_ERR_ARG;
+    }
+  }
   if (cfg->max_steps < 2 || !(cfg->dt > 0.0) || cfg->tran_itl < 1 ||
       (cfg->precision != VJX_PREC_F64 && cfg->precision != VJX_PREC_F32)) {
     set_error("vjx_transient_run: invalid configuration");
@@ -474,6 +485,25 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   int step_idx = 1;
   int64_t total_nr = 0;
   int rejected = 0;
+  int replay_nonconv = 0;
+  // measurement window: opened (after a stream synchronisation) at the first
+  // step attempt whose accepted-step count reaches window_from_step
+  struct Window {
+    bool open = false;
+    int step0 = 0, rej0 = 0;
+    int64_t nr0 = 0;
+    std::chrono::steady_clock::time_point t0;
+    int start(hipStream_t st, int step, int rej, int64_t nr, bool mark) {
+      if (mark) launch_window_mark(0, st);
+      VJX_TRY_HIP(hipStreamSynchronize(st));
+      open = true;
+      step0 = step;
+      rej0 = rej;
+      nr0 = nr;
+      t0 = std::chrono::steady_clock::now();
+      return VJX_OK;
+    }
+  } window;
   double t = 0.0;
   double dt = cfg->dt;
   auto t0 = std::chrono::steady_clock::now();
@@ -496,6 +526,10 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     }
     launch_fill(n_unk, cfg->abstol, w.floor.p, s);
     for (int i = 0; i < n_steps; ++i) {
+      if (i + 1 >= cfg->window_from_step && !window.open) {
+        rc = window.start(s, i + 1, 0, total_nr, cfg->window_from_step > 0);
+        if (rc < 0) return rc;
+      }
       const double t_next = ((double)i + 1.0) * dt;
       rc = D.set_sources(t_next);
       if (rc < 0) return rc;
@@ -534,7 +568,14 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     int hist_count = 1, warmup = 0, consec = 0;
     double min_dt_used = dt, max_dt_used = dt;
     auto hist_ptr = [&](int j) { return w.hist.p + (size_t)slot[j] * n_total; };
-    while (t < cfg->t_stop && step_idx < cfg->max_steps) {
+    const bool replay = cfg->n_replay > 1 && cfg->replay_times_host;
+    if (replay) dt = cfg->replay_times_host[1] - cfg->replay_times_host[0];
+    while (replay ? step_idx < std::min(cfg->n_replay, cfg->max_steps)
+                  : (t < cfg->t_stop && step_idx < cfg->max_steps)) {
+      if (step_idx >= cfg->window_from_step && !window.open) {
+        rc = window.start(s, step_idx, rejected, total_nr, cfg->window_from_step > 0);
+        if (rc < 0) return rc;
+      }
       const double dt_cur = dt;
       if (!(dt_cur > 0.0)) {
         // full_mna.py:2076 clamps the next dt to t_stop - t_next also after a
@@ -557,6 +598,8 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
         min_dt_used = std::min(min_dt_used, dt_cur);
         max_dt_used = std::max(max_dt_used, dt_cur);
         dt = new_dt;
+        if (replay && step_idx < cfg->n_replay)
+          dt = cfg->replay_times_host[step_idx] - cfg->replay_times_host[step_idx - 1];
         continue;
       }
       const double inv = 1.0 / dt_cur;
@@ -575,7 +618,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
         c2 = (om * om) / (op1 * dt_cur);
         e_int = -2.0 / 9.0;
       }
-      const double t_next = t + dt_cur;
+      const double t_next = replay ? cfg->replay_times_host[step_idx] : t + dt_cur;
       rc = D.set_sources(t_next);
       if (rc < 0) return rc;
       const bool can_predict = (warmup >= cfg->warmup_steps) && (hist_count >= 2);
@@ -622,7 +665,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       total_nr += iters;
       const bool nr_failed = !conv;
       const bool at_min = dt_cur <= cfg->min_dt;
-      const bool fail_min = nr_failed && at_min;
+      bool fail_min = nr_failed && at_min;
       // LTE (adaptive.py:221-339)
       launch_status_reset(w.st.p, s);
       launch_lte(n_total, w.Xw.p, w.Vp.p, w.Vmax.p, e_int / (e_int - pred_err), cfg->reltol,
@@ -637,8 +680,8 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       const bool lte_reject = (dt_cur / dt_lte > cfg->redo_factor) && can_predict && conv;
       const bool nr_reject = nr_failed && !at_min;
       const bool force = consec >= cfg->max_consecutive_rejects;
-      const bool reject = (lte_reject || nr_reject) && !force;
-      const bool accept = !reject;
+      const bool reject = (lte_reject || nr_reject) && !force && !replay;
+      bool accept = !reject;
       if (accept)
         consec = 0;
       else if (lte_reject)
@@ -650,7 +693,16 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       double new_dt = nr_failed ? (at_min ? dt_rec : dt_halved) : dt_from_lte;
       new_dt = std::min(std::max(new_dt, cfg->min_dt), max_dt);
       new_dt = std::min(new_dt, cfg->t_stop - t_next);
-      const bool use_new = accept && !fail_min;
+      bool use_new = accept && !fail_min;
+      if (replay) {
+        // the step is the replayed grid's: taken whatever the LTE says
+        accept = true;
+        use_new = true;
+        fail_min = false;
+        if (nr_failed) ++replay_nonconv;
+        if (step_idx + 1 < cfg->n_replay)
+          new_dt = cfg->replay_times_host[step_idx + 1] - cfg->replay_times_host[step_idx];
+      }
       if (accept) t = t_next;
       if (use_new) {
         // the limit state advances with the new solution (full_mna.py:2162-2167)
@@ -681,11 +733,24 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
     res->min_dt_used = min_dt_used;
     res->max_dt_used = max_dt_used;
   }
+  if (window.open && cfg->window_from_step > 0) launch_window_mark(1, s);
   VJX_TRY_HIP(hipGetLastError());
   VJX_TRY_HIP(hipStreamSynchronize(s));
-  res->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const auto t_end = std::chrono::steady_clock::now();
+  res->wall_time = std::chrono::duration<double>(t_end - t0).count();
   res->n_steps = step_idx;
-  res->rejected_steps = rejected;
+  res->rejected_steps = (cfg->n_replay > 1 && cfg->replay_times_host) ? replay_nonconv : rejected;
+  if (window.open) {
+    res->window_steps = step_idx - window.step0;
+    res->window_rejected = rejected - window.rej0;
+    res->window_nr_iterations = total_nr - window.nr0;
+    res->window_time = std::chrono::duration<double>(t_end - window.t0).count();
+  } else {
+    res->window_steps = 0;
+    res->window_rejected = 0;
+    res->window_nr_iterations = 0;
+    res->window_time = 0.0;
+  }
   res->total_nr_iterations = total_nr;
   res->total_evals = D.total_evals;
   res->repivots = D.repivots;

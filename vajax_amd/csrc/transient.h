@@ -21,6 +21,7 @@ void launch_noi_clamp(int nd, const int32_t* rows, double* X, hipStream_t s);
 void launch_copy(int n, const double* a, double* b, hipStream_t s);
 // out[0] = *a and, when b is given, out[1] = *b (out: mapped pinned host memory)
 void launch_status_export(const DevStatus* a, const DevStatus* b, DevStatus* out, hipStream_t s);
+void launch_window_mark(int which, hipStream_t s);
 // b[i] = a[idx[i]], i < n
 void launch_gather(int n, const int32_t* idx, const double* a, double* b, hipStream_t s);
 void launch_dqdt(int n, double c0, double c1, double d1, double c2, const double* Q,

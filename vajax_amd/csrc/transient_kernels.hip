@@ -109,6 +109,10 @@ __global__ void k_status_export(const DevStatus* __restrict__ a, const DevStatus
   if (b) out[1] = *b;
 }
 
+// Marks the boundaries of a measurement window in kernel traces (a profiler
+// keeps the dispatches between two of these); it does no work.
+__global__ void k_window_mark(int which) { (void)which; }
+
 __global__ void k_gather(int n, const int32_t* __restrict__ idx, const double* __restrict__ a,
                          double* b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -229,6 +233,7 @@ void launch_fill(int n, double v, double* x, hipStream_t s) {
 void launch_copy(int n, const double* a, double* b, hipStream_t s) {
   if (n) k_copy<<<nb(n), 256, 0, s>>>(n, a, b);
 }
+void launch_window_mark(int which, hipStream_t s) { k_window_mark<<<1, 64, 0, s>>>(which); }
 void launch_status_export(const DevStatus* a, const DevStatus* b, DevStatus* out, hipStream_t s) {
   k_status_export<<<1, 64, 0, s>>>(a, b, out);
 }

@@ -488,12 +488,16 @@ class CircuitEngine:
                 refine_steps: Optional[int] = None, max_steps: Optional[int] = None,
                 use_graph: bool = False, band_width: Optional[int] = None,
                 autotune: bool = False, nd_leaf: Optional[int] = None,
-                checkpoint_interval: Optional[int] = None, **_ignored) -> None:
+                checkpoint_interval: Optional[int] = None, replay_times=None,
+                window_from_step: int = 0, **_ignored) -> None:
         """``checkpoint_interval``: output rows kept on the device before they
         are copied to the host (full_mna.py:784-1000 checkpoint mode): None
         keeps all rows resident unless they would take more than
         ``OUTPUT_RESIDENT_BYTES``, then chunks of that size; <= 0 keeps all
-        rows resident; k > 0 chunks of k rows."""
+        rows resident; k > 0 chunks of k rows.  ``replay_times``: accepted
+        times of another run to step through exactly (test harness, adaptive
+        loop).  ``window_from_step``: the statistics ``window_*`` cover the
+        steps from this accepted-step count on."""
         if self.topology is None:
             self.parse()
         if nd_leaf is not None:
@@ -535,7 +539,11 @@ class CircuitEngine:
             chunk = 0 if fits else max(1, OUTPUT_RESIDENT_BYTES // row_bytes)
         else:
             chunk = max(int(checkpoint_interval), 0)
-        self._prep = dict(output_chunk=int(chunk))
+        self._prep = dict(output_chunk=int(chunk), window_from_step=int(window_from_step))
+        if replay_times is not None:
+            rt = np.ascontiguousarray(replay_times, dtype=np.float64)
+            self._prep["replay_times"] = rt
+            max_steps = max(int(max_steps), len(rt))
         self._prep.update(t_stop=float(t_stop), dt=float(dt), cfg=cfg, precision=precision,
                           refine_steps=(1 if precision == "f32" else 0) if refine_steps is None
                           else int(refine_steps), max_steps=int(max_steps), use_graph=use_graph,
@@ -560,6 +568,11 @@ class CircuitEngine:
         tc.autotune = 1 if p.get("autotune") else 0
         tc.repivot_growth = float(o.repivot_growth)
         tc.output_chunk = p.get("output_chunk", 0)
+        tc.window_from_step = p.get("window_from_step", 0)
+        rt = p.get("replay_times")
+        if rt is not None:
+            tc.replay_times_host = L.ptr(rt, C.c_double)
+            tc.n_replay = len(rt)
         tc.reltol = o.reltol
         tc.abstol = o.abstol
         tc.vntol = o.vntol
@@ -647,6 +660,10 @@ class CircuitEngine:
             "dc_iterations": res.dc_iterations,
             "dc_time": res.dc_time,
             "repivots": res.repivots,
+            "window_steps": res.window_steps,
+            "window_rejected": res.window_rejected,
+            "window_nr_iterations": int(res.window_nr_iterations),
+            "window_time": res.window_time,
             "strategy": "fixed_step_full_mna" if tc.fixed_step else "adaptive_full_mna",
             "precision": self._prep["precision"],
             "return_code": rc,
