@@ -252,6 +252,15 @@ def test_chunked_output_matches_resident_output(fixed):
     assert np.array_equal(a.times, b.times)
     assert np.array_equal(a.stats["V_out"], b.stats["V_out"])
     assert np.array_equal(a.stats["I_out"], b.stats["I_out"])
+    # and the chunked run against the oracle (same accepted grid; the
+    # limited junctions keep the bar at 1e-9 V)
+    _, ref = run_oracle(net, t_stop, 1e-12, fixed=fixed, tran_fs=1.0 if fixed else None)
+    n = min(len(ref.times), len(b.times))
+    same = np.isclose(b.times[:n], ref.times[:n], rtol=1e-9, atol=0)
+    k = n if same.all() else int(np.argmin(same))
+    assert k >= min(n, 3 * 7), (k, n)
+    assert float(np.max(np.abs(b.stats["V_out"][:k] - ref.V[:k]))) <= 1e-9
+    assert float(np.max(np.abs(b.stats["I_out"][:k, :ref.I.shape[1]] - ref.I[:k]))) <= 1e-9
 
 
 def test_transient_rc_fixed_step():

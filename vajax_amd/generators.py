@@ -414,3 +414,138 @@ def batch_mc(n_instances: int = 256, gates: int = 2000, ragged=(500, 1000, 2000,
     nets = [monte_carlo(base, seed + 1 + b, sigma) for b in range(n_instances)]
     nets += [gate_dag(g, seed=seed + 1000 + j) for j, g in enumerate(ragged)]
     return pack(nets, title=f"batch_mc{n_instances}x{gates}_ragged{len(ragged)}_s{seed}")
+
+
+def array_multiplier_netlist(n: int = 64) -> str:
+    """VACASK netlist text of an n x n array multiplier built from CMOS gate
+    subcircuits, the topology the reference's mul64 benchmark describes
+    (vajax/benchmarks/data/mul64/runme.sim:8-13 and the comments of its
+    generator): gates not / nor / and (NAND + inverter) on the nmos / pmos
+    wrappers of the including netlist, xor = not(nor(and(a, not b),
+    and(not a, b))), half adder = xor + and, full adder = two xor, two and, a
+    nor and an inverter for the carry; partial products pp[i][j] = a[i] and
+    b[j]; adder rows: row 0 half adders, rows 1..n-2 a half adder at column 0
+    and full adders at columns 1..n-2; outputs p[0] = pp[0][0] through two
+    inverters, p[1..n-1] the column-0 sums, p[n..2n-3] the last row's sums at
+    columns 1..n-2 and p[2n-2] its carry at column n-2.  Written here, so a
+    netlist's generate block of that name is never executed."""
+    L = ["global vdd vss", ""]
+
+    def sub(name, ports, insts):
+        L.append(f"subckt {name}({' '.join(ports)})")
+        for iname, master, nodes, params in insts:
+            p = "".join(f" {k}={v}" for k, v in params.items())
+            L.append(f"  {iname} ({' '.join(nodes)}) {master}{p}")
+        L.append("ends")
+        L.append("")
+
+    P = {"w": "1u", "l": "0.2u"}
+    N = {"w": "0.5u", "l": "0.2u"}
+    sub("not", ["out", "in"], [("mp", "pmos", ["out", "in", "vdd", "vdd"], P),
+                               ("mn", "nmos", ["out", "in", "vss", "vss"], N)])
+    sub("nor", ["out", "in1", "in2"], [
+        ("mp2", "pmos", ["int", "in2", "vdd", "vdd"], P),
+        ("mp1", "pmos", ["out", "in1", "int", "vdd"], P),
+        ("mn1", "nmos", ["out", "in1", "vss", "vss"], N),
+        ("mn2", "nmos", ["out", "in2", "vss", "vss"], N)])
+    sub("and", ["out", "in1", "in2"], [
+        ("mp2", "pmos", ["outx", "in2", "vdd", "vdd"], P),
+        ("mp1", "pmos", ["outx", "in1", "vdd", "vdd"], P),
+        ("mn1", "nmos", ["outx", "in1", "int", "vss"], N),
+        ("mn2", "nmos", ["int", "in2", "vss", "vss"], N),
+        ("mp3", "pmos", ["out", "outx", "vdd", "vdd"], P),
+        ("mn3", "nmos", ["out", "outx", "vss", "vss"], N)])
+    sub("xor", ["out", "a", "b"], [
+        ("inv_a", "not", ["na", "a"], {}), ("inv_b", "not", ["nb", "b"], {}),
+        ("a1", "and", ["t1", "a", "nb"], {}), ("a2", "and", ["t2", "na", "b"], {}),
+        ("n1", "nor", ["t3", "t1", "t2"], {}), ("inv_out", "not", ["out", "t3"], {})])
+    sub("ha", ["sum", "cout", "a", "b"], [("x1", "xor", ["sum", "a", "b"], {}),
+                                         ("a1", "and", ["cout", "a", "b"], {})])
+    sub("fa", ["sum", "cout", "a", "b", "cin"], [
+        ("x1", "xor", ["ab_xor", "a", "b"], {}), ("x2", "xor", ["sum", "ab_xor", "cin"], {}),
+        ("a1", "and", ["ab_and", "a", "b"], {}), ("a2", "and", ["xor_cin", "ab_xor", "cin"], {}),
+        ("n1", "nor", ["cout_n", "ab_and", "xor_cin"], {}),
+        ("inv_cout", "not", ["cout", "cout_n"], {})])
+    last = n - 2
+
+    def sum_name(row, col):
+        if col == 0:
+            return f"p{row + 1}"
+        if row == last:
+            return f"p{n + col - 1}" if col < n - 1 else f"s{row}_{col}"
+        return f"s{row}_{col}"
+
+    def carry_name(row, col):
+        if row == last and col == n - 2:
+            return f"p{2 * n - 2}"
+        return f"c{row}_{col}"
+
+    insts = []
+    for i in range(n):
+        for j in range(n):
+            insts.append((f"pp_{i}_{j}", "and", [f"pp{i}_{j}", f"a{i}", f"b{j}"], {}))
+    insts.append(("buf_p0_inv", "not", ["p0_n", "pp0_0"], {}))
+    insts.append(("buf_p0", "not", ["p0", "p0_n"], {}))
+    for j in range(n - 1):
+        insts.append((f"ha_0_{j}", "ha", [sum_name(0, j), carry_name(0, j), f"pp0_{j + 1}",
+                                          f"pp1_{j}"], {}))
+    for row in range(1, n - 1):
+        for j in range(n - 1):
+            b_in = f"pp{row + 1}_{j}"
+            a_in = sum_name(row - 1, j + 1) if j < n - 2 else carry_name(row - 1, n - 2)
+            if j == 0:
+                insts.append((f"ha_{row}_{j}", "ha", [sum_name(row, j), carry_name(row, j), a_in,
+                                                      b_in], {}))
+            else:
+                insts.append((f"fa_{row}_{j}", "fa", [sum_name(row, j), carry_name(row, j), a_in,
+                                                      b_in, carry_name(row, j - 1)], {}))
+    ports = ([f"a{i}" for i in range(n)] + [f"b{i}" for i in range(n)]
+             + [f"p{i}" for i in range(2 * n - 1)])
+    sub(f"multiplier_{n}x{n}", ports, insts)
+    return "\n".join(L)
+
+
+def array_multiplier_testbench(n: int = 64, a_bits=None, b_bits=None, t_stop: float = 2e-9,
+                               dt: float = 2e-12, vdd: float = 1.2) -> str:
+    """VACASK netlist text of an n x n array multiplier testbench in the
+    arrangement of the reference's mul64 benchmark
+    (vajax/benchmarks/data/mul64/runme.sim): nmos / pmos wrapper subcircuits
+    (w, l; PSP103 cards, which the front end maps onto the EKV substitute),
+    the multiplier from :func:`array_multiplier_netlist`, every input driven
+    through a 1 ohm resistor by a 0.1 ns pulse from 0 to its bit (all bits
+    high by default), supplies vdd / vss, a uic transient with the
+    benchmark's tran_lteratio = 1.5."""
+    a_bits = [1] * n if a_bits is None else list(a_bits)
+    b_bits = [1] * n if b_bits is None else list(b_bits)
+    L = [f"// {n}x{n} array multiplier testbench", "",
+         "subckt nmos (d g s b)", "parameters w=1u l=0.2u",
+         "  m (d g s b) psp103n w=w l=l", "ends", "",
+         "subckt pmos (d g s b)", "parameters w=1u l=0.2u",
+         "  m (d g s b) psp103p w=w l=l", "ends", "",
+         "model psp103n psp103va (type=1)", "model psp103p psp103va (type=-1)", "",
+         array_multiplier_netlist(n), "",
+         "model v vsource", "model r sp_resistor", "",
+         f"parameters vdd={vdd!r}", "",
+         "subckt drv(out)", "  parameters v0=0 v1=1", "  parameters rs=1 delay=0.1n rise=0.1n",
+         '  vdrv (int 0) v type="pulse" val0=v0*vdd val1=v1*vdd delay=delay rise=rise',
+         "  rdrv (int out) r r=rs", "ends", "",
+         "vdd (vdd 0) v dc=vdd", "vss (vss 0) v dc=0", "",
+         "subckt mul_test()"]
+    ports = ([f"a{i}" for i in range(n)] + [f"b{i}" for i in range(n)]
+             + [f"p{i}" for i in range(2 * n - 1)])
+    L.append(f"  u1 ({' '.join(ports)}) multiplier_{n}x{n}")
+    for i in range(n):
+        L.append(f"  da{i} (a{i}) drv v0=0 v1={int(a_bits[i])}")
+    for i in range(n):
+        L.append(f"  db{i} (b{i}) drv v0=0 v1={int(b_bits[i])}")
+    L += ["ends", "", "control", "  options tran_lteratio=1.5", '  elaborate circuit("mul_test")',
+          f'  analysis tr tran stop={t_stop!r} step={dt!r} icmode="uic"', "endc", ""]
+    return "\n".join(L)
+
+
+def array_multiplier_mosfets(n: int) -> int:
+    """MOSFETs of :func:`array_multiplier_netlist`: and 6, not 2, nor 4,
+    xor 22, half adder 28, full adder 62; n^2 partial products, two
+    inverters, n - 1 half adders in row 0 and, in each of rows 1..n-2, one
+    half adder and n - 2 full adders."""
+    return 6 * n * n + 4 + 28 * (n - 1) + (n - 2) * (28 + 62 * (n - 2))

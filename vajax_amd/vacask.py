@@ -22,8 +22,9 @@ vajax/utils/safe_eval.py:19-200):
   **`` and unary signs, and a whitelist of math functions -- evaluated over
   Python's ``ast`` without ``eval``;
 * ``generate`` blocks (netlist-embedded scripts that the reference runs to
-  produce netlist text) are refused: this front end never executes code from
-  a netlist.
+  produce netlist text) are never executed: the array multiplier block of the
+  reference's mul64 benchmark is replaced by the same topology from an
+  in-tree generator, any other block is refused.
 
 Device mapping onto the models built here: ``resistor``/``sp_resistor``,
 ``capacitor``/``sp_capacitor``, ``vsource``, ``isource``, ``diode``/``sp_diode``
@@ -36,6 +37,7 @@ card's ``type`` polarity and the instance's ``w``/``l``).
 from __future__ import annotations
 
 import ast
+import functools
 import math
 import operator
 import re
@@ -77,6 +79,7 @@ def _scale(letters: str) -> float:
     return 1.0
 
 
+@functools.lru_cache(maxsize=65536)
 def parse_number(text: str) -> Optional[float]:
     """A number with an optional scale suffix ("1u", "0.1n", "1.5meg", "10pF",
     "2e-3"); None when ``text`` is not one."""
@@ -98,12 +101,17 @@ def evaluate(expr: Union[str, float], env: Dict[str, float]) -> float:
     v = parse_number(expr)
     if v is not None:
         return v
+    return float(_eval_node(_parse_expr(expr), env, expr))
+
+
+@functools.lru_cache(maxsize=65536)
+def _parse_expr(expr: str):
+    """The syntax tree of an expression (scale suffixes folded into numbers)."""
     src = _NUM.sub(lambda m: repr(float(m.group(1)) * _scale(m.group(2))), expr.strip())
     try:
-        tree = ast.parse(src, mode="eval")
+        return ast.parse(src, mode="eval").body
     except SyntaxError as e:
         raise ExpressionError(f"bad expression {expr!r}") from e
-    return float(_eval_node(tree.body, env, expr))
 
 
 def _eval_node(n, env, expr):
@@ -379,7 +387,8 @@ class _Parser:
             raise FileNotFoundError(f"include {name!r}: {path} not found")
         if self.depth > 16:
             raise NetlistSyntaxError(f"include {name!r}: nested too deeply")
-        inner = _Parser(tokenize(path.read_text()), self.c, path.parent, self.depth + 1)
+        inner = _Parser(tokenize(expand_generate_blocks(path.read_text())), self.c, path.parent,
+                        self.depth + 1)
         inner.parse(top=False)
 
     def params(self, multiline: bool = False) -> Dict[str, str]:
@@ -512,6 +521,33 @@ class _Parser:
                 self.rest_of_line()  # postprocess, alter, abort, clear, unknown
 
 
+_GENERATE = re.compile(r'^[ \t]*generate[ \t]+"([^"]+)"[ \t]*<<<FILE[ \t]*\n(.*?)^>>>FILE[ \t]*$',
+                       re.M | re.S)
+_MULTIPLIER = re.compile(r"multiplier_(\d+)x(\d+)\.py$")
+
+
+def expand_generate_blocks(text: str) -> str:
+    """``generate "<script>" <<<FILE ... >>>FILE`` blocks are scripts the
+    reference runs at parse time, splicing their output into the netlist.
+    They are never executed here: a block named ``multiplier_<N>x<N>.py``
+    (the array multiplier of vajax/benchmarks/data/mul64/runme.sim:8-13) is
+    replaced by the same topology from the in-tree generator
+    (:func:`vajax_amd.generators.array_multiplier_netlist`); any other block is
+    refused."""
+    from .generators import array_multiplier_netlist
+
+    def repl(m):
+        name = m.group(1)
+        mm = _MULTIPLIER.search(name)
+        if mm and mm.group(1) == mm.group(2):
+            return (f"// generate {name}: array multiplier from the in-tree generator\n"
+                    + array_multiplier_netlist(int(mm.group(1))) + "\n")
+        raise NetlistSyntaxError(f"generate {name!r}: netlist-embedded scripts are not "
+                                 "executed; expand the block to netlist text first")
+
+    return _GENERATE.sub(repl, text)
+
+
 def parse_netlist(source: Union[str, Path], text: Optional[str] = None) -> Circuit:
     """Parse a VACASK netlist from a file path, or from ``text`` (includes then
     resolve against ``source`` when it names a directory or file)."""
@@ -524,7 +560,7 @@ def parse_netlist(source: Union[str, Path], text: Optional[str] = None) -> Circu
         p = Path(source)
         base = p if p.is_dir() else p.parent
     c = Circuit()
-    _Parser(tokenize(text), c, base).parse(top=True)
+    _Parser(tokenize(expand_generate_blocks(text)), c, base).parse(top=True)
     return c
 
 
@@ -622,6 +658,7 @@ def _num(v) -> float:
 def to_netlist(c: Circuit, title: Optional[str] = None) -> Netlist:
     """Flatten and map onto the device models built here."""
     b = NetlistBuilder(title or c.title or "netlist", ground=c.ground or "0")
+    card_values: Dict[int, Dict[str, object]] = {}
     for path, terms, master, params, models in flatten(c):
         model = models.get(master)
         module = (model.module if model else master).lower()
@@ -630,15 +667,28 @@ def to_netlist(c: Circuit, title: Optional[str] = None) -> Netlist:
             raise NotImplementedError(f"{path}: no device model for module {module!r}")
         mparams: Dict[str, object] = {}
         if model is not None:
-            for k, v in model.params.items():
-                mparams[k] = _value(k, v, {})
+            # a model card's parameters are constants: evaluated once per card
+            mparams = card_values.get(id(model))
+            if mparams is None:
+                mparams = {k: _value(k, v, {}) for k, v in model.params.items()}
+                card_values[id(model)] = mparams
         p = {k.lower(): v for k, v in {**mparams, **params}.items()}
         if kind in ("vsource", "isource"):
             if len(terms) != 2:
                 raise ValueError(f"{path}: a source has 2 terminals")
             wave = str(p.pop("type", "dc")).strip('"').lower()
+            points = p.pop("wave", None)
+            if isinstance(points, str):
+                body = points.strip()
+                if not (body.startswith("[") and body.endswith("]")):
+                    raise ValueError(f"{path}: pwl wave must be a [t0, v0, t1, v1, ...] list")
+                body = body[1:-1].strip()
+                points = [evaluate(x, {}) for x in body.split(",")] if body else []
+            if "sinephase" in p and "phase" not in p:
+                p["phase"] = p.pop("sinephase")
             kw = {k: _num(v) for k, v in p.items() if k in SOURCE_DEFAULTS}
-            b.source(kind, path, terms[0], terms[1], type=wave, **kw)
+            b.source(kind, path, terms[0], terms[1], type=wave,
+                     wave=points if wave == "pwl" else None, **kw)
             continue
         if kind == "mosfet_psp":
             # PSP103 is not available: the EKV substitute of the synthetic
