@@ -142,15 +142,22 @@ def kernel_bytes(topo, info, precision: str):
     solves = (4 + vb) * offd + (4 + 8 + 3 * vb) * 2 * n
     if precision == "f32":       # second solve of the refinement step + FP64 SpMV
         solves = 2 * solves
+    # dense trailing block: its slots gathered and written back, the
+    # contributing L / U slots gathered, and D (K x K FP64) read and written
+    # once per panel step it takes part in (counted once)
+    K, m = info.get("dense_k", 0), info.get("dense_m", 0)
+    lu_dense = 8 * (3 * K * K + 2 * K * m) if K else 0
     return {"eval": ev, "stamp": stamp, "lu_gather": lu_gather, "lu_tasks": lu_tasks,
-            "lu_schur": lu_schur, "lu_scale": lu_scale, "solve_tasks": solves, "solve_hub": 0}
+            "lu_schur": lu_schur, "lu_scale": lu_scale, "solve_tasks": solves, "solve_hub": 0,
+            "lu_dense": lu_dense}
 
 
 KERNEL_NAMES = {"eval": "k_eval<MOSFET> (+R/C)", "stamp": "k_chunk_sum/k_reduce_jac/k_chunk_res/k_residual",
                 "lu_gather": "k_lu_gather", "lu_tasks": "k_lu_band_lds (band tasks)",
                 "lu_schur": "k_lu_wide (Schur step)",
                 "lu_scale": "k_lu_scale (level programs only)", "solve_tasks": "k_tri_band_lde (forward + backward)",
-                "solve_hub": "k_tri_hub_part/k_tri_hub_finish"}
+                "solve_hub": "k_tri_hub_part/k_tri_hub_finish",
+                "lu_dense": "k_dgemm_sub/k_dense_panel (dense trailing block, FP64 MFMA)"}
 
 
 # kernel class -> kernel names in the rocprofv3 traffic summaries

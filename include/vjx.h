@@ -140,6 +140,10 @@ typedef struct {
   double tuned_solve_ms;                /* solve time the autotuner measured (0: not run) */
   double tuned_factor_ms;               /* refactorisation time it measured (0: not run) */
   int32_t lu_pipeline;                  /* LU band tasks: 0 by task count, 1 shallow, 2 deep */
+  /* dense trailing block factorised with MFMA: its size (0: none), the
+   * earlier pivots contributing to it and its FP64 operations per refactor */
+  int32_t dense_k, dense_m;
+  double dense_flops;
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
@@ -174,6 +178,15 @@ int vjx_lu_refactor_cpu(vjx_pattern* pat, const double* data_host, int32_t preci
                         double* lu_host);
 int vjx_lu_solve_cpu(vjx_pattern* pat, const double* data_host, int32_t precision,
                      const double* b_host, double* x_host);
+/* The dense-block kernels of the refactorisation on their own (device
+ * arrays, row-major), for tests: dense LU without pivoting of a K x K matrix
+ * in place (unit L below the diagonal, U on and above; panels of 32
+ * columns), and C[M x N] -= A[M x Kd] B[Kd x N] on the FP64 matrix cores.
+ * No counterpart in the reference. */
+int vjx_dense_lu(int32_t K, double* D_dev);
+int vjx_dense_gemm_sub(int32_t M, int32_t N, int32_t Kd, const double* A_dev, int32_t lda,
+                       const double* B_dev, int32_t ldb, double* C_dev, int32_t ldc);
+
 /* Copy the device LU values (FP64 or FP32 widened) to the host. */
 int vjx_lu_download(vjx_pattern* pat, double* lu_host);
 
@@ -515,9 +528,10 @@ typedef struct {
   /* per kernel class, from event pairs around each launch: 0 device eval,
    * 1 stamping reductions + residual, 2 LU gather, 3 LU band tasks (or LU
    * level launches), 4 LU Schur steps, 5 LU scaling, 6 solve band tasks (or
-   * solve level launches), 7 solve hub rows */
-  double kernel_ms[8];
-  int64_t kernel_launches[8];
+   * solve level launches), 7 solve hub rows, 8 dense trailing block of the
+   * refactorisation */
+  double kernel_ms[9];
+  int64_t kernel_launches[9];
 } vjx_phase_times;
 
 int vjx_profile_enable(int32_t enable);

@@ -49,6 +49,7 @@ class LuInfo(C.Structure):
         ("band_width", C.c_int32), ("solve_band_width", C.c_int32),
         ("tuned_solve_ms", C.c_double), ("tuned_factor_ms", C.c_double),
         ("lu_pipeline", C.c_int32),
+        ("dense_k", C.c_int32), ("dense_m", C.c_int32), ("dense_flops", C.c_double),
     ]
 
     def as_dict(self):
@@ -146,11 +147,11 @@ class NewtonResult(C.Structure):
 
 class PhaseTimes(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("lu_ms", C.c_double), ("solve_ms", C.c_double),
-                ("n_iterations", C.c_int64), ("kernel_ms", C.c_double * 8),
-                ("kernel_launches", C.c_int64 * 8)]
+                ("n_iterations", C.c_int64), ("kernel_ms", C.c_double * 9),
+                ("kernel_launches", C.c_int64 * 9)]
 
     KERNEL_CLASSES = ("eval", "stamp", "lu_gather", "lu_tasks", "lu_schur", "lu_scale",
-                      "solve_tasks", "solve_hub")
+                      "solve_tasks", "solve_hub", "lu_dense")
 
     def as_dict(self):
         d = {k: getattr(self, k) for k in ("build_ms", "lu_ms", "solve_ms", "n_iterations")}
@@ -192,6 +193,9 @@ _PROTOS = {
     "vjx_build_system": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp,
                                    vp, vp, vp, vp]),
+    "vjx_dense_lu": (C.c_int, [C.c_int32, vp]),
+    "vjx_dense_gemm_sub": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp, C.c_int32,
+                                     vp, C.c_int32]),
     "vjx_model_eval_split": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(EvalSplit)] + [vp] * 15),
     "vjx_model_eval": (C.c_int, [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "vjx_eval_stamp": (C.c_int, [vp, C.c_int32, vp, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp,

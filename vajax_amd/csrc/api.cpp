@@ -134,6 +134,9 @@ int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* o) {
   o->tuned_solve_ms = pat->tuned_solve_ms;
   o->tuned_factor_ms = pat->tuned_factor_ms;
   o->lu_pipeline = pat->lu.lu_deep;
+  o->dense_k = s.dense_k;
+  o->dense_m = s.dense_m;
+  o->dense_flops = s.dense_flops;
   o->fact_task_updates = o->fact_schur_updates = o->fact_task_targets = o->fact_schur_targets = 0;
   if (!s.fact_band.band_w.empty()) {
     std::vector<char> schur(s.fact_band.items.size(), 0);

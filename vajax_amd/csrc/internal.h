@@ -132,6 +132,9 @@ struct DevLU {
   DBuf<float> lu32;
   DBuf<double> y64, z64, r64, x64;
   DBuf<float> y32, z32;
+  // dense trailing block (sym.dense_k > 0): slot maps and FP64 work arrays
+  DBuf<int32_t> dense_slot, dense_pos, dl_slot, dl_pos, du_slot, du_pos;
+  DBuf<double> dense_D, dense_Lp, dense_Up;
   int narrow_block = 512;
   int wide_block = 256;
 };
@@ -188,6 +191,7 @@ enum KernelClass {
   KC_LU_SCALE,     // k_lu_scale
   KC_TRI_TASKS,    // k_tri_band (and level launches of the level program)
   KC_TRI_HUB,      // chunked hub rows of the solves
+  KC_LU_DENSE,     // dense trailing block: gathers, MFMA products, panels, write-back
   KC_COUNT
 };
 void ktimer_enable(bool on);
@@ -246,6 +250,12 @@ void launch_residual(const ResidualArgs& a, hipStream_t s);
 
 template <typename T>
 void launch_lu_refactor(DevLU& lu, const double* csr, T* lu_vals, DevStatus* st, hipStream_t s);
+// the dense trailing block of a refactorisation (dense.hip)
+template <typename T>
+void launch_dense_block(DevLU& L, T* lu, DevStatus* st, hipStream_t s);
+void launch_dense_lu(int K, double* D, hipStream_t s);
+void launch_dgemm_sub(int M, int N, int Kd, const double* A, int lda, const double* B, int ldb,
+                      double* C, int ldc, hipStream_t s);
 // hub_parts (may be null): the forward and backward solves' hub chunk
 // partials, in place of the band programs' own buffers
 template <typename T>

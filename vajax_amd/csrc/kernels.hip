@@ -1808,6 +1808,7 @@ void launch_lu_refactor(DevLU& L, const double* csr, T* lu, DevStatus* st, hipSt
   k_lu_gather<T><<<nblk(nnz, 256), 256, 0, s>>>(nnz, L.src.p, csr, lu, st);
   ktimer_end(KC_LU_GATHER, s);
   run_fact_groups<T>(L, lu, st, s);
+  launch_dense_block<T>(L, lu, st, s);
   // band programs scaled L inside their tasks
   const int nl = L.fact_band.on ? 0 : (int)L.l_slot.n;
   if (nl > 0) {

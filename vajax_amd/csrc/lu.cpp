@@ -172,6 +172,15 @@ int pattern_upload(vjx_pattern* pat) {
   VJX_TRY_HIP(L.task_scale.upload(s.task_scale));
   VJX_TRY_HIP(L.task_scale_ptr.upload(s.task_scale_ptr));
   VJX_TRY_HIP(L.lds_scale.upload(s.lds_scale));
+  VJX_TRY_HIP(L.dense_slot.upload(s.dense_slot));
+  VJX_TRY_HIP(L.dense_pos.upload(s.dense_pos));
+  VJX_TRY_HIP(L.dl_slot.upload(s.dl_slot));
+  VJX_TRY_HIP(L.dl_pos.upload(s.dl_pos));
+  VJX_TRY_HIP(L.du_slot.upload(s.du_slot));
+  VJX_TRY_HIP(L.du_pos.upload(s.du_pos));
+  VJX_TRY_HIP(L.dense_D.alloc((size_t)s.dense_k * s.dense_k));
+  VJX_TRY_HIP(L.dense_Lp.alloc((size_t)s.dense_k * s.dense_m));
+  VJX_TRY_HIP(L.dense_Up.alloc((size_t)s.dense_k * s.dense_m));
   if (s.fact_band.band_w.empty()) {
     L.upd_lu.release();
   } else {

@@ -583,6 +583,38 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
     }
 
   s.analyze_ms[2] = ms_since(t0) - s.analyze_ms[0] - s.analyze_ms[1];
+  // ---- dense trailing block (band programs only) ----
+  int s0 = n;  // first dense pivot; n: none
+  if (opt.band_width > 0 && opt.dense_min > 0 && n >= 2 * opt.dense_min) {
+    auto trailing = [&](int K) {  // LU entries with row and column >= n - K
+      const int lo = n - K;
+      int64_t c = 0;
+      for (int i = lo; i < n; ++i) {
+        auto b = s.col.begin() + s.rowptr[i], e = s.col.begin() + s.rowptr[i + 1];
+        c += e - std::lower_bound(b, e, lo);
+      }
+      return c;
+    };
+    int best = 0;
+    for (int K = opt.dense_min; K <= n / 2; K += std::max(16, K / 8)) {
+      if ((double)trailing(K) >= opt.dense_density * (double)K * (double)K)
+        best = K;
+      else if (best)
+        break;
+    }
+    if (best) s0 = n - best;
+  }
+  if (s0 < n) {
+    const int K = n - s0;
+    s.dense_start = s0;
+    s.dense_k = K;
+    for (int i = s0; i < n; ++i)
+      for (int e = s.rowptr[i]; e < s.rowptr[i + 1]; ++e)
+        if (s.col[e] >= s0) {
+          s.dense_slot.push_back(e);
+          s.dense_pos.push_back((i - s0) * K + (s.col[e] - s0));
+        }
+  }
   // CSC view of the LU pattern: column k -> slots, sorted by row
   std::vector<int32_t> ccnt(n + 1, 0);
   for (int64_t e = 0; e < nnz_lu; ++e) ccnt[s.col[e] + 1]++;
@@ -596,6 +628,40 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
         cslot[d] = e;
         crow[d] = i;
       }
+  }
+  std::vector<char> in_gemm(n, 0);  // pivots whose dense-block updates are a matrix product
+  if (s0 < n) {
+    // earlier pivots with L rows and U columns inside the dense block
+    const int K = n - s0;
+    // a pivot whose contribution covers at least 1/32 of the block joins the
+    // matrix product; the others update the block's slots by scalar updates
+    // in the scalar program (the block is assembled after it)
+    std::vector<int32_t> P;
+    for (int k = 0; k < s0; ++k) {
+      int64_t nl = 0, nu = 0;
+      for (int d = ccnt[k]; d < ccnt[k + 1]; ++d) nl += crow[d] >= s0;
+      for (int e = s.diag[k] + 1; e < s.rowptr[k + 1]; ++e) nu += s.col[e] >= s0;
+      if (nl > 0 && nu > 0 && 32 * nl * nu >= (int64_t)K * K) {
+        P.push_back(k);
+        in_gemm[k] = 1;
+      }
+    }
+    const int m = (int)P.size();
+    s.dense_m = m;
+    for (int p = 0; p < m; ++p) {
+      const int k = P[p];
+      for (int d = ccnt[k]; d < ccnt[k + 1]; ++d)
+        if (crow[d] >= s0) {
+          s.dl_slot.push_back(cslot[d]);
+          s.dl_pos.push_back((crow[d] - s0) * m + p);
+        }
+      for (int e = s.diag[k] + 1; e < s.rowptr[k + 1]; ++e)
+        if (s.col[e] >= s0) {
+          s.du_slot.push_back(e);
+          s.du_pos.push_back(p * K + (s.col[e] - s0));
+        }
+    }
+    s.dense_flops = 2.0 * K * (double)K * m + 2.0 * K * (double)K * K / 3.0;
   }
   // pivot levels
   std::vector<int32_t> level(n, 0);
@@ -611,10 +677,17 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
     level[k] = lv;
     nlev = std::max(nlev, lv + 1);
   }
+  if (s0 < n) {
+    // the scalar program covers the pivots before the dense block only
+    nlev = 0;
+    for (int k = 0; k < s0; ++k) nlev = std::max(nlev, level[k] + 1);
+  }
   s.n_fact_levels = nlev;
   // update contributions grouped by level, sorted by (target slot, pivot)
   std::vector<std::vector<int32_t>> by_level(nlev);
-  for (int k = 0; k < n; ++k) by_level[level[k]].push_back(k);
+  for (int k = 0; k < s0; ++k) by_level[level[k]].push_back(k);
+  // rows and columns of the dense block belong to no band's pivots
+  auto lev = [&](int x) { return x >= s0 ? INT32_MAX : level[x]; };
   s.tgt_pos.clear();
   s.tgt_seg.assign(1, 0);
   s.upd_l.clear();
@@ -716,13 +789,14 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
           const int i = crow[d], lpos = cslot[d];
           for (int e = s.diag[k] + 1; e < s.rowptr[k + 1]; ++e) {
             const int j = s.col[e];
+            if (i >= s0 && j >= s0 && in_gemm[k]) continue;  // the dense block's product
             const int32_t t = find(i, j);
             if (t < 0) {
               err = "internal: fill closure violated";
               return false;
             }
             // a row or column of a pivot in this band belongs to the pivot's task
-            if (level[i] < b1 || level[j] < b1)
+            if (lev(i) < b1 || lev(j) < b1)
               own.push_back({task, level[k] - b0, t, k, lpos, e});
             else
               sch.push_back({t, k, lpos, e});
@@ -1282,12 +1356,39 @@ bool lu_check_schedules(const LuSymbolic& s, std::string& err) {
         }
         scaled_band[l] = (int32_t)b;
       }
+  const int ds = s.dense_start >= 0 ? s.dense_start : s.n;
   for (int i = 0; i < s.n; ++i)
     for (int e = s.rowptr[i]; e < s.diag[i]; ++e)
-      if (scaled_band[e] < 0) {
+      if (scaled_band[e] < 0 && !(i >= ds && s.col[e] >= ds)) {
         err = "refactorisation: L slot " + std::to_string(e) + " never scaled";
         return false;
       }
+  if (ds < s.n) {
+    // dense block: every slot inside it once, none written or scaled by the
+    // scalar program; its inputs Lp / Up are final, scaled L and U slots
+    const int K = s.dense_k, m = s.dense_m;
+    std::vector<char> seen((size_t)K * K, 0);
+    size_t expect = 0;
+    for (int i = ds; i < s.n; ++i)
+      for (int e = s.rowptr[i]; e < s.rowptr[i + 1]; ++e) expect += s.col[e] >= ds;
+    bool ok = s.dense_slot.size() == expect && s.dense_pos.size() == expect;
+    for (size_t q = 0; ok && q < s.dense_slot.size(); ++q) {
+      const int e = s.dense_slot[q], pos = s.dense_pos[q];
+      const int i = (int)(std::upper_bound(s.rowptr.begin(), s.rowptr.end(), e) -
+                          s.rowptr.begin()) - 1;
+      ok = pos >= 0 && pos < K * K && !seen[pos] && i >= ds && s.col[e] >= ds &&
+           pos == (i - ds) * K + (s.col[e] - ds) && scaled_band[e] < 0;
+      if (ok) seen[pos] = 1;
+    }
+    for (size_t q = 0; ok && q < s.dl_slot.size(); ++q)
+      ok = s.dl_pos[q] >= 0 && s.dl_pos[q] < K * m && scaled_band[s.dl_slot[q]] >= 0;
+    for (size_t q = 0; ok && q < s.du_slot.size(); ++q)
+      ok = s.du_pos[q] >= 0 && s.du_pos[q] < m * K && s.col[s.du_slot[q]] >= ds;
+    if (!ok) {
+      err = "refactorisation: dense block maps inconsistent";
+      return false;
+    }
+  }
   for (size_t k = 0; k < items.size(); ++k) {
     const int t = items[k].first;
     const int64_t b = items[k].second / 256;
@@ -1731,7 +1832,10 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
     size_t u = 0;
     for (int v : b.verts)
       for (int e = indptr[v]; e < indptr[v + 1]; ++e) lv[u++] = vals[e];
-    if (!analyze_single((int)b.verts.size(), b.indptr.data(), b.indices.data(), lv.data(), opt,
+    // packed batches keep every block on the scalar program (no dense block)
+    AnalyzeOptions bopt = opt;
+    bopt.dense_min = 0;
+    if (!analyze_single((int)b.verts.size(), b.indptr.data(), b.indices.data(), lv.data(), bopt,
                         sym[k], err))
       return false;
   }
@@ -1890,6 +1994,57 @@ static int item_class(const LevelSchedule& S, int l, int it) {
   return 0;
 }
 
+// The dense block on the CPU (FP64, whatever T): assemble, subtract the
+// earlier pivots' contributions, blocked LU without pivoting in panels of
+// kDensePanel columns, write back.  The same algorithm as the device kernels;
+// the sums run in another order, so the two agree to rounding.
+template <typename T>
+static void dense_block_host(const LuSymbolic& s, T* lu) {
+  const int K = s.dense_k, m = s.dense_m;
+  std::vector<double> D((size_t)K * K, 0.0), Lp((size_t)K * m, 0.0), Up((size_t)m * K, 0.0);
+  for (size_t q = 0; q < s.dense_slot.size(); ++q) D[s.dense_pos[q]] = (double)lu[s.dense_slot[q]];
+  for (size_t q = 0; q < s.dl_slot.size(); ++q) Lp[s.dl_pos[q]] = (double)lu[s.dl_slot[q]];
+  for (size_t q = 0; q < s.du_slot.size(); ++q) Up[s.du_pos[q]] = (double)lu[s.du_slot[q]];
+  for (int i = 0; i < K; ++i)
+    for (int p = 0; p < m; ++p) {
+      const double l = Lp[(size_t)i * m + p];
+      if (l == 0.0) continue;
+      for (int j = 0; j < K; ++j) D[(size_t)i * K + j] -= l * Up[(size_t)p * K + j];
+    }
+  dense_lu_host(K, D.data());
+  for (size_t q = 0; q < s.dense_slot.size(); ++q) lu[s.dense_slot[q]] = (T)D[s.dense_pos[q]];
+}
+
+void dense_lu_host(int K, double* D) {
+  for (int o = 0; o < K; o += kDensePanel) {
+    const int w = std::min(kDensePanel, K - o);
+    for (int c = o; c < o + w; ++c) {  // the panel's diagonal block
+      for (int r = c + 1; r < o + w; ++r) {
+        D[(size_t)r * K + c] /= D[(size_t)c * K + c];
+        for (int c2 = c + 1; c2 < o + w; ++c2)
+          D[(size_t)r * K + c2] -= D[(size_t)r * K + c] * D[(size_t)c * K + c2];
+      }
+    }
+    for (int r = o + w; r < K; ++r)  // L21 = A21 U11^-1
+      for (int c = o; c < o + w; ++c) {
+        double x = D[(size_t)r * K + c];
+        for (int c2 = o; c2 < c; ++c2) x -= D[(size_t)r * K + c2] * D[(size_t)c2 * K + c];
+        D[(size_t)r * K + c] = x / D[(size_t)c * K + c];
+      }
+    for (int j = o + w; j < K; ++j)  // U12 = L11^-1 A12
+      for (int r = o; r < o + w; ++r) {
+        double y = D[(size_t)r * K + j];
+        for (int r2 = o; r2 < r; ++r2) y -= D[(size_t)r * K + r2] * D[(size_t)r2 * K + j];
+        D[(size_t)r * K + j] = y;
+      }
+    for (int r = o + w; r < K; ++r)  // A22 -= L21 U12
+      for (int c = o; c < o + w; ++c) {
+        const double l = D[(size_t)r * K + c];
+        for (int j = o + w; j < K; ++j) D[(size_t)r * K + j] -= l * D[(size_t)c * K + j];
+      }
+  }
+}
+
 template <typename T>
 void lu_refactor_host(const LuSymbolic& s, const double* a, T* lu) {
   const int64_t nnz = s.rowptr[s.n];
@@ -1928,6 +2083,7 @@ void lu_refactor_host(const LuSymbolic& s, const double* a, T* lu) {
     }
   if (!bands)
     for (size_t e = 0; e < s.l_slot.size(); ++e) lu[s.l_slot[e]] /= lu[s.l_diag[e]];
+  if (s.dense_start >= 0) dense_block_host<T>(s, lu);
 }
 
 template <typename T>

@@ -168,6 +168,21 @@ struct LuSymbolic {
   LevelSchedule bwd;             // items = rows, U solve
   BandSchedule fwd_band;         // band programs of the two solves (empty: level launches)
   BandSchedule bwd_band;
+  // Dense trailing block: pivots dense_start .. n-1 (dense_k of them) are
+  // factorised as a dense row-major dense_k x dense_k matrix D after the
+  // scalar program.  D is assembled from the slots (i, j), i, j >=
+  // dense_start (dense_slot -> dense_pos) as the scalar program leaves them:
+  // A's values less the updates of earlier pivots whose contribution to the
+  // block is small; the dense_m earlier pivots with large contributions
+  // (covering at least 1/32 of the block) are left out of the scalar program
+  // there and subtracted as one product D -= Lp Up, Lp [dense_k x dense_m]
+  // from their L slots (dl_slot -> dl_pos), Up [dense_m x dense_k] from their
+  // U slots (du_slot -> du_pos); then D = L U in place (unit L, no pivoting:
+  // the pivots are the analysis's) and the slots are written back.
+  // dense_start = -1: no dense block.
+  int dense_start = -1, dense_k = 0, dense_m = 0;
+  std::vector<int32_t> dense_slot, dense_pos, dl_slot, dl_pos, du_slot, du_pos;
+  double dense_flops = 0.0;  // 2 K^2 m (contributions) + 2 K^3 / 3 (dense LU)
   // statistics
   int64_t nnz_l = 0, nnz_u = 0, n_updates = 0;
   double flops = 0.0;
@@ -189,6 +204,11 @@ struct AnalyzeOptions {
   int nd_leaf = 0;            // nested-dissection leaf size (min-degree inside);
                               // 0: max(48, n / 128) of the block being ordered
   int band_width = 16;        // bands of this many levels (0: one launch per level)
+  // dense trailing block: the largest trailing block of the LU pattern (at
+  // least dense_min rows) whose density reaches dense_density is factorised
+  // as a dense matrix (0: never); band programs of single-block systems only
+  int dense_min = 96;
+  double dense_density = 0.4;
   int solve_band_width = 0;   // solve bands (0: band_width)
   int max_band_tasks = 2048;  // workgroups per band launch
 };
@@ -207,6 +227,12 @@ bool csr_validate(int n, int nnz, const int32_t* indptr, const int32_t* indices,
 
 // Dependency check of the band programs (every row once, after what it reads).
 bool lu_check_schedules(const LuSymbolic& s, std::string& err);
+
+// Columns per panel of the blocked dense LU of the dense trailing block.
+constexpr int kDensePanel = 32;
+// Dense LU without pivoting of a row-major K x K matrix in place (unit L
+// below the diagonal, U on and above), in panels of kDensePanel columns.
+void dense_lu_host(int K, double* D);
 
 // CPU execution of the same numeric program (reference for the GPU kernels).
 template <typename T>

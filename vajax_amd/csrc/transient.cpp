@@ -451,7 +451,10 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
                   w.iout.p + row * mo, s);
     return VJX_OK;
   };
-  rc = emit(0, 0);
+  // row 0: the initial state; the adaptive loop records the DC source
+  // currents there (full_mna.py:1021-1023), the fixed-step loop leaves them
+  // at zero (full_mna.py:1259-1263)
+  rc = emit(0, cfg->fixed_step ? 1 : 0);
   if (rc < 0) return rc;
   {
     // Vmax = |X0|
