@@ -26,6 +26,7 @@ extern "C" {
 #define VJX_ERR_HIP (-2)      /* HIP runtime error */
 #define VJX_ERR_SINGULAR (-3) /* structurally singular matrix */
 #define VJX_ERR_STATE (-4)    /* call out of sequence (e.g. solve before refactor) */
+#define VJX_ERR_FILL (-5)     /* analysis abandoned: LU fill beyond the fill limit */
 
 #define VJX_PREC_F64 0 /* factorise and solve in FP64 */
 #define VJX_PREC_F32 1 /* factorise in FP32, FP64 residual + refinement */
@@ -166,6 +167,13 @@ int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width);
  * caches one symbolic analysis per pattern); a new value drops the analysis
  * like a new band width. */
 int vjx_pattern_set_ordering(vjx_pattern* pat, int32_t ordering, int32_t nd_leaf);
+/* Fill limit of later analyses: an analysis whose L + U would hold more than
+ * max_lu_nnz entries stops early with VJX_ERR_FILL and leaves the previous
+ * analysis in effect (0 = no limit, the default).  The transient loop's pivot
+ * monitor re-analyses under a limit of four times the current factor.  No
+ * counterpart in the reference (UMFPACK's numeric factorisation re-pivots
+ * within its fixed symbolic analysis, vajax/sparse/umfpack_ffi.cpp:287-316). */
+int vjx_pattern_set_fill_limit(vjx_pattern* pat, int64_t max_lu_nnz);
 /* Autotuner of the launch plan: with the matrix data_dev (device), time the
  * numeric refactorisation for refactorisation bands of 12, 16 and 24 pivot
  * levels (a fresh analysis each: same ordering and pivots) and each LU

@@ -217,3 +217,42 @@ def test_packed_batch_band_programs_checked():
     lu = SparseLU(cir.indptr, cir.indices)
     lu.analyze(csr)
     lu.check_schedules()
+
+
+@pytest.mark.parametrize("packed", [False, True], ids=["single", "packed"])
+def test_fill_limit_abandons_the_analysis_and_keeps_the_previous_one(packed):
+    # the transient loop's pivot monitor re-analyses under a fill limit: an
+    # analysis beyond it fails with VJX_ERR_FILL and the previous pivots stay
+    import ctypes as C
+
+    from vajax_amd import _lib as L
+
+    if packed:
+        net = G.pack([G.monte_carlo(G.gate_dag(150, seed=4), 10 + b, 0.03) for b in range(3)])
+    else:
+        net = G.gate_dag(300, seed=5)
+    cir, csr, A = _system(net)
+    lu = SparseLU(cir.indptr, cir.indices)
+    lu.analyze(csr)
+    info = lu.info()
+    nnz = info["nnz_l"] + info["nnz_u"]
+    vals = lu.refactor_cpu(csr)
+    p0, q0 = lu.perm()
+    _, csr2, A2 = _system(net, seed=7)
+    d2 = np.ascontiguousarray(csr2)
+    lu.set_fill_limit(nnz // 2)
+    assert L.lib.vjx_lu_analyze(lu.h, L.ptr(d2, C.c_double)) == L.VJX_ERR_FILL
+    assert "fill limit" in L.last_error()
+    after = lu.info()
+    assert after["nnz_l"] + after["nnz_u"] == nnz
+    p1, q1 = lu.perm()
+    assert np.array_equal(p0, p1) and np.array_equal(q0, q1)
+    assert np.array_equal(lu.refactor_cpu(csr), vals)
+    # without a limit the same matrix analyses and solves
+    lu.set_fill_limit(0)
+    lu.analyze(d2)
+    b = np.random.default_rng(3).standard_normal(cir.n_aug)
+    x = lu.solve_cpu(d2, b)
+    assert np.linalg.norm(A2 @ x - b) <= 1e-12 * np.linalg.norm(b)
+    with pytest.raises(L.VjxError):
+        lu.set_fill_limit(-1)

@@ -20,6 +20,7 @@ VJX_ERR_ARG = -1
 VJX_ERR_HIP = -2
 VJX_ERR_SINGULAR = -3
 VJX_ERR_STATE = -4
+VJX_ERR_FILL = -5
 VJX_PREC_F64 = 0
 VJX_PREC_F32 = 1
 VJX_METHOD = {"be": 0, "trap": 1, "gear2": 2}
@@ -182,6 +183,7 @@ _PROTOS = {
     "vjx_lu_check_schedules": (C.c_int, [vp]),
     "vjx_pattern_set_band_width": (C.c_int, [vp, C.c_int32]),
     "vjx_pattern_set_ordering": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "vjx_pattern_set_fill_limit": (C.c_int, [vp, C.c_int64]),
     "vjx_lu_autotune": (C.c_int, [vp, vp]),
     "vjx_lu_refactor_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p]),
     "vjx_lu_solve_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p, f64p]),

@@ -77,6 +77,7 @@ int vjx_lu_analyze(vjx_pattern* pat, const double* data_host) {
   if (!lu_analyze(pat->n, pat->indptr.data(), pat->indices.data(), data_host, pat->opts, fresh,
                   err)) {
     set_error("vjx_lu_analyze: " + err);
+    if (err.find("fill limit") != std::string::npos) return VJX_ERR_FILL;
     return err.find("singular") != std::string::npos ? VJX_ERR_SINGULAR : VJX_ERR_ARG;
   }
   pat->lu.sym = std::move(fresh);
@@ -227,6 +228,15 @@ int vjx_pattern_set_ordering(vjx_pattern* pat, int32_t ordering, int32_t nd_leaf
     pat->lu.factored_prec = -1;
     pat->uploaded = false;
   }
+  return VJX_OK;
+}
+
+int vjx_pattern_set_fill_limit(vjx_pattern* pat, int64_t max_lu_nnz) {
+  if (!pat || max_lu_nnz < 0) {
+    set_error("vjx_pattern_set_fill_limit: null pattern or negative limit");
+    return VJX_ERR_ARG;
+  }
+  pat->opts.max_lu_nnz = max_lu_nnz;
   return VJX_OK;
 }
 

@@ -324,8 +324,8 @@ struct ColStore {
 // Left-looking LU with threshold partial pivoting (Gilbert-Peierls).
 static bool gp_factor(int n, const std::vector<int32_t>& cp, const std::vector<int32_t>& ri,
                       const std::vector<double>& vx, const std::vector<int32_t>& q, double tol,
-                      std::vector<int32_t>& pinv, ColStore& L, ColStore& U, int& n_offdiag,
-                      double& min_ratio, std::string& err) {
+                      int64_t max_nnz, std::vector<int32_t>& pinv, ColStore& L, ColStore& U,
+                      int& n_offdiag, double& min_ratio, std::string& err) {
   pinv.assign(n, -1);
   std::vector<double> x(n, 0.0);
   std::vector<char> mark(n, 0);
@@ -412,6 +412,11 @@ static bool gp_factor(int n, const std::vector<int32_t>& cp, const std::vector<i
       }
     }
     L.ptr.push_back((int64_t)L.idx.size());
+    if (max_nnz > 0 && (int64_t)(L.idx.size() + U.idx.size()) > max_nnz) {
+      err = "LU fill beyond the fill limit of " + std::to_string(max_nnz) + " entries at pivot " +
+            std::to_string(k) + " of " + std::to_string(n);
+      return false;
+    }
     pinv[ipiv] = k;
     for (int t = top; t < n; ++t) {
       x[xi[t]] = 0.0;
@@ -524,8 +529,8 @@ static bool analyze_single(int n, const int32_t* indptr, const int32_t* indices,
   for (int k = 0; k < n; ++k) s.qinv[s.q[k]] = k;
 
   ColStore L, U;
-  if (!gp_factor(n, cp, ri, vx, s.q, opt.pivot_tol, s.pinv, L, U, s.n_offdiag_pivots,
-                 s.min_pivot_ratio, err))
+  if (!gp_factor(n, cp, ri, vx, s.q, opt.pivot_tol, opt.max_lu_nnz, s.pinv, L, U,
+                 s.n_offdiag_pivots, s.min_pivot_ratio, err))
     return false;
   s.p.assign(n, 0);
   for (int r = 0; r < n; ++r) s.p[s.pinv[r]] = r;
@@ -1855,6 +1860,11 @@ bool lu_analyze(int n, const int32_t* indptr, const int32_t* indices, const doub
   }
   if (nnz_lu > (int64_t)INT32_MAX || n_upd > (int64_t)INT32_MAX) {
     err = "LU of the packed system exceeds 2^31 entries";
+    return false;
+  }
+  if (opt.max_lu_nnz > 0 && nnz_lu > opt.max_lu_nnz) {
+    err = "LU fill of the packed system beyond the fill limit of " +
+          std::to_string(opt.max_lu_nnz) + " entries";
     return false;
   }
   s.q.resize(n);

@@ -211,6 +211,9 @@ struct AnalyzeOptions {
   double dense_density = 0.4;
   int solve_band_width = 0;   // solve bands (0: band_width)
   int max_band_tasks = 2048;  // workgroups per band launch
+  // abandon the analysis once L + U hold more entries than this (0: no
+  // limit); the error text then names the fill limit
+  int64_t max_lu_nnz = 0;
 };
 
 // Rebuild the level sets and band programs of the two solves (e.g. for a

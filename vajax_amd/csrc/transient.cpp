@@ -60,6 +60,9 @@ struct Driver {
   int n_lim = 0;
   int64_t total_evals = 0;
   int repivots = 0;
+  // L + U entries of the first analysis the monitor replaces; re-analyses run
+  // under a fill limit of four times this
+  int64_t fill_base = 0;
   double gmin_dev = -1.0;  // device simparam $gmin; < 0: the options' gmin
 
   int upload_scalars(double c0, double c1, double d1, double c2, double gshunt, int it) {
@@ -251,6 +254,12 @@ struct Driver {
     max_f = std::numeric_limits<double>::infinity();
     vjx_pattern* pat = c->pat;
     bool redone = false;  // this iteration already re-pivoted once
+    // at most one re-pivot per solve that does not cure the factor: one that
+    // leaves it non-finite or above the growth threshold (a diverging
+    // iterate, whose Jacobian holds entries of 1e30 and beyond, is no
+    // pivoting problem), or an analysis that fails, disarms the monitor for
+    // the rest of this solve
+    bool armed = true;
     while (!conv && iters < cfg->tran_itl) {
       int rc = upload_scalars(c0, c1, d1, c2, gshunt, iters);
       if (rc < 0) return rc;
@@ -270,13 +279,21 @@ struct Driver {
       const DevStatus& lu_st = w.h_st[1];
       const double maxl = bits_to_double(lu_st.maxl_bits);
       const double growth = cfg->repivot_growth > 0.0 ? cfg->repivot_growth : 1e12;
-      if (lu_st.nonfinite_in == 0 && !redone && (lu_st.nonfinite > 0 || maxl > growth) &&
-          repivots < 64) {
+      const bool bad_factor = lu_st.nonfinite > 0 || maxl > growth;
+      if (redone && bad_factor) armed = false;
+      if (lu_st.nonfinite_in == 0 && !redone && bad_factor && armed && repivots < 64) {
         redone = true;
         std::vector<double> h(c->nse);
         VJX_TRY_HIP(hipMemcpy(h.data(), w.csr.p, sizeof(double) * c->nse, hipMemcpyDeviceToHost));
+        if (!fill_base) fill_base = (int64_t)pat->lu.sym.col.size();
+        const int64_t user_limit = pat->opts.max_lu_nnz;
+        pat->opts.max_lu_nnz =
+            user_limit > 0 ? std::min(user_limit, 4 * fill_base) : 4 * fill_base;
         rc = vjx_lu_analyze(pat, h.data());
-        if (rc != VJX_ERR_SINGULAR) {
+        pat->opts.max_lu_nnz = user_limit;
+        if (rc == VJX_ERR_SINGULAR || rc == VJX_ERR_FILL) {
+          armed = false;  // the previous pivots stay; the iteration stands
+        } else {
           if (rc < 0) return rc;
           rc = pattern_upload(pat);
           if (rc < 0) return rc;

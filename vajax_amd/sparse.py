@@ -70,6 +70,13 @@ class SparseLU:
         L.check(L.lib.vjx_pattern_set_ordering(self.h, code, int(nd_leaf)),
                 "vjx_pattern_set_ordering")
 
+    def set_fill_limit(self, max_lu_nnz: int) -> None:
+        """Later analyses stop with VJX_ERR_FILL (the previous analysis stays
+        in effect) once L + U would hold more than ``max_lu_nnz`` entries;
+        0 = no limit."""
+        L.check(L.lib.vjx_pattern_set_fill_limit(self.h, int(max_lu_nnz)),
+                "vjx_pattern_set_fill_limit")
+
     def check_schedules(self) -> None:
         L.check(L.lib.vjx_lu_check_schedules(self.h), "vjx_lu_check_schedules")
 
