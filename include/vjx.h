@@ -161,8 +161,10 @@ int vjx_lu_check_schedules(const vjx_pattern* pat);
  * the reference (a launch-plan knob). */
 int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width);
 /* Fill-reducing column order of the analysis: 0 = minimum degree on A+A^T,
- * 1 = natural, 2 = nested dissection with minimum-degree leaves of at most
- * nd_leaf rows (0: max(48, n / 128) per diagonal block; default 2, 0).  The
+ * 1 = natural, 2 or 3 = nested dissection with minimum-degree leaves of at
+ * most nd_leaf rows, its vertex separators from BFS level structures (2) or
+ * from multilevel bisection with node refinement (3); nd_leaf 0 = max(48,
+ * n / 128) per diagonal block (2) or 256 (3); default 3, 0.  The
  * role of UMFPACK's ordering choice (vajax/sparse/umfpack_ffi.cpp:140-179
  * caches one symbolic analysis per pattern); a new value drops the analysis
  * like a new band width. */

@@ -66,3 +66,12 @@ def waveform_error(times_a, V_a, times_b, V_b):
         vb = np.interp(times_a, times_b, V_b[:, j])
         err = max(err, float(np.max(np.abs(V_a[:, j] - vb))))
     return err
+
+
+def grid_deviation(times, ref_times):
+    """Largest relative difference between two accepted time grids of the
+    same length (0 where both are 0)."""
+    t = np.asarray(times, dtype=np.float64)
+    r = np.asarray(ref_times, dtype=np.float64)
+    scale = np.maximum(np.abs(r), 1e-300)
+    return float(np.max(np.abs(t - r) / scale))

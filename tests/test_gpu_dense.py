@@ -76,14 +76,18 @@ def _multiplier_jacobian(n):
 
 
 @pytest.mark.parametrize("precision", [L.VJX_PREC_F64, L.VJX_PREC_F32])
-def test_multiplier_refactorisation_with_dense_block(precision):
+@pytest.mark.parametrize("ordering", ["mlnd", "nd"])
+def test_multiplier_refactorisation_with_dense_block(ordering, precision):
     import torch
 
     from vajax_amd.sparse import SparseLU
 
-    print("dense: 8x8 multiplier", file=sys.stderr, flush=True)
-    cir, topo, A, csr = _multiplier_jacobian(8)
+    # the 16x16 multiplier keeps a dense trailing block of 144 rows under the
+    # default (multilevel) ordering and of 518 under BFS level structures
+    print(f"dense: 16x16 multiplier, ordering {ordering}", file=sys.stderr, flush=True)
+    cir, topo, A, csr = _multiplier_jacobian(16)
     lu = SparseLU(topo.indptr, topo.indices)
+    lu.set_ordering(ordering)
     lu.analyze(csr)
     lu.check_schedules()
     info = lu.info()

@@ -14,7 +14,7 @@ from oracle import mna as omna
 from oracle import models as omod
 from vajax_amd import generators as G
 
-from .helpers import run_gpu, run_oracle, waveform_error
+from .helpers import run_gpu, run_oracle, waveform_error, grid_deviation
 
 pytestmark = pytest.mark.gpu
 
@@ -280,7 +280,8 @@ def test_transient_rc_adaptive():
     _, res = run_gpu(net, 10e-9, 1e-12)
     assert res.stats["n_steps"] == len(ref.times)
     assert res.stats["rejected_steps"] == ref.stats["rejected_steps"]
-    assert np.allclose(res.times, ref.times, rtol=1e-12, atol=0)
+    dev = grid_deviation(res.times, ref.times)
+    assert dev <= 1e-12, dev
     assert np.max(np.abs(res.stats["V_out"] - ref.V)) <= 1e-9
 
 
@@ -446,14 +447,17 @@ def test_lu_pipelines_are_bit_identical():
 
 def test_repivot_path_keeps_parity():
     """Force the pivot-growth monitor from the host (growth limit 1e-300):
-    every Newton iteration re-runs the symbolic analysis on its matrix (up to
-    the library's cap of 64), drops the captured graph and redoes the
-    iteration from the saved iterate; the waveform must not change."""
+    the first iteration of every Newton solve re-runs the symbolic analysis
+    on its matrix, drops the captured graph and redoes the iteration from the
+    saved iterate; the redo still exceeds the limit, so the monitor stands
+    down for the rest of that solve (one re-pivot per solve: each of the 40
+    steps; the DAG starts from zero initial conditions, without a DC solve).
+    The waveform must not change."""
     net = G.gate_dag(150, seed=23)
     _, ref = run_oracle(net, 40e-12, 1e-12, fixed=True, tran_fs=1.0)
     _, res = run_gpu(net, 40e-12, 1e-12, fixed=True, tran_fs=1.0, use_graph=True,
                      repivot_growth=1e-300)
-    assert res.stats["repivots"] == 64
+    assert res.stats["repivots"] == len(res.times) - 1 == 40
     assert np.array_equal(res.times, ref.times)
     err = float(np.max(np.abs(res.stats["V_out"] - ref.V)))
     assert err <= 1e-9, err

@@ -31,7 +31,7 @@ CXX_FLAGS = ["-x", "c++", "-O3", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__"
              "-I/opt/rocm/include", "-ffp-contract=off"]
 
 HIP_SOURCES = ["kernels.hip", "transient_kernels.hip", "noalloc.hip", "dense.hip"]
-CXX_SOURCES = ["symbolic.cpp", "api.cpp", "lu.cpp", "circuit.cpp", "transient.cpp"]
+CXX_SOURCES = ["symbolic.cpp", "separator.cpp", "api.cpp", "lu.cpp", "circuit.cpp", "transient.cpp"]
 
 
 def _run(cmd):

@@ -216,14 +216,16 @@ int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width) {
 }
 
 int vjx_pattern_set_ordering(vjx_pattern* pat, int32_t ordering, int32_t nd_leaf) {
-  if (!pat || ordering < 0 || ordering > 2 || (ordering == 2 && nd_leaf != 0 && nd_leaf < 4)) {
-    set_error("vjx_pattern_set_ordering: ordering must be 0 (minimum degree), 1 (natural) or "
-              "2 (nested dissection, leaf 0 = automatic or >= 4)");
+  const bool nd = ordering == 2 || ordering == 3;
+  if (!pat || ordering < 0 || ordering > 3 || (nd && nd_leaf != 0 && nd_leaf < 4)) {
+    set_error("vjx_pattern_set_ordering: ordering must be 0 (minimum degree), 1 (natural), "
+              "2 or 3 (nested dissection, BFS or multilevel separators; leaf 0 = automatic "
+              "or >= 4)");
     return VJX_ERR_ARG;
   }
-  if (pat->opts.ordering != ordering || (ordering == 2 && pat->opts.nd_leaf != nd_leaf)) {
+  if (pat->opts.ordering != ordering || (nd && pat->opts.nd_leaf != nd_leaf)) {
     pat->opts.ordering = ordering;
-    if (ordering == 2) pat->opts.nd_leaf = nd_leaf;
+    if (nd) pat->opts.nd_leaf = nd_leaf;
     pat->lu.analyzed = false;
     pat->lu.factored_prec = -1;
     pat->uploaded = false;

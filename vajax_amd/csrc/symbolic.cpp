@@ -1,6 +1,8 @@
 // Symbolic LU analysis on the host (see symbolic.h for the contract).
 #include "symbolic.h"
 
+#include "separator.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -163,16 +165,31 @@ static void md_subset(const Adj& adj, const std::vector<int32_t>& nodes, std::ve
 
 namespace {
 
-// Nested dissection with BFS level-structure vertex separators.
+// Nested dissection: vertex separators from BFS level structures, or
+// multilevel separators (separator.cpp).
 struct Dissector {
   const Adj& adj;
   int leaf;
+  bool multilevel;
   std::vector<int32_t> stamp, dist, loc;
   std::vector<int32_t>& order;
   int next_stamp = 1;
 
-  Dissector(const Adj& a, int n, int leaf_, std::vector<int32_t>& ord)
-      : adj(a), leaf(leaf_), stamp(n, 0), dist(n, -1), loc(n, -1), order(ord) {}
+  Dissector(const Adj& a, int n, int leaf_, bool ml, std::vector<int32_t>& ord)
+      : adj(a), leaf(leaf_), multilevel(ml), stamp(n, 0), dist(n, -1), loc(n, -1), order(ord) {}
+
+  void split(const std::vector<int32_t>& nodes, const std::vector<int8_t>& part) {
+    std::vector<int32_t> A, B, S;
+    for (size_t i = 0; i < nodes.size(); ++i)
+      (part[i] == 0 ? A : part[i] == 1 ? B : S).push_back(nodes[i]);
+    if (A.empty() || B.empty()) {
+      md_subset(adj, nodes, loc, order);
+      return;
+    }
+    run(A);
+    run(B);
+    md_subset(adj, S, loc, order);
+  }
 
   // BFS inside the current subset (stamp == s); returns visit order and
   // fills dist.
@@ -211,6 +228,12 @@ struct Dissector {
     for (auto& c : comps) clear_dist(c);
     if (comps.size() > 1) {
       for (auto& c : comps) run(c);
+      return;
+    }
+    if (multilevel) {
+      std::vector<int8_t> part;
+      vertex_separator(adj, nodes, loc, (uint64_t)s, part);
+      split(nodes, part);
       return;
     }
     // pseudo-peripheral start: two sweeps to the far end
@@ -297,12 +320,16 @@ static void order_columns(int n, const int32_t* indptr, const int32_t* indices, 
     if (!is_dense[v]) rest.push_back(v);
   order.clear();
   order.reserve(n);
-  if (method == 2) {
-    // larger leaves trade a deeper elimination for less fill: on the 65k-gate
-    // DAG (n = 130k) leaves of ~1000 rows cut the LU updates by 2.7x against
-    // 48-row leaves with fewer levels; small blocks keep 48
-    const int eff = leaf > 0 ? leaf : std::max(48, n / 128);
-    Dissector nd(adj, n, std::max(4, eff), order);
+  if (method == 2 || method == 3) {
+    // BFS separators: larger leaves trade a deeper elimination for less fill
+    // (on the 65k-gate DAG, n = 130k, leaves of ~1000 rows cut the LU updates
+    // by 2.7x against 48-row leaves); small blocks keep 48.  Multilevel
+    // separators are narrow enough that 256-row leaves give the fewest
+    // updates and levels together (65k-gate DAG: 25.9 M updates in 161
+    // levels against 26.9 M in 229 with 1016-row leaves; 64x64 array
+    // multiplier: 50.9 M in 345 against 58.5 M in 451)
+    const int eff = leaf > 0 ? leaf : method == 3 ? 256 : std::max(48, n / 128);
+    Dissector nd(adj, n, std::max(4, eff), method == 3, order);
     nd.run(rest);
   } else {
     std::vector<int32_t> loc(n, -1);

@@ -200,9 +200,12 @@ struct LuSymbolic {
 struct AnalyzeOptions {
   double pivot_tol = 0.1;
   int wide_threshold = 2048;  // levels with at least this many items get a grid
-  int ordering = 2;           // 0 = minimum degree, 1 = natural, 2 = nested dissection
+  int ordering = 3;           // 0 = minimum degree, 1 = natural, 2 = nested
+                              // dissection (BFS level-structure separators),
+                              // 3 = nested dissection (multilevel separators)
   int nd_leaf = 0;            // nested-dissection leaf size (min-degree inside);
-                              // 0: max(48, n / 128) of the block being ordered
+                              // 0: 256 (ordering 3) or max(48, n / 128) of the
+                              // block being ordered (ordering 2)
   int band_width = 16;        // bands of this many levels (0: one launch per level)
   // dense trailing block: the largest trailing block of the LU pattern (at
   // least dense_min rows) whose density reaches dense_density is factorised

@@ -503,7 +503,7 @@ class CircuitEngine:
         if nd_leaf is not None:
             # nested-dissection leaf size of the LU ordering (fill versus
             # depth of the elimination); a change re-runs the analysis
-            self.device.lu.set_ordering("nd", nd_leaf)
+            self.device.lu.set_ordering("mlnd", nd_leaf)
         if band_width is not None:
             # launch plan of the triangular solves (levels per band; 0 = one
             # launch per level); a change re-runs the symbolic analysis

@@ -62,11 +62,13 @@ class SparseLU:
         from the next analysis."""
         L.check(L.lib.vjx_pattern_set_band_width(self.h, int(width)), "vjx_pattern_set_band_width")
 
-    def set_ordering(self, ordering: str = "nd", nd_leaf: int = 0) -> None:
+    def set_ordering(self, ordering: str = "mlnd", nd_leaf: int = 0) -> None:
         """Fill-reducing order of the next analysis: "md" (minimum degree),
-        "natural" or "nd" (nested dissection, minimum-degree leaves of at most
-        ``nd_leaf`` rows; 0 = max(48, n / 128) per diagonal block)."""
-        code = {"md": 0, "natural": 1, "nd": 2}[ordering]
+        "natural", "nd" (nested dissection with BFS level-structure
+        separators) or "mlnd" (nested dissection with multilevel separators,
+        the default); minimum-degree leaves of at most ``nd_leaf`` rows (0 =
+        256 for "mlnd", max(48, n / 128) per diagonal block for "nd")."""
+        code = {"md": 0, "natural": 1, "nd": 2, "mlnd": 3}[ordering]
         L.check(L.lib.vjx_pattern_set_ordering(self.h, code, int(nd_leaf)),
                 "vjx_pattern_set_ordering")
 
