@@ -23,7 +23,9 @@ speed-up is stated per Newton iteration as well.
 
 ``--summarize-trace kernel_trace.csv`` instead prints a markdown table of the
 kernels a rocprofv3 kernel trace holds between the two window markers
-(k_window_mark) of the timed run.
+(k_window_mark) of the timed run; ``--summarize-pmc`` turns FETCH_SIZE and
+WRITE_SIZE counter passes into the HBM bytes per dispatch the roofline's
+``traffic`` reads.
 """
 
 from __future__ import annotations
@@ -160,14 +162,13 @@ KERNEL_NAMES = {"eval": "k_eval<MOSFET> (+R/C)", "stamp": "k_chunk_sum/k_reduce_
                 "lu_dense": "k_dgemm_sub/k_dense_panel (dense trailing block, FP64 MFMA)"}
 
 
-# kernel class -> kernel names in the rocprofv3 traffic summaries
+# kernel class -> kernel-name prefixes in the rocprofv3 traffic summaries
 TRAFFIC_KERNELS = {
-    "eval": ["k_eval<2>"], "stamp": ["k_reduce_jac", "k_residual", "k_chunk_sum", "k_chunk_res"],
-    "lu_gather": ["k_lu_gather<double>"], "lu_tasks": ["k_lu_band_lds<double, 4, 3, 8>", "k_lu_band_lds<double, 1, 2, 2>",
-                 "k_lu_band<double>"],
-    "lu_schur": ["k_lu_wide<double, true>"], "lu_scale": ["k_lu_scale<double>"],
-    "solve_tasks": ["k_tri_band_lde<double, true>", "k_tri_band_lde<double, false>",
-                    "k_tri_band<double, true>", "k_tri_band<double, false>"]}
+    "eval": ["k_eval<"], "stamp": ["k_reduce_jac", "k_residual", "k_chunk_sum", "k_chunk_res"],
+    "lu_gather": ["k_lu_gather<double>"], "lu_tasks": ["k_lu_band_lds<double", "k_lu_band<double>"],
+    "lu_schur": ["k_lu_wide<double"], "lu_scale": ["k_lu_scale<double>"],
+    "solve_tasks": ["k_tri_band_lde<double", "k_tri_band<double"],
+    "lu_dense": ["k_dgemm_sub", "k_dense_panel", "k_dense_diag"]}
 
 
 def measured_traffic(args, cls):
@@ -180,11 +181,59 @@ def measured_traffic(args, cls):
         return None
     with open(path) as fh:
         data = json.load(fh)["kernels"]
-    ks = [data[k] for k in TRAFFIC_KERNELS.get(cls, []) if k in data]
+    ks = [v for k, v in data.items() if any(k.startswith(p) for p in TRAFFIC_KERNELS.get(cls, []))]
     n = sum(k["dispatches"] for k in ks)
     if n == 0:
         return None
     return int(sum(k["traffic_bytes"] * k["dispatches"] for k in ks) / n)
+
+
+def _kernel_name(raw):
+    """Kernel name without the return type, the namespace and the argument
+    list."""
+    name = raw.replace("vjx::(anonymous namespace)::", "").replace("vjx::", "").split("(")[0].strip()
+    return name[5:] if name.startswith("void ") else name
+
+
+def _counter_means(path):
+    """kernel -> (dispatches, mean counter value per dispatch) of a rocprofv3
+    --pmc CSV with one counter."""
+    import csv
+
+    per = {}
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            d = per.setdefault(_kernel_name(r["Kernel_Name"]), {})
+            d[r["Dispatch_Id"]] = d.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    return {k: (len(v), sum(v.values()) / len(v)) for k, v in per.items()}
+
+
+def summarize_pmc(paths, json_out):
+    """HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE passes (KiB per
+    dispatch), each scaled by what the same counter reads for the 1 GiB
+    device-to-device copies of ``--copy-calibration``; writes the JSON that
+    the roofline's ``traffic`` reads and prints a markdown table."""
+    fetch, write, cal_f, cal_w = (_counter_means(p) for p in paths)
+    copy = "__amd_rocclr_copyBuffer"
+    gib_kib = float(1 << 20)
+    ff = gib_kib / cal_f[copy][1]
+    wf = gib_kib / cal_w[copy][1]
+    out = {"fetch_factor": ff, "write_factor": wf, "kernels": {}}
+    for k in sorted(set(fetch) | set(write), key=lambda k: -fetch.get(k, (0, 0.0))[0]):
+        nf, f = fetch.get(k, (0, 0.0))
+        nw, w = write.get(k, (0, 0.0))
+        out["kernels"][k] = {"dispatches": max(nf, nw), "fetch_kib": f, "write_kib": w,
+                             "traffic_bytes": (f * ff + w * wf) * 1024.0}
+    with open(json_out, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(f"* FETCH_SIZE of the copy: {cal_f[copy][1]:.1f} KiB per 1 GiB read -> factor {ff:.4f}")
+    print(f"* WRITE_SIZE of the copy: {cal_w[copy][1]:.1f} KiB per 1 GiB written -> factor "
+          f"{wf:.4f}\n")
+    print("| kernel | dispatches | FETCH KiB | WRITE KiB | traffic MB/dispatch (scaled) |")
+    print("|---|---:|---:|---:|---:|")
+    for k, v in out["kernels"].items():
+        print(f"| `{k}` | {v['dispatches']} | {v['fetch_kib']:.1f} | {v['write_kib']:.1f} | "
+              f"{v['traffic_bytes'] / 1e6:.2f} |")
 
 
 def host_threads() -> int:
@@ -272,37 +321,25 @@ def run_batched(args):
         f"n_unknowns={topo.n_augmented} nse={topo.nse}")
     out = {"workload": net.title, "instances": n_inst, "unknowns": topo.n_augmented,
            "mosfets": int(net.groups["mosfet"].count), "unit": "instance·steps/s"}
+    wb = args.warmup
     for prec in ("f32", "f64"):
         t0 = time.time()
-        run_gpu(eng, args.warmup, prec, True)
-        log(f"batch warm-up {prec} {time.time() - t0:.1f}s")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = run_gpu(eng, args.batch_steps, prec, True)
-        torch.cuda.synchronize()
-        call = time.perf_counter() - t0
-        st = res.stats
-        steps = st["n_steps"] - 1
-        el = st["wall_time"]  # step-loop clock, as for the single circuit
-        out[prec] = {"value": round(n_inst * steps / el, 3), "ms_per_step": round(el / max(steps, 1) * 1e3, 4),
-                     "call_ms_per_step": round(call / max(steps, 1) * 1e3, 4),
-                     "accepted_steps": steps, "newton_iterations": st["total_nr_iterations"],
-                     "newton_iters_per_s": round(st["total_nr_iterations"] / el, 3)}
-        log(f"batch {prec}: {steps} steps in {el:.3f}s")
+        run_gpu(eng, 3, prec, True)  # analysis, launch-plan autotuner, graph capture
+        log(f"batch set-up run {prec} {time.time() - t0:.1f}s")
+        # the same measurement window as the single circuit: untimed steps
+        # through the supply ramp, then the timed accepted steps
+        res = run_gpu(eng, wb + args.batch_steps, prec, True, window_from=wb + 1)
+        w = window_stats(res)
+        out[prec] = {"value": round(n_inst * w["accepted_steps"] / w["loop_s"], 3), **w}
+        log(f"batch {prec}: {w['accepted_steps']} steps in {w['loop_s']:.3f}s")
     # the plain form: one instance at a time, FP64
     single = CircuitEngine(G.gate_dag(args.batch_gates, seed=args.seed))
     single.parse()
-    run_gpu(single, args.warmup, "f64", True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res = run_gpu(single, args.batch_steps, "f64", True)
-    torch.cuda.synchronize()
-    call = time.perf_counter() - t0
-    steps = res.stats["n_steps"] - 1
-    el = res.stats["wall_time"]
-    out["single_f64"] = {"value": round(steps / el, 3), "ms_per_step": round(el / max(steps, 1) * 1e3, 4),
-                         "call_ms_per_step": round(call / max(steps, 1) * 1e3, 4),
-                         "accepted_steps": steps, "workload": f"gate_dag_{args.batch_gates}_s{args.seed}"}
+    run_gpu(single, 3, "f64", True)
+    res = run_gpu(single, wb + args.batch_steps, "f64", True, window_from=wb + 1)
+    w = window_stats(res)
+    out["single_f64"] = {"value": round(w["accepted_steps"] / w["loop_s"], 3), **w,
+                         "workload": f"gate_dag_{args.batch_gates}_s{args.seed}"}
     info = eng.device.lu.info()
     out["lu"] = {k: info[k] for k in ("n_blocks", "n_block_patterns", "nnz_l", "nnz_u", "n_updates",
                                       "fact_levels")}
@@ -314,9 +351,10 @@ def run_batched(args):
     return out
 
 
-def summarize_trace(path):
-    """Markdown table of the kernels in a rocprofv3 kernel-trace CSV between
-    the first and the last k_window_mark dispatch (the timed window)."""
+def summarize_trace(path, window=0):
+    """Markdown table of the kernels in a rocprofv3 kernel-trace CSV inside
+    one timed window: between the start and end k_window_mark dispatches of
+    the window-th timed run of the process (0: the first)."""
     import csv
 
     with open(path, newline="") as fh:
@@ -324,14 +362,15 @@ def summarize_trace(path):
     key = "Start_Timestamp" if "Start_Timestamp" in rows[0] else "Start_Time"
     endk = "End_Timestamp" if "End_Timestamp" in rows[0] else "End_Time"
     rows.sort(key=lambda r: int(r[key]))
-    marks = [i for i, r in enumerate(rows) if r["Kernel_Name"].startswith("k_window_mark")]
-    if len(marks) < 2:
-        raise SystemExit("no window markers in the trace")
-    win = rows[marks[0] + 1:marks[-1]]
-    t0, t1 = int(rows[marks[0]][endk]), int(rows[marks[-1]][key])
+    marks = [i for i, r in enumerate(rows) if _kernel_name(r["Kernel_Name"]).startswith("k_window_mark")]
+    if len(marks) < 2 * window + 2:
+        raise SystemExit(f"the trace holds {len(marks) // 2} windows, not {window + 1}")
+    a, b = marks[2 * window], marks[2 * window + 1]
+    win = rows[a + 1:b]
+    t0, t1 = int(rows[a][endk]), int(rows[b][key])
     agg = {}
     for r in win:
-        name = r["Kernel_Name"].split("(")[0].replace("vjx::(anonymous namespace)::", "")
+        name = _kernel_name(r["Kernel_Name"])
         d = int(r[endk]) - int(r[key])
         a = agg.setdefault(name, [0, 0])
         a[0] += 1
@@ -369,14 +408,24 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="Monte Carlo instances (0: skip)")
     ap.add_argument("--batch-gates", type=int, default=2000)
     ap.add_argument("--batch-ragged", default="500,1000,2000,4000")
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=50,
+                    help="timed accepted steps of the batch (after --warmup untimed ones)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers (reference kernel for HBM counters)")
     ap.add_argument("--summarize-trace", default=None,
                     help="print the timed window of a rocprofv3 kernel_trace.csv and exit")
+    ap.add_argument("--window", type=int, default=0,
+                    help="with --summarize-trace: which timed window of the process (0: first)")
+    ap.add_argument("--summarize-pmc", default=None,
+                    help="FETCH,WRITE,CAL_FETCH,CAL_WRITE: four rocprofv3 --pmc CSVs (the bench "
+                         "and --copy-calibration); write --pmc-json, print a table and exit")
+    ap.add_argument("--pmc-json", default="hbm_traffic.json")
     args = ap.parse_args()
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace)
+        summarize_trace(args.summarize_trace, args.window)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc.split(","), args.pmc_json)
         return
     if args.copy_calibration:
         import torch
