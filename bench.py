@@ -351,6 +351,79 @@ def run_batched(args):
     return out
 
 
+MFMA_F64_PEAK_TFLOPS = 78.6
+MUL64_REFERENCE_MS_PER_STEP = 648.0  # JAX GPU, cuDSS f32 + refinement, Tesla T4, PSP103
+
+
+def run_mul64(args):
+    """The reference's own benchmark circuit, mul64
+    (vajax/benchmarks/data/mul64/runme.sim: the 64x64 array multiplier of its
+    generate block, every input driven high through 1 ohm, a uic transient to
+    2 ns with 2 ps steps and tran_lteratio = 1.5), built by this build's
+    generator of the same topology, with the EKV substitute for PSP103 (no
+    internal nodes: 133 592 unknowns where the reference's PSP103 circuit has
+    666 409).  A set-up run (analysis, autotuner, graph), then the whole
+    transient timed on the step loop's clock, then a short eager pass for the
+    kernel split, including the dense trailing block on the FP64 matrix
+    cores."""
+    from vajax_amd import _lib as L
+    from vajax_amd import generators as G
+    from vajax_amd import vacask
+    from vajax_amd.engine import CircuitEngine
+
+    t0 = time.time()
+    net = vacask.load("", text=G.array_multiplier_testbench(64))
+    eng = CircuitEngine(net)
+    eng.parse()
+    topo = eng.topology
+    log(f"mul64 setup {time.time() - t0:.1f}s: n_unknowns={topo.n_augmented} nse={topo.nse}")
+
+    def run(use_graph, max_steps=None, window_from=0):
+        eng.prepare(t_stop=2e-9, dt=2e-12, precision="f64", use_graph=use_graph, autotune=True,
+                    max_steps=max_steps, window_from_step=window_from)
+        return eng.run_transient()
+
+    t0 = time.time()
+    run(True, max_steps=3)
+    setup_s = time.time() - t0
+    log(f"mul64 analysis + autotune {setup_s:.1f}s")
+    res = run(True, window_from=1)
+    w = window_stats(res)
+    log(f"mul64: {w['accepted_steps']} steps in {w['loop_s']:.2f}s")
+    L.lib.vjx_profile_enable(1)
+    L.lib.vjx_profile_reset()
+    run(False, max_steps=60)
+    pt = L.PhaseTimes()
+    L.lib.vjx_profile_read(C.byref(pt))
+    L.lib.vjx_profile_enable(0)
+    ph = pt.as_dict()
+    iters = max(ph["n_iterations"], 1)
+    info = eng.device.lu.info()
+    out = {"workload": "mul64 (64x64 array multiplier, reference benchmark topology)",
+           "mosfets": int(net.groups["mosfet"].count), "unknowns": topo.n_augmented,
+           "csr_nnz": topo.nse, "t_stop_ns": 2.0, "dt_ps": 2.0, **w,
+           "end_time_ns": round(float(res.times[-1]) * 1e9, 4), "analysis_s": round(setup_s, 2),
+           "lu": {k: info[k] for k in ("nnz_l", "nnz_u", "n_updates", "fact_levels", "dense_k",
+                                       "dense_m")},
+           "kernel_ms_per_newton_iter": {name: round(kt["ms"] / iters, 4)
+                                         for name, kt in ph["kernels"].items() if kt["launches"]},
+           "reference_ms_per_step": MUL64_REFERENCE_MS_PER_STEP,
+           "reference_note": "JAX GPU (cuDSS FP32 factor + FP64 refinement) on a Tesla T4 with "
+                             "PSP103 (666 409 unknowns); this run: EKV substitute, 133 592 "
+                             "unknowns, MI355X -- different model and hardware"}
+    dense = ph["kernels"].get("lu_dense")
+    if dense and dense["launches"]:
+        ms = dense["ms"] / iters  # one refactorisation per Newton iteration
+        tf = info["dense_flops"] / (ms * 1e-3) / 1e12
+        out["dense_block"] = {
+            "kernels": KERNEL_NAMES["lu_dense"], "k": info["dense_k"], "m": info["dense_m"],
+            "flops_per_refactor": info["dense_flops"], "ms_per_refactor": round(ms, 4),
+            "launches_per_refactor": round(dense["launches"] / iters, 1),
+            "achieved_TFLOPs": round(tf, 4), "peak_TFLOPs": MFMA_F64_PEAK_TFLOPS,
+            "frac": round(tf / MFMA_F64_PEAK_TFLOPS, 6)}
+    return out
+
+
 def summarize_trace(path, window=0):
     """Markdown table of the kernels in a rocprofv3 kernel-trace CSV inside
     one timed window: between the start and end k_window_mark dispatches of
@@ -410,6 +483,10 @@ def main():
     ap.add_argument("--batch-ragged", default="500,1000,2000,4000")
     ap.add_argument("--batch-steps", type=int, default=50,
                     help="timed accepted steps of the batch (after --warmup untimed ones)")
+    ap.add_argument("--no-mul64", action="store_true",
+                    help="skip the mul64 leg (the reference's 64x64 multiplier benchmark)")
+    ap.add_argument("--mul64-only", action="store_true",
+                    help="run only the mul64 leg and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers (reference kernel for HBM counters)")
     ap.add_argument("--summarize-trace", default=None,
@@ -444,6 +521,9 @@ def main():
 
     import torch
 
+    if args.mul64_only:
+        print(json.dumps(run_mul64(args)), flush=True)
+        return
     from vajax_amd import _lib as L
     from vajax_amd.engine import CircuitEngine
 
@@ -556,6 +636,10 @@ def main():
     if args.batch > 0:
         batched = run_batched(args)
 
+    mul64 = None
+    if not args.no_mul64:
+        mul64 = run_mul64(args)
+
     line = {
         "metric": METRIC,
         "value": win["newton_iters_per_s"],
@@ -596,6 +680,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "mul64": mul64,
         "mixed_precision": mixed,
     }
     if cpu:
