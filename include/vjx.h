@@ -176,6 +176,12 @@ int vjx_pattern_set_ordering(vjx_pattern* pat, int32_t ordering, int32_t nd_leaf
  * counterpart in the reference (UMFPACK's numeric factorisation re-pivots
  * within its fixed symbolic analysis, vajax/sparse/umfpack_ffi.cpp:287-316). */
 int vjx_pattern_set_fill_limit(vjx_pattern* pat, int64_t max_lu_nnz);
+/* Dense trailing block of later analyses: the largest trailing block of the
+ * LU pattern with at least min_rows rows whose density reaches `density`
+ * (0 < density <= 1) is factorised as a dense matrix on the FP64 matrix
+ * cores; min_rows 0 = never (default 96, 0.4).  A change drops the analysis.
+ * No counterpart in the reference (a supernode-shape choice). */
+int vjx_pattern_set_dense(vjx_pattern* pat, int32_t min_rows, double density);
 /* Autotuner of the launch plan: with the matrix data_dev (device), time the
  * numeric refactorisation for refactorisation bands of 12, 16 and 24 pivot
  * levels (a fresh analysis each: same ordering and pivots) and each LU

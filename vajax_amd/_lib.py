@@ -184,6 +184,7 @@ _PROTOS = {
     "vjx_pattern_set_band_width": (C.c_int, [vp, C.c_int32]),
     "vjx_pattern_set_ordering": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "vjx_pattern_set_fill_limit": (C.c_int, [vp, C.c_int64]),
+    "vjx_pattern_set_dense": (C.c_int, [vp, C.c_int32, C.c_double]),
     "vjx_lu_autotune": (C.c_int, [vp, vp]),
     "vjx_lu_refactor_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p]),
     "vjx_lu_solve_cpu": (C.c_int, [vp, f64p, C.c_int32, f64p, f64p]),

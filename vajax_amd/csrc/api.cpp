@@ -242,6 +242,21 @@ int vjx_pattern_set_fill_limit(vjx_pattern* pat, int64_t max_lu_nnz) {
   return VJX_OK;
 }
 
+int vjx_pattern_set_dense(vjx_pattern* pat, int32_t min_rows, double density) {
+  if (!pat || min_rows < 0 || !(density > 0.0 && density <= 1.0)) {
+    set_error("vjx_pattern_set_dense: min_rows >= 0 and 0 < density <= 1");
+    return VJX_ERR_ARG;
+  }
+  if (pat->opts.dense_min != min_rows || pat->opts.dense_density != density) {
+    pat->opts.dense_min = min_rows;
+    pat->opts.dense_density = density;
+    pat->lu.analyzed = false;
+    pat->lu.factored_prec = -1;
+    pat->uploaded = false;
+  }
+  return VJX_OK;
+}
+
 int vjx_lu_get_pattern(const vjx_pattern* pat, int32_t* rowptr, int32_t* col) {
   if (!pat || !pat->lu.analyzed) {
     set_error("vjx_lu_get_pattern: no analysis yet");

@@ -71,9 +71,12 @@ inline int reduction_class(int len) {
 // and the chunk partials are then added in chunk order.
 constexpr int kHubChunk = 4096;
 constexpr int kMaxBandWidth = 64;  // levels per band (LDS table of one task)
-// LU values a band task may hold in LDS: 60 KiB in FP64, so that with the
-// kernel's own reduction and level tables one workgroup stays within 64 KiB
-constexpr int kLdsSlots = 7680;
+// LU values a band task may hold in LDS: the 160 KiB per workgroup of gfx950
+// less 4 KiB for the kernel's own reduction and level tables, in FP64 (19 968
+// values; 16-bit local indices).  A band launch asks for what its largest
+// task needs, so bands of small tasks keep their small footprint; a band with
+// a task beyond this runs from HBM.
+constexpr int kLdsSlots = (160 * 1024 - 4096) / 8;
 // Static LDS of the entries-in-LDS solve kernel (reduction tree and level
 // tables), kept out of the dynamic budget with margin.
 constexpr int kTriStaticLds = 4096;

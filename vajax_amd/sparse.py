@@ -79,6 +79,13 @@ class SparseLU:
         L.check(L.lib.vjx_pattern_set_fill_limit(self.h, int(max_lu_nnz)),
                 "vjx_pattern_set_fill_limit")
 
+    def set_dense(self, min_rows: int = 96, density: float = 0.4) -> None:
+        """Dense trailing block of the next analysis: the largest trailing
+        block of at least ``min_rows`` rows with LU density >= ``density``
+        (``min_rows`` 0: none)."""
+        L.check(L.lib.vjx_pattern_set_dense(self.h, int(min_rows), float(density)),
+                "vjx_pattern_set_dense")
+
     def check_schedules(self) -> None:
         L.check(L.lib.vjx_lu_check_schedules(self.h), "vjx_lu_check_schedules")
 
