@@ -9,20 +9,19 @@
 //      scalar program left them), Lp / Up from the L columns and U rows of the
 //      earlier pivots with large contributions to the block;
 //   2. D -= Lp Up (FP64 MFMA);
-//   3. D = L U in place, blocked by panels of kDensePanel columns: the
-//      panel's diagonal block in LDS, then L21 = A21 U11^-1 row by row and
-//      U12 = L11^-1 A12 column by column (all rows and columns in parallel),
-//      then A22 -= L21 U12 (FP64 MFMA);
+//   3. D = L U in place, blocked by panels of kDensePanel columns: one launch
+//      per panel factorises its diagonal block in LDS (in every workgroup)
+//      and solves 64-row tiles of L21 = A21 U11^-1 and 64-column tiles of
+//      U12 = L11^-1 A12, then A22 -= L21 U12 (FP64 MFMA);
 //   4. the slots written back (FP64 or FP32 factors), folding max |L| and the
 //      non-finite count into the pivot-growth status.
 //
 // The matrix product: C[M x N] -= A[M x Kd] B[Kd x N], row-major with leading
-// dimensions, one 64 x 64 tile of C per workgroup of four wavefronts, each a
-// 32 x 32 sub-tile as 2 x 2 v_mfma_f64_16x16x4f64 accumulators; A and B in
-// 16-deep slices through LDS.  Operand layout of v_mfma_f64_16x16x4f64
-// (wave64): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
-// D[(l / 16) + 4 r][l % 16], r = 0..3: accumulator rows interleave across the
-// four 16-lane groups.
+// dimensions, one 32 x 32 tile of C per workgroup of four wavefronts that
+// split Kd and add their partial tiles in a fixed order.  Operand layout of
+// v_mfma_f64_16x16x4f64 (wave64): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[(l / 16) + 4 r][l % 16], r = 0..3:
+// accumulator rows interleave across the four 16-lane groups.
 #include "block_flags.h"
 #include "internal.h"
 
@@ -38,110 +37,128 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
+// C[M x N] -= A[M x Kd] B[Kd x N]: one 32 x 32 tile of C per workgroup; each
+// of its four wavefronts accumulates the whole tile (2 x 2 blocks of
+// v_mfma_f64_16x16x4f64) over every fourth 4-deep step of Kd, reading its
+// operands straight from memory in the instruction's lane layout, and the
+// four partial tiles are added in wavefront order through LDS before C is
+// updated (a fixed summation order: the result does not depend on timing).
 __global__ void __launch_bounds__(256) k_dgemm_sub(int M, int N, int Kd,
                                                    const double* __restrict__ A, int lda,
                                                    const double* __restrict__ B, int ldb,
                                                    double* C, int ldc) {
-  __shared__ double As[64][17];
-  __shared__ double Bs[16][64];
+  __shared__ double part[4][16][64];
   const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = (int)blockIdx.y * 64, n0 = (int)blockIdx.x * 64;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int m0 = (int)blockIdx.y * 32, n0 = (int)blockIdx.x * 32;
   dbl4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < Kd; k0 += 16) {
+  const int ar = lane & 15, ak = lane >> 4;  // A[row][k] / B[k][col] of this lane
+  for (int k0 = wave * 4; k0 < Kd; k0 += 16) {
+    const int k = k0 + ak;
+    double a[2], b[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = tid + u * 256;
-      const int r = q >> 4, c = q & 15, gr = m0 + r, gc = k0 + c;
-      As[r][c] = (gr < M && gc < Kd) ? A[(size_t)gr * lda + gc] : 0.0;
+    for (int t = 0; t < 2; ++t) {
+      const int r = m0 + t * 16 + ar, c = n0 + t * 16 + ar;
+      a[t] = (r < M && k < Kd) ? A[(size_t)r * lda + k] : 0.0;
+      b[t] = (k < Kd && c < N) ? B[(size_t)k * ldb + c] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = tid + u * 256;
-      const int r = q >> 6, c = q & 63, gr = k0 + r, gc = n0 + c;
-      Bs[r][c] = (gr < Kd && gc < N) ? B[(size_t)gr * ldb + gc] : 0.0;
-    }
-    __syncthreads();
+    for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int kk = 0; kk < 16; kk += 4) {
-      double a[2], b[2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        a[t] = As[wr + t * 16 + (lane & 15)][kk + (lane >> 4)];
-        b[t] = Bs[kk + (lane >> 4)][wc + t * 16 + (lane & 15)];
-      }
-#pragma unroll
-      for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
-    }
-    __syncthreads();
+      for (int tj = 0; tj < 2; ++tj)
+        acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
   }
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wr + ti * 16 + (lane >> 4) + 4 * r;
-        const int col = n0 + wc + tj * 16 + (lane & 15);
-        if (row < M && col < N) C[(size_t)row * ldc + col] -= acc[ti][tj][r];
-      }
+      for (int r = 0; r < 4; ++r) part[wave][ti * 8 + tj * 4 + r][lane] = acc[ti][tj][r];
+  __syncthreads();
+  // thread -> (accumulator element, lane); lane l of element (ti, tj, r) is
+  // C[m0 + 16 ti + l / 16 + 4 r][n0 + 16 tj + l % 16]
+  for (int q = tid; q < 16 * 64; q += 256) {
+    const int e = q >> 6, l = q & 63;
+    const double v = ((part[0][e][l] + part[1][e][l]) + part[2][e][l]) + part[3][e][l];
+    const int ti = e >> 3, tj = (e >> 2) & 1, r = e & 3;
+    const int row = m0 + ti * 16 + (l >> 4) + 4 * r;
+    const int col = n0 + tj * 16 + (l & 15);
+    if (row < M && col < N) C[(size_t)row * ldc + col] -= v;
+  }
 }
 
-// The panel's w x w diagonal block (rows and columns o .. o+w-1) factorised in
-// place in LDS by one workgroup: unit L below the diagonal, U on and above.
-__global__ void __launch_bounds__(256) k_dense_diag(int K, int o, int w, double* D) {
-  __shared__ double P[kDensePanel][kDensePanel + 1];
+// One panel of the blocked LU (columns o .. o+w-1, w <= kDensePanel): every
+// workgroup loads the w x w diagonal block into LDS and factorises it (unit
+// L below the diagonal, U on and above; workgroup 0 writes it back), then
+// solves one 64-row tile of L21 = A21 U11^-1 (workgroups [0, nrb)) or one
+// 64-column tile of U12 = L11^-1 A12 (the next ncb), staged through LDS
+// with coalesced loads and stores; one thread per row / column runs the
+// substitution in the order of the host program.
+__global__ void __launch_bounds__(256) k_dense_panel(int K, int o, int w, int nrb, int ncb,
+                                                     double* D) {
+  constexpr int W = kDensePanel;
+  __shared__ double P[W][W + 1];
+  __shared__ double T[64][W + 1];
   const int tid = (int)threadIdx.x;
-  for (int q = tid; q < w * w; q += 256) P[q / w][q % w] = D[(size_t)(o + q / w) * K + o + q % w];
+  for (int q = tid; q < W * W; q += 256) {
+    const int r = q / W, c = q % W;
+    if (r < w && c < w) P[r][c] = D[(size_t)(o + r) * K + o + c];
+  }
   __syncthreads();
   for (int c = 0; c < w; ++c) {
     if (tid > c && tid < w) P[tid][c] /= P[c][c];
     __syncthreads();
-    for (int q = tid; q < w * w; q += 256) {
-      const int r = q / w, c2 = q % w;
-      if (r > c && c2 > c) P[r][c2] -= P[r][c] * P[c][c2];
+    for (int q = tid; q < W * W; q += 256) {
+      const int r = q / W, c2 = q % W;
+      if (r > c && r < w && c2 > c && c2 < w) P[r][c2] -= P[r][c] * P[c][c2];
     }
     __syncthreads();
   }
-  for (int q = tid; q < w * w; q += 256) D[(size_t)(o + q / w) * K + o + q % w] = P[q / w][q % w];
-}
-
-// L21 = A21 U11^-1 (blocks [0, nbr): one row per thread) and U12 = L11^-1 A12
-// (the other blocks: one column per thread), reading the factorised diagonal
-// block into LDS.
-__global__ void __launch_bounds__(256) k_dense_panel(int K, int o, int w, int nbr, double* D) {
-  __shared__ double P[kDensePanel][kDensePanel + 1];
-  const int tid = (int)threadIdx.x;
-  for (int q = tid; q < w * w; q += 256) P[q / w][q % w] = D[(size_t)(o + q / w) * K + o + q % w];
-  __syncthreads();
-  if ((int)blockIdx.x < nbr) {
-    const int r = o + w + (int)blockIdx.x * 256 + tid;
-    if (r >= K) return;
-    double x[kDensePanel];
-    double* row = D + (size_t)r * K + o;
-    for (int c = 0; c < w; ++c) {
-      double v = row[c];
-      for (int c2 = 0; c2 < c; ++c2) v -= x[c2] * P[c2][c];
-      x[c] = v / P[c][c];
+  const int b = (int)blockIdx.x;
+  if (b == 0)
+    for (int q = tid; q < W * W; q += 256) {
+      const int r = q / W, c = q % W;
+      if (r < w && c < w) D[(size_t)(o + r) * K + o + c] = P[r][c];
     }
-    for (int c = 0; c < w; ++c) row[c] = x[c];
-  } else {
-    const int j = o + w + ((int)blockIdx.x - nbr) * 256 + tid;
-    if (j >= K) return;
-    double y[kDensePanel];
-    for (int r = 0; r < w; ++r) {
-      double v = D[(size_t)(o + r) * K + j];
-      for (int r2 = 0; r2 < r; ++r2) v -= P[r][r2] * y[r2];
-      y[r] = v;
+  if (b < nrb) {
+    const int r0 = o + w + b * 64, nr = min(64, K - r0);
+    for (int q = tid; q < 64 * W; q += 256) {
+      const int i = q / W, c = q % W;
+      if (i < nr && c < w) T[i][c] = D[(size_t)(r0 + i) * K + o + c];
     }
-    for (int r = 0; r < w; ++r) D[(size_t)(o + r) * K + j] = y[r];
+    __syncthreads();
+    if (tid < nr)
+      for (int c = 0; c < w; ++c) {
+        double v = T[tid][c];
+        for (int c2 = 0; c2 < c; ++c2) v -= T[tid][c2] * P[c2][c];
+        T[tid][c] = v / P[c][c];
+      }
+    __syncthreads();
+    for (int q = tid; q < 64 * W; q += 256) {
+      const int i = q / W, c = q % W;
+      if (i < nr && c < w) D[(size_t)(r0 + i) * K + o + c] = T[i][c];
+    }
+  } else if (b < nrb + ncb) {
+    const int j0 = o + w + (b - nrb) * 64, nc = min(64, K - j0);
+    for (int q = tid; q < W * 64; q += 256) {
+      const int r = q / 64, j = q % 64;
+      if (r < w && j < nc) T[j][r] = D[(size_t)(o + r) * K + j0 + j];
+    }
+    __syncthreads();
+    if (tid < nc)
+      for (int r = 0; r < w; ++r) {
+        double v = T[tid][r];
+        for (int r2 = 0; r2 < r; ++r2) v -= P[r][r2] * T[tid][r2];
+        T[tid][r] = v;
+      }
+    __syncthreads();
+    for (int q = tid; q < W * 64; q += 256) {
+      const int r = q / 64, j = q % 64;
+      if (r < w && j < nc) D[(size_t)(o + r) * K + j0 + j] = T[j][r];
+    }
   }
 }
 
@@ -190,18 +207,17 @@ __global__ void __launch_bounds__(256) k_dense_scatter(int n, int K,
 void launch_dgemm_sub(int M, int N, int Kd, const double* A, int lda, const double* B, int ldb,
                       double* C, int ldc, hipStream_t s) {
   if (M <= 0 || N <= 0 || Kd <= 0) return;
-  dim3 grid((unsigned)nblk(N, 64), (unsigned)nblk(M, 64));
+  dim3 grid((unsigned)nblk(N, 32), (unsigned)nblk(M, 32));
   k_dgemm_sub<<<grid, 256, 0, s>>>(M, N, Kd, A, lda, B, ldb, C, ldc);
 }
 
 void launch_dense_lu(int K, double* D, hipStream_t s) {
   for (int o = 0; o < K; o += kDensePanel) {
     const int w = std::min(kDensePanel, K - o);
-    k_dense_diag<<<1, 256, 0, s>>>(K, o, w, D);
     const int nt = K - o - w;
+    const int nb = nblk(nt, 64);
+    k_dense_panel<<<std::max(1, 2 * nb), 256, 0, s>>>(K, o, w, nb, nb, D);
     if (nt <= 0) break;
-    const int nbr = nblk(nt, 256);
-    k_dense_panel<<<2 * nbr, 256, 0, s>>>(K, o, w, nbr, D);
     launch_dgemm_sub(nt, nt, w, D + (size_t)(o + w) * K + o, K, D + (size_t)o * K + o + w, K,
                      D + (size_t)(o + w) * K + o + w, K, s);
   }
