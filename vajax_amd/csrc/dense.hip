@@ -11,8 +11,8 @@
 //   2. D -= Lp Up (FP64 MFMA);
 //   3. D = L U in place, blocked by panels of kDensePanel columns: one launch
 //      per panel factorises its diagonal block in LDS (in every workgroup)
-//      and solves 64-row tiles of L21 = A21 U11^-1 and 64-column tiles of
-//      U12 = L11^-1 A12, then A22 -= L21 U12 (FP64 MFMA);
+//      and solves the rows of L21 = A21 U11^-1 and the columns of
+//      U12 = L11^-1 A12 one per thread, then A22 -= L21 U12 (FP64 MFMA);
 //   4. the slots written back (FP64 or FP32 factors), folding max |L| and the
 //      non-finite count into the pivot-growth status.
 //
@@ -36,6 +36,14 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 }
 
 inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// Lane `lane`'s value of v (lane uniform across the wavefront).
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)bits, lane);
+  const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(bits >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // C[M x N] -= A[M x Kd] B[Kd x N]: one 32 x 32 tile of C per workgroup; each
 // of its four wavefronts accumulates the whole tile (2 x 2 blocks of
@@ -90,75 +98,86 @@ __global__ void __launch_bounds__(256) k_dgemm_sub(int M, int N, int Kd,
   }
 }
 
-// One panel of the blocked LU (columns o .. o+w-1, w <= kDensePanel): every
-// workgroup loads the w x w diagonal block into LDS and factorises it (unit
-// L below the diagonal, U on and above; workgroup 0 writes it back), then
-// solves one 64-row tile of L21 = A21 U11^-1 (workgroups [0, nrb)) or one
-// 64-column tile of U12 = L11^-1 A12 (the next ncb), staged through LDS
-// with coalesced loads and stores; one thread per row / column runs the
-// substitution in the order of the host program.
+// One panel of the blocked LU (columns o .. o+w-1, w <= kDensePanel): the
+// first wavefront of every workgroup factorises the w x w diagonal block in
+// registers (unit L below the diagonal, U on and above; workgroup 0 writes it
+// back) and leaves it in LDS, then
+// each thread solves one row of L21 = A21 U11^-1 (workgroups [0, nrb), 256
+// rows each) or one column of U12 = L11^-1 A12 (the next ncb, 256 columns
+// each) in registers, by substitution in the order of the host program (the
+// loops are unrolled over kDensePanel, so the row stays in registers and the
+// diagonal block's LDS reads are issued ahead of the dependent chain).
 __global__ void __launch_bounds__(256) k_dense_panel(int K, int o, int w, int nrb, int ncb,
                                                      double* D) {
   constexpr int W = kDensePanel;
   __shared__ double P[W][W + 1];
-  __shared__ double T[64][W + 1];
   const int tid = (int)threadIdx.x;
-  for (int q = tid; q < W * W; q += 256) {
-    const int r = q / W, c = q % W;
-    if (r < w && c < w) P[r][c] = D[(size_t)(o + r) * K + o + c];
+  const int b = (int)blockIdx.x;
+  double x[W];
+  if (tid < 64) {
+    // wavefront 0: lane r holds row r of the diagonal block; step c divides
+    // the rows below c by the pivot and subtracts the multiple of row c,
+    // whose entries every lane reads from lane c (no barrier between steps)
+#pragma unroll
+    for (int k = 0; k < W; ++k)
+      x[k] = (tid < w && k < w) ? D[(size_t)(o + tid) * K + o + k] : 0.0;
+    const bool below = tid < w;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      if (c < w) {
+        const double piv = lane_value(x[c], c);
+        const bool act = below && tid > c;
+        const double l = act ? x[c] / piv : x[c];
+        x[c] = l;
+#pragma unroll
+        for (int k = c + 1; k < W; ++k) {
+          const double u = lane_value(x[k], c);
+          if (act) x[k] -= l * u;
+        }
+      }
+    }
+    if (below) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        P[tid][k] = x[k];
+        if (b == 0 && k < w) D[(size_t)(o + tid) * K + o + k] = x[k];
+      }
+    }
   }
   __syncthreads();
-  for (int c = 0; c < w; ++c) {
-    if (tid > c && tid < w) P[tid][c] /= P[c][c];
-    __syncthreads();
-    for (int q = tid; q < W * W; q += 256) {
-      const int r = q / W, c2 = q % W;
-      if (r > c && r < w && c2 > c && c2 < w) P[r][c2] -= P[r][c] * P[c][c2];
-    }
-    __syncthreads();
-  }
-  const int b = (int)blockIdx.x;
-  if (b == 0)
-    for (int q = tid; q < W * W; q += 256) {
-      const int r = q / W, c = q % W;
-      if (r < w && c < w) D[(size_t)(o + r) * K + o + c] = P[r][c];
-    }
   if (b < nrb) {
-    const int r0 = o + w + b * 64, nr = min(64, K - r0);
-    for (int q = tid; q < 64 * W; q += 256) {
-      const int i = q / W, c = q % W;
-      if (i < nr && c < w) T[i][c] = D[(size_t)(r0 + i) * K + o + c];
-    }
-    __syncthreads();
-    if (tid < nr)
-      for (int c = 0; c < w; ++c) {
-        double v = T[tid][c];
-        for (int c2 = 0; c2 < c; ++c2) v -= T[tid][c2] * P[c2][c];
-        T[tid][c] = v / P[c][c];
+    const int r = o + w + b * 256 + tid;
+    if (r >= K) return;
+    double* row = D + (size_t)r * K + o;
+#pragma unroll
+    for (int c = 0; c < W; ++c) x[c] = c < w ? row[c] : 0.0;
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      if (c < w) {
+        double v = x[c];
+#pragma unroll
+        for (int c2 = 0; c2 < c; ++c2) v -= x[c2] * P[c2][c];
+        x[c] = v / P[c][c];
       }
-    __syncthreads();
-    for (int q = tid; q < 64 * W; q += 256) {
-      const int i = q / W, c = q % W;
-      if (i < nr && c < w) D[(size_t)(r0 + i) * K + o + c] = T[i][c];
-    }
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      if (c < w) row[c] = x[c];
   } else if (b < nrb + ncb) {
-    const int j0 = o + w + (b - nrb) * 64, nc = min(64, K - j0);
-    for (int q = tid; q < W * 64; q += 256) {
-      const int r = q / 64, j = q % 64;
-      if (r < w && j < nc) T[j][r] = D[(size_t)(o + r) * K + j0 + j];
-    }
-    __syncthreads();
-    if (tid < nc)
-      for (int r = 0; r < w; ++r) {
-        double v = T[tid][r];
-        for (int r2 = 0; r2 < r; ++r2) v -= P[r][r2] * T[tid][r2];
-        T[tid][r] = v;
+    const int j = o + w + (b - nrb) * 256 + tid;
+    if (j >= K) return;
+#pragma unroll
+    for (int r = 0; r < W; ++r) x[r] = r < w ? D[(size_t)(o + r) * K + j] : 0.0;
+#pragma unroll
+    for (int r = 0; r < W; ++r)
+      if (r < w) {
+        double v = x[r];
+#pragma unroll
+        for (int r2 = 0; r2 < r; ++r2) v -= P[r][r2] * x[r2];
+        x[r] = v;
       }
-    __syncthreads();
-    for (int q = tid; q < W * 64; q += 256) {
-      const int r = q / 64, j = q % 64;
-      if (r < w && j < nc) D[(size_t)(o + r) * K + j0 + j] = T[j][r];
-    }
+#pragma unroll
+    for (int r = 0; r < W; ++r)
+      if (r < w) D[(size_t)(o + r) * K + j] = x[r];
   }
 }
 
@@ -215,7 +234,7 @@ void launch_dense_lu(int K, double* D, hipStream_t s) {
   for (int o = 0; o < K; o += kDensePanel) {
     const int w = std::min(kDensePanel, K - o);
     const int nt = K - o - w;
-    const int nb = nblk(nt, 64);
+    const int nb = nblk(nt, 256);
     k_dense_panel<<<std::max(1, 2 * nb), 256, 0, s>>>(K, o, w, nb, nb, D);
     if (nt <= 0) break;
     launch_dgemm_sub(nt, nt, w, D + (size_t)(o + w) * K + o, K, D + (size_t)o * K + o + w, K,
