@@ -664,6 +664,7 @@ def main():
             "lu_levels": lu_info["fact_levels"],
             "solve_bands": [lu_info["fwd_bands"], lu_info["bwd_bands"]],
             "band_width": lu_info["band_width"],
+            "lu_pipeline": lu_info["lu_pipeline"],
             "solve_band_width": lu_info["solve_band_width"],
             "integration": "gear2 adaptive",
             "precision": args.precision,

@@ -371,10 +371,21 @@ int pattern_autotune_factor(vjx_pattern* pat, const double* data_dev) {
   if (pat->nnz)
     VJX_TRY_HIP(hipMemcpy(host.data(), data_dev, sizeof(double) * pat->nnz, hipMemcpyDeviceToHost));
   const int width0 = pat->opts.band_width;
-  int best_w = width0, best_deep = L.lu_deep;
+  const int deep0 = L.lu_deep;
+  int best_w = width0, best_deep = deep0;
   float best_ms = -1.f;
-  const int widths[] = {width0, 12, 16, 24};
-  for (int wi = 0; wi < 4 && rc >= 0; ++wi) {
+  int widths[] = {width0, 12, 16, 24};
+  int n_widths = 4;
+  // VJX_LU_BANDS=w pins the band width (A/B measurements and profiling
+  // passes that must see the plan of another run); the pipeline is still timed
+  if (const char* env = std::getenv("VJX_LU_BANDS")) {
+    const int w = std::atoi(env);
+    if (w > 0 && w <= kMaxBandWidth) {
+      widths[0] = w;
+      n_widths = 1;
+    }
+  }
+  for (int wi = 0; wi < n_widths && rc >= 0; ++wi) {
     const int w = widths[wi];
     if (wi > 0 && w == width0) continue;
     if (w != pat->opts.band_width) {
@@ -384,14 +395,16 @@ int pattern_autotune_factor(vjx_pattern* pat, const double* data_dev) {
       rc = pattern_upload(pat);
       if (rc < 0) break;
     }
-    for (int deep = 0; deep <= 2 && rc >= 0; ++deep) {
+    // the current plan (timed first) is kept unless another is at least 10 %
+    // faster: plans closer than that trade places from run to run with the
+    // timing noise, and another band width also changes the order of the
+    // Schur sums (the rounding of the factors)
+    for (int i = 0; i < 3 && rc >= 0; ++i) {
+      const int deep = (deep0 + i) % 3;
       L.lu_deep = deep;
       float ms = 0.f;
       rc = time_refactor(ms);
-      // another band width changes the order of the Schur sums (rounding of
-      // the factors), so it must beat the current one by 3 % to be kept
-      if (w != best_w) ms *= 1.03f;
-      if (rc >= 0 && (best_ms < 0.f || ms < best_ms)) {
+      if (rc >= 0 && (best_ms < 0.f || ms < 0.9f * best_ms)) {
         best_ms = ms;
         best_w = w;
         best_deep = deep;
