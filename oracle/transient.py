@@ -113,10 +113,19 @@ class LinearSolver:
         self.dense = dense
 
     def solve(self, csr_data, f):
+        """delta = -J^-1 f.  A non-finite or singular system gives a NaN
+        update instead of an exception: UMFPACK's singular-matrix warning is
+        tolerated and its solution is returned (umfpack_ffi.cpp:297, 320), so
+        the iteration simply fails the convergence tests."""
+        if not (np.all(np.isfinite(csr_data)) and np.all(np.isfinite(f))):
+            return np.full(f.shape, np.nan)
         if self.dense:
             J = mna.csr_matrix(self.cir, csr_data).toarray()
             J = J + 1e-14 * np.eye(J.shape[0])
-            return np.linalg.solve(J, -f)
+            try:
+                return np.linalg.solve(J, -f)
+            except np.linalg.LinAlgError:
+                return np.full(f.shape, np.nan)
         A = mna.csr_matrix(self.cir, csr_data).tocsc()
         # full partial pivoting: CMOS DC points leave cut-off nodes tied only
         # by gmin, where a 0.1 diagonal threshold can meet an exact zero pivot
@@ -125,7 +134,10 @@ class LinearSolver:
         except RuntimeError:
             if A.shape[0] > 6000:
                 raise
-            return np.linalg.solve(A.toarray(), -f)
+            try:
+                return np.linalg.solve(A.toarray(), -f)
+            except np.linalg.LinAlgError:
+                return np.full(f.shape, np.nan)
         return lu.solve(-f)
 
 

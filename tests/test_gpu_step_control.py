@@ -54,3 +54,42 @@ def test_step_control_matches_oracle(case):
     assert err <= 1e-12, err
     assert res.stats["total_nr_iterations"] == ref.stats["total_nr_iterations"]
     assert res.stats["rejected_steps"] == ref.stats["rejected_steps"]
+
+
+def _overflowing_dag():
+    """A 20-gate DAG whose second primary input rises to 1e200 V at 30 ps
+    (the other inputs stay low): on every Newton iterate after that edge the
+    MOSFET currents, which grow as the square of that voltage, overflow, so
+    the Jacobian holds infinite entries."""
+    import dataclasses
+
+    from vajax_amd import generators as G
+
+    net = G.gate_dag(20, seed=1)
+    vs = net.vsources
+    params = {k: np.array(v, dtype=np.float64) for k, v in vs.params.items()}
+    params["val1"][2:] = 0.0
+    params["val1"][1] = 1e200
+    params["delay"][1] = 30e-12
+    return dataclasses.replace(net, vsources=dataclasses.replace(vs, params=params))
+
+
+def test_overflowing_iterate_is_a_newton_failure_not_an_error():
+    """A non-finite Jacobian is not a pivoting problem: the pivot monitor
+    leaves it alone, the iteration fails the convergence tests (the
+    reference's UMFPACK returns the non-finite solution of such a system,
+    umfpack_ffi.cpp:297, 320), and the step control rejects the step and
+    cuts dt down to min_dt and its force-accepts, taking the oracle's
+    decisions step for step instead of the run ending in an error."""
+    net = _overflowing_dag()
+    _, ref = run_oracle(net, 0.2e-9, 1e-12, max_steps=80)
+    _, res = run_gpu(net, 0.2e-9, 1e-12, max_steps=80)
+    assert ref.stats["rejected_steps"] >= 30 and ref.stats["min_dt_used"] <= 1e-15
+    assert len(res.times) == len(ref.times)
+    dev = grid_deviation(res.times, ref.times)
+    assert dev <= 1e-12, dev
+    assert res.stats["rejected_steps"] == ref.stats["rejected_steps"]
+    assert res.stats["total_nr_iterations"] == ref.stats["total_nr_iterations"]
+    assert np.all(np.isfinite(res.stats["V_out"]))
+    err = float(np.max(np.abs(res.stats["V_out"] - ref.V)))
+    assert err <= 1e-9, err

@@ -417,3 +417,23 @@ def test_kat_dense_system():
     np.testing.assert_allclose(J, expect, rtol=1e-15, atol=0)
     # f at X = 0: the current source stamps +I at p (n2), branch rows -E
     np.testing.assert_allclose(f, [0.0, 0.0, 1e-6, -1.0, -0.5], rtol=1e-15, atol=0)
+
+
+def test_oracle_solver_returns_nan_for_singular_or_nonfinite_systems():
+    # UMFPACK's singular-matrix warning is tolerated and its non-finite
+    # solution returned (umfpack_ffi.cpp:297, 320): the oracle's solve gives
+    # a NaN update instead of raising, so Newton fails its tests
+    from types import SimpleNamespace
+
+    from oracle.transient import LinearSolver
+
+    cir = SimpleNamespace(n_aug=2, indptr=np.array([0, 2, 4]), indices=np.array([0, 1, 0, 1]))
+    f = np.array([1.0, 2.0])
+    for dense in (False, True):
+        solver = LinearSolver(cir, dense=dense)
+        x = solver.solve(np.array([2.0, 1.0, 1.0, 3.0]), f)
+        assert np.allclose(np.array([[2.0, 1.0], [1.0, 3.0]]) @ x, -f)
+        assert np.all(np.isnan(solver.solve(np.array([1.0, np.inf, 1.0, 1.0]), f)))
+    # an exactly singular system: NaN from the dense fallback of the sparse path
+    x = LinearSolver(cir).solve(np.array([1.0, 1.0, 1.0, 1.0]), f)
+    assert np.all(np.isnan(x)) or not np.all(np.isfinite(x))
