@@ -201,3 +201,32 @@ def test_array_multiplier_16x16_window(tmp_path):
         d = np.abs(rr.stats["V_out"] - ref.V)
         worst = float(np.max(d - 1e-6 * np.abs(ref.V)))
         assert worst <= bar, (prec, float(np.max(d)))
+
+
+def test_array_multiplier_64x64_whole_run_properties():
+    """The reference's mul64 benchmark circuit at full size (the 64x64
+    multiplier testbench, 266 408 MOSFETs; too large for the oracle): its
+    whole 2 ns transient completes with the dense trailing block on the
+    matrix cores, every node voltage stays finite and within half a volt of
+    the supply rails, and mixed precision on the FP64 run's grid stays within
+    1e-6 V."""
+    net = V.load("", text=G.array_multiplier_testbench(64))
+    assert net.groups["mosfet"].count == G.array_multiplier_mosfets(64) == 266408
+    _progress("mul64: FP64 adaptive run over 2 ns")
+    eng, res = run_gpu(net, 2e-9, 2e-12, use_graph=True, autotune=True)
+    info = eng.device.lu.info()
+    assert info["dense_k"] >= 96 and info["dense_m"] > 0, info
+    assert abs(res.times[-1] - 2e-9) <= 1e-15
+    V64 = res.stats["V_out"]
+    assert np.all(np.isfinite(V64))
+    # within half a volt of the rails (edges couple through the terminal
+    # charges: -0.25 V undershoot measured); a diverging run would leave them
+    _progress(f"mul64: node voltages in [{float(V64.min()):.3f}, {float(V64.max()):.3f}] V")
+    assert float(V64.min()) >= -0.5 and float(V64.max()) <= 1.2 + 0.5
+    assert res.stats["rejected_steps"] <= len(res.times) // 4
+    _progress(f"mul64: {len(res.times)} rows; mixed precision on the same grid")
+    _, r32 = run_gpu(net, 2e-9, 2e-12, use_graph=True, replay_times=res.times, precision="f32")
+    assert r32.stats["rejected_steps"] == 0
+    err = float(np.max(np.abs(r32.stats["V_out"] - V64)))
+    _progress(f"mul64: max |dV| mixed vs FP64 = {err:.3e} V")
+    assert err <= 1e-6, err
