@@ -81,8 +81,14 @@ def window_stats(res):
                             round(float(res.times[-1]) * 1e12, 3)]}
 
 
-# model parameter columns read by k_eval (the MOSFET's include its init cache)
-N_PAR = {"resistor": 1, "capacitor": 1, "mosfet": 11}
+# model parameter columns read by k_eval (the MOSFETs' include their init cache)
+N_PAR = {"resistor": 1, "capacitor": 1, "mosfet": 11, "ekv26": 52}
+# FP64 VALU operations of one EKV 2.6 instance eval (value + three partials),
+# from rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 over the eval
+# kernel of the 65k-gate EKV 2.6 DAG (64 lanes per instruction, FMA = 2),
+# divided by its instances per launch
+EKV26_F64_OPS_PER_INSTANCE = None
+FP64_VALU_PEAK_TFLOPS = 78.6
 
 
 def mixed_error(eng, ref_res, use_graph):
@@ -424,6 +430,57 @@ def run_mul64(args):
     return out
 
 
+def run_ekv26(args):
+    """The 65k-gate DAG with every MOSFET the EKV 2.6 model of
+    vajax/devices/models/ekv/ekv.va (charges, junctions, short-channel
+    effects; 52-column instance cache, dual-number Jacobian): the step loop
+    at a PSP-class eval cost, timed like the main leg, then an eager pass for
+    the eval kernel's time, FP64 rate and HBM fraction."""
+    from vajax_amd import _lib as L
+    from vajax_amd import generators as G
+    from vajax_amd.engine import CircuitEngine
+
+    t0 = time.time()
+    net = G.gate_dag(args.gates, seed=args.seed, model="ekv26")
+    eng = CircuitEngine(net)
+    eng.parse()
+    topo = eng.topology
+    log(f"ekv26 setup {time.time() - t0:.1f}s: n_unknowns={topo.n_augmented}")
+    run_gpu(eng, 3, "f64", True)
+    total = args.warmup + args.steps
+    res = run_gpu(eng, total, "f64", True, window_from=args.warmup + 1)
+    w = window_stats(res)
+    log(f"ekv26: {w['ms_per_step']} ms/step, {w['newton_iters_per_s']} Newton iters/s")
+    L.lib.vjx_profile_enable(1)
+    L.lib.vjx_profile_reset()
+    run_gpu(eng, min(total, 60), "f64", False)
+    pt = L.PhaseTimes()
+    L.lib.vjx_profile_read(C.byref(pt))
+    L.lib.vjx_profile_enable(0)
+    ph = pt.as_dict()
+    iters = max(ph["n_iterations"], 1)
+    n_dev = int(net.groups["ekv26"].count)
+    ev = ph["kernels"]["eval"]
+    avg_ms = ev["ms"] / max(ev["launches"], 1)
+    alg = kernel_bytes(topo, eng.device.lu.info(), "f64")["eval"]
+    gbs = alg / (avg_ms * 1e-3) / 1e9
+    out = {"workload": f"gate_dag_{args.gates}_seed{args.seed} with EKV 2.6 MOSFETs",
+           "mosfets": n_dev, "unknowns": topo.n_augmented, **w,
+           "phase_ms_per_newton_iter": {"build": round(ph["build_ms"] / iters, 4),
+                                        "lu": round(ph["lu_ms"] / iters, 4),
+                                        "solve": round(ph["solve_ms"] / iters, 4)},
+           "eval": {"kernel": "k_eval<EKV26>", "avg_launch_us": round(avg_ms * 1e3, 2),
+                    "ms_per_newton_iter": round(ev["ms"] / iters, 4),
+                    "alg_bytes_per_launch": int(alg),
+                    "achieved_GBs": round(gbs, 2), "hbm_frac": round(gbs / HBM_PEAK_GBS, 5)}}
+    if EKV26_F64_OPS_PER_INSTANCE:
+        tf = EKV26_F64_OPS_PER_INSTANCE * n_dev / (avg_ms * 1e-3) / 1e12
+        out["eval"].update({"f64_ops_per_instance": EKV26_F64_OPS_PER_INSTANCE,
+                            "achieved_f64_TFLOPs": round(tf, 3),
+                            "f64_valu_frac": round(tf / FP64_VALU_PEAK_TFLOPS, 5)})
+    return out
+
+
 def summarize_trace(path, window=0):
     """Markdown table of the kernels in a rocprofv3 kernel-trace CSV inside
     one timed window: between the start and end k_window_mark dispatches of
@@ -487,6 +544,10 @@ def main():
                     help="skip the mul64 leg (the reference's 64x64 multiplier benchmark)")
     ap.add_argument("--mul64-only", action="store_true",
                     help="run only the mul64 leg and print its JSON")
+    ap.add_argument("--no-ekv26", action="store_true",
+                    help="skip the EKV 2.6 leg (the DAG with the full EKV 2.6 MOSFET model)")
+    ap.add_argument("--ekv26-only", action="store_true",
+                    help="run only the EKV 2.6 leg and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers (reference kernel for HBM counters)")
     ap.add_argument("--summarize-trace", default=None,
@@ -523,6 +584,9 @@ def main():
 
     if args.mul64_only:
         print(json.dumps(run_mul64(args)), flush=True)
+        return
+    if args.ekv26_only:
+        print(json.dumps(run_ekv26(args)), flush=True)
         return
     from vajax_amd import _lib as L
     from vajax_amd.engine import CircuitEngine
@@ -640,6 +704,10 @@ def main():
     if not args.no_mul64:
         mul64 = run_mul64(args)
 
+    ekv26 = None
+    if not args.no_ekv26:
+        ekv26 = run_ekv26(args)
+
     line = {
         "metric": METRIC,
         "value": win["newton_iters_per_s"],
@@ -682,6 +750,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "mul64": mul64,
+        "ekv26": ekv26,
         "mixed_precision": mixed,
     }
     if cpu:

@@ -217,7 +217,8 @@ int vjx_lu_download(vjx_pattern* pat, double* lu_host);
 #define VJX_MODEL_CAPACITOR 1
 #define VJX_MODEL_MOSFET 2
 #define VJX_MODEL_DIODE 3 /* junction diode with $limit pnjlim: one limit state */
-#define VJX_MODEL_COUNT 4
+#define VJX_MODEL_EKV26 4 /* EKV 2.6 MOSFET with junctions (vajax/devices/models/ekv/ekv.va) */
+#define VJX_MODEL_COUNT 5
 #define VJX_MAX_MODELS 8
 
 /* Independent-source waveforms (vajax/analysis/sources.py:25-294).  Each

@@ -39,13 +39,15 @@ import numpy as np
 
 from vajax_amd.netlist import Netlist
 
+from . import ekv26 as EKV
 from . import models as M
 
-TERMS = {"resistor": 2, "capacitor": 2, "mosfet": 4, "diode": 2}
+TERMS = {"resistor": 2, "capacitor": 2, "mosfet": 4, "diode": 2, "ekv26": 4}
 VPAIRS = {"resistor": [(0, 1)], "capacitor": [(0, 1)], "mosfet": [(1, 3), (2, 3), (0, 3)],
+          "ekv26": [(1, 3), (2, 3), (0, 3)],
           "diode": [(0, 1)]}
 # limit states per instance (one per $limit call)
-N_LIM = {"resistor": 0, "capacitor": 0, "mosfet": 0, "diode": 1}
+N_LIM = {"resistor": 0, "capacitor": 0, "mosfet": 0, "diode": 1, "ekv26": 0}
 
 
 @dataclass
@@ -147,7 +149,8 @@ def setup(net: Netlist, gmin: float = 1e-12) -> OracleCircuit:
             csr[:, k] = [pos_of[(a, b)] if a >= 0 else -1 for a, b in zip(rr.tolist(), cc.tolist())]
         res_idx = np.where(nodes > 0, nodes - 1, -1)
         cache = (M.mosfet_cache(grp.params) if mname == "mosfet" else
-                 M.diode_cache(grp.params) if mname == "diode" else {})
+                 M.diode_cache(grp.params) if mname == "diode" else
+                 EKV.cache(grp.params) if mname == "ekv26" else {})
         models.append(OracleModel(mname, nodes, res_idx, csr, grp.params, cache))
     n_lim = 0
     for om in models:
@@ -201,6 +204,11 @@ def _eval_plain(om: OracleModel, V: np.ndarray):
         vsb = V[n[:, 2]] - V[n[:, 3]]
         vdb = V[n[:, 0]] - V[n[:, 3]]
         return M.eval_mosfet(vgb, vsb, vdb, om.cache)
+    if om.name == "ekv26":
+        vgb = V[n[:, 1]] - V[n[:, 3]]
+        vsb = V[n[:, 2]] - V[n[:, 3]]
+        vdb = V[n[:, 0]] - V[n[:, 3]]
+        return EKV.eval_ekv26(vgb, vsb, vdb, om.cache)
     raise ValueError(om.name)
 
 

@@ -24,7 +24,7 @@ VJX_ERR_FILL = -5
 VJX_PREC_F64 = 0
 VJX_PREC_F32 = 1
 VJX_METHOD = {"be": 0, "trap": 1, "gear2": 2}
-MODEL_IDS = {"resistor": 0, "capacitor": 1, "mosfet": 2, "diode": 3}
+MODEL_IDS = {"resistor": 0, "capacitor": 1, "mosfet": 2, "diode": 3, "ekv26": 4}
 MAX_MODELS = 8
 
 i32p = C.POINTER(C.c_int32)

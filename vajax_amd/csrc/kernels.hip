@@ -57,6 +57,10 @@ template <>
 struct ModelTraits<VJX_MODEL_DIODE> {
   static constexpr int NV = 1, NR = 2, NJ = 4, NP = 4, NL = 1;
 };
+template <>
+struct ModelTraits<VJX_MODEL_EKV26> {
+  static constexpr int NV = 3, NR = 4, NJ = 16, NP = 52, NL = 0;
+};
 
 // resistor.va:23-27 -- I(A,B) <+ V(A,B)/r
 __device__ __forceinline__ void eval_resistor(const double* vb, const double* par, int i, int N,
@@ -263,6 +267,264 @@ __device__ __forceinline__ void eval_diode(const double* vb, const double* par, 
 
 // One instance of model MID; li / lr / lc / lo (limit state in, lim_rhs
 // resistive / reactive, limit state out) are used by limited models only.
+// ---- EKV 2.6 (vajax/devices/models/ekv/ekv.va) --------------------------------
+//
+// The EKV 2.6 charge-based MOSFET of ekv.va: short- and narrow-channel effects
+// on the pinch-off voltage, velocity saturation and channel-length modulation,
+// mobility reduction by the vertical field (E0 form, or THETA when E0 = 0),
+// impact-ionisation substrate current, the node charges QD, QS, QG, and the
+// drain/source junction diodes with breakdown, trap-assisted tunnelling and
+// depletion charge.  Derivatives by forward-mode dual numbers over the three
+// branch voltages V(g,b), V(s,b), V(d,b), in place of the file's hand-written
+// derivatives.  Left out: noise, and the series-resistance correction of the
+// drain current (ekv.va:697-700), which is the identity at the default RSH = 0
+// (the host rejects RSH > 0).  Temperature and geometry terms are the instance
+// cache below; the thermal voltage is k T / q at the device temperature.
+struct D3 {
+  double v, d0, d1, d2;
+};
+__device__ __forceinline__ D3 dc(double v) { return D3{v, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) {
+  return D3{a.v + b.v, a.d0 + b.d0, a.d1 + b.d1, a.d2 + b.d2};
+}
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) {
+  return D3{a.v - b.v, a.d0 - b.d0, a.d1 - b.d1, a.d2 - b.d2};
+}
+__device__ __forceinline__ D3 operator-(D3 a) { return D3{-a.v, -a.d0, -a.d1, -a.d2}; }
+__device__ __forceinline__ D3 operator*(D3 a, D3 b) {
+  return D3{a.v * b.v, a.d0 * b.v + a.v * b.d0, a.d1 * b.v + a.v * b.d1, a.d2 * b.v + a.v * b.d2};
+}
+__device__ __forceinline__ D3 operator*(double k, D3 a) {
+  return D3{k * a.v, k * a.d0, k * a.d1, k * a.d2};
+}
+__device__ __forceinline__ D3 operator*(D3 a, double k) { return k * a; }
+__device__ __forceinline__ D3 operator+(D3 a, double k) { return D3{a.v + k, a.d0, a.d1, a.d2}; }
+__device__ __forceinline__ D3 operator+(double k, D3 a) { return a + k; }
+__device__ __forceinline__ D3 operator-(D3 a, double k) { return D3{a.v - k, a.d0, a.d1, a.d2}; }
+__device__ __forceinline__ D3 operator-(double k, D3 a) { return D3{k - a.v, -a.d0, -a.d1, -a.d2}; }
+__device__ __forceinline__ D3 operator/(D3 a, D3 b) {
+  const double q = a.v / b.v;
+  return D3{q, (a.d0 - q * b.d0) / b.v, (a.d1 - q * b.d1) / b.v, (a.d2 - q * b.d2) / b.v};
+}
+__device__ __forceinline__ D3 operator/(D3 a, double k) { return D3{a.v / k, a.d0 / k, a.d1 / k, a.d2 / k}; }
+__device__ __forceinline__ D3 operator/(double k, D3 b) { return dc(k) / b; }
+__device__ __forceinline__ D3 dsqrt(D3 a) {
+  const double r = sqrt(a.v), g = 0.5 / r;
+  return D3{r, g * a.d0, g * a.d1, g * a.d2};
+}
+__device__ __forceinline__ D3 dlog(D3 a) {
+  const double g = 1.0 / a.v;
+  return D3{log(a.v), g * a.d0, g * a.d1, g * a.d2};
+}
+__device__ __forceinline__ D3 dexp(D3 a) {
+  const double e = exp(a.v);
+  return D3{e, e * a.d0, e * a.d1, e * a.d2};
+}
+
+// Normalized inversion charge q from x = (VP - V)/Vt, the file's approximate
+// inverse of x = 2 q + ln q (ekv.va:428-444): returns yk = q.
+__device__ __forceinline__ D3 ekv_q(D3 x) {
+  if (x.v > -0.35) {
+    const D3 z0 = 2.0 / (1.3 + x - dlog(x + 1.6));
+    const D3 zk = (2.0 + z0) / (1.0 + x + dlog(z0));
+    return (1.0 + x + dlog(zk)) / (2.0 + zk);
+  }
+  if (x.v > -15.0) {
+    const D3 z0 = 1.55 + dexp(-x);
+    const D3 zk = (2.0 + z0) / (1.0 + x + dlog(z0));
+    return (1.0 + x + dlog(zk)) / (2.0 + zk);
+  }
+  if (x.v > -23.0) return 1.0 / (2.0 + dexp(-x));
+  return dexp(x) + 1e-64;
+}
+
+// Instance cache columns, in order (the voltage-independent terms of
+// ekv.va:298-388 and :767-788, computed once on the host by models.py).
+enum Ekv26Col {
+  EK_TYPE, EK_VTO_S, EK_PHI_T, EK_GAMMA_S, EK_KP_WEFF, EK_LEFF, EK_UCRIT_T, EK_LAMBDA, EK_LC,
+  EK_WETA_W, EK_LETA_L, EK_DVFB, EK_E0, EK_THETA, EK_T0, EK_ETA_QI, EK_IBN, EK_IBA_IBB,
+  EK_LC_IBB, EK_WLCOX, EK_VC, EK_LOG_VC_VT, EK_VT,
+  // junction diodes (ekv.va:763-848)
+  EK_M, EK_JK, EK_IS_D, EK_IS_S, EK_BV, EK_XJBV, EK_GMIN, EK_TUN_W, EK_TUN_PD, EK_TUN_AD,
+  EK_TUN_PS, EK_TUN_AS, EK_KG, EK_KSW, EK_KA, EK_VTSSWG, EK_VTSSW, EK_VTS, EK_CJA_D, EK_CJP_D,
+  EK_CJG, EK_CJA_S, EK_CJP_S, EK_MJ, EK_MJSW, EK_MJSWG, EK_PB, EK_PBSW, EK_PBSWG, EK_NCOL
+};
+
+// Trap-assisted tunnelling term  exp(v k vt / max(vt + v, 1e-3)) - 1  (ekv.va:800-802).
+__device__ __forceinline__ D3 ekv_tunnel(D3 v, double k, double vt) {
+  const D3 den = vt + v.v > 1.0e-3 ? vt + v : dc(1.0e-3);
+  return dexp(v * k * vt / den) - 1.0;
+}
+
+// One junction diode between the drain (or source) and the bulk, v = TYPE V(x,b):
+// current (ekv.va:792-803) and charge (ekv.va:818-831), both without TYPE M.
+__device__ __forceinline__ void ekv_junction(D3 v, double is, double tun_p, double tun_a,
+                                             double cja, double cjp, const double* c, D3& cur,
+                                             D3& q) {
+  D3 arg = -v * c[EK_JK];
+  if (arg.v < -40.0) arg = dc(-40.0);
+  const D3 t0 = (c[EK_BV] - v) * c[EK_JK];
+  const D3 fbd = t0.v > 70.0 ? dc(1.0) : 1.0 + c[EK_XJBV] * dexp(-t0);
+  D3 tun = -c[EK_TUN_W] * ekv_tunnel(v, c[EK_KG], c[EK_VTSSWG]);
+  tun = tun - tun_p * ekv_tunnel(v, c[EK_KSW], c[EK_VTSSW]);
+  tun = tun - tun_a * ekv_tunnel(v, c[EK_KA], c[EK_VTS]);
+  cur = is * (1.0 - dexp(arg)) * fbd + v * c[EK_GMIN] + tun;
+  D3 cap;
+  if (v.v > 0.0) {
+    cap = cja * dexp(-c[EK_MJ] * dlog(1.0 + v / c[EK_PB])) +
+          cjp * dexp(-c[EK_MJSW] * dlog(1.0 + v / c[EK_PBSW])) +
+          c[EK_CJG] * dexp(-c[EK_MJSWG] * dlog(1.0 + v / c[EK_PBSWG]));
+  } else {
+    cap = cja * (1.0 - c[EK_MJ] * v / c[EK_PB]) + cjp * (1.0 - c[EK_MJSW] * v / c[EK_PBSW]) +
+          c[EK_CJG] * (1.0 - c[EK_MJSWG] * v / c[EK_PBSWG]);
+  }
+  q = cap * v;
+}
+
+// Branch voltages vb[0] = V(g,b), vb[1] = V(s,b), vb[2] = V(d,b); residual
+// order d, g, s, b; Jacobian keys row-major over (d, g, s, b).
+__device__ __forceinline__ void eval_ekv26(const double* vb, const double* par, int i, int N,
+                                           double* rr, double* rc, double* jr, double* jc) {
+  double c[EK_NCOL];
+#pragma unroll
+  for (int k = 0; k < EK_NCOL; ++k) c[k] = par[(size_t)k * N + i];
+  const double TYPE = c[EK_TYPE], VTO_S = c[EK_VTO_S], PHI_T = c[EK_PHI_T];
+  const double GAMMA_S = c[EK_GAMMA_S], KP_Weff = c[EK_KP_WEFF], Leff = c[EK_LEFF];
+  const double UCRIT_T = c[EK_UCRIT_T], LAMBDA = c[EK_LAMBDA], Lc = c[EK_LC];
+  const double WETA_W = c[EK_WETA_W], LETA_L = c[EK_LETA_L], deltaVFB = c[EK_DVFB];
+  const double E0 = c[EK_E0], THETA = c[EK_THETA], T0 = c[EK_T0], eta_qi = c[EK_ETA_QI];
+  const double IBN = c[EK_IBN], IBA_IBB = c[EK_IBA_IBB], Lc_IBB = c[EK_LC_IBB];
+  const double WLCox = c[EK_WLCOX], Vc = c[EK_VC], log_Vc_Vt = c[EK_LOG_VC_VT], Vt = c[EK_VT];
+  const double inv_Vt = 1.0 / Vt, Vt_01 = 0.1 * Vt, Vt_4 = 4.0 * Vt;
+  const double Vt_Vt_2 = 2.0 * Vt * Vt, Vt_Vt_16 = 16.0 * Vt * Vt;
+  const double GAMMA_sqrt_PHI = GAMMA_S * sqrt(PHI_T);
+  const double Lc_LAMBDA = Lc * LAMBDA, Lc_UCRIT = Lc * UCRIT_T, inv_UCRIT = 1.0 / UCRIT_T;
+  const D3 vg = D3{TYPE * vb[0], TYPE, 0.0, 0.0};
+  const D3 vs = D3{TYPE * vb[1], 0.0, TYPE, 0.0};
+  const D3 vd = D3{TYPE * vb[2], 0.0, 0.0, TYPE};
+  // source/drain interchange (ekv.va:390-402)
+  const bool fwd = !(vd.v - vs.v < 0);
+  const D3 VG = vg, VS = fwd ? vs : vd, VD = fwd ? vd : vs;
+  const double Mode = fwd ? 1.0 : -1.0;
+  // effective gate voltage and pinch-off voltage (ekv.va:403-425)
+  const D3 VGstar = VG - VTO_S - deltaVFB + PHI_T + GAMMA_sqrt_PHI;
+  const D3 VGprime = 0.5 * (VGstar + dsqrt(VGstar * VGstar + 2.0 * Vt_Vt_16));
+  const D3 PHI_VS = PHI_T + VS;
+  const D3 sqrt_PHI_VS = dsqrt(0.5 * (PHI_VS + dsqrt(PHI_VS * PHI_VS + Vt_Vt_16)));
+  const D3 PHI_VD = PHI_T + VD;
+  const D3 sqrt_PHI_VD = dsqrt(0.5 * (PHI_VD + dsqrt(PHI_VD * PHI_VD + Vt_Vt_16)));
+  const D3 big_sqrt_VP0 = dsqrt(VGprime + 0.25 * GAMMA_S * GAMMA_S);
+  const D3 VP0 = VGprime - PHI_T - GAMMA_S * (big_sqrt_VP0 - 0.5 * GAMMA_S);
+  const D3 sqrt_PHI_VP0 = dsqrt(VP0 + PHI_T + Vt_01);
+  const D3 GAMMAstar = GAMMA_S - LETA_L * (sqrt_PHI_VS + sqrt_PHI_VD) + WETA_W * sqrt_PHI_VP0;
+  const D3 GAMMAprime = 0.5 * (GAMMAstar + dsqrt(GAMMAstar * GAMMAstar + Vt_01));
+  const D3 big_sqrt_VP = dsqrt(VGprime + 0.25 * GAMMAprime * GAMMAprime);
+  const D3 VP = VGprime - PHI_T - GAMMAprime * (big_sqrt_VP - 0.5 * GAMMAprime);
+  // forward normalized current (ekv.va:427-447)
+  const D3 qf = ekv_q((VP - VS) * inv_Vt);
+  const D3 if_ = qf * (1.0 + qf);
+  const D3 sqrt_if = dsqrt(if_);
+  // saturation voltage (ekv.va:449-459)
+  const double Vt_Vc = Vt / Vc;
+  const D3 VDSS = Vc * (dsqrt(0.25 + sqrt_if * Vt_Vc) - 0.5);
+  const D3 Vds = 0.5 * (VD - VS);
+  const D3 deltaV_2 = Vt_Vt_16 * (LAMBDA * (sqrt_if - VDSS * inv_Vt) + 15.625e-3);
+  const D3 Vip = dsqrt(VDSS * VDSS + deltaV_2) - dsqrt((Vds - VDSS) * (Vds - VDSS) + deltaV_2);
+  const D3 VDSSprime =
+      Vc * (dsqrt(0.25 + (sqrt_if - 0.75 * dlog(if_)) * Vt_Vc) - 0.5) + log_Vc_Vt;
+  // reverse normalized current with the saturation voltage (ekv.va:461-486)
+  const D3 Vdsprime = Vds - VDSSprime;
+  const D3 qrp = ekv_q((VP - Vds - VS - dsqrt(VDSSprime * VDSSprime + deltaV_2) +
+                        dsqrt(Vdsprime * Vdsprime + deltaV_2)) *
+                       inv_Vt);
+  const D3 irprime = qrp * (1.0 + qrp);
+  // channel length modulation and velocity saturation (ekv.va:489-493)
+  const D3 deltaL = Lc_LAMBDA * dlog(1.0 + (Vds - Vip) / Lc_UCRIT);
+  const D3 Lprime = Leff - deltaL + (Vds + Vip) * inv_UCRIT;
+  const double Lmin = 0.1 * Leff;
+  const D3 Leq = 0.5 * (Lprime + dsqrt(Lprime * Lprime + Lmin * Lmin));
+  // reverse normalized current (ekv.va:498-518)
+  const D3 qr = ekv_q((VP - VD) * inv_Vt);
+  const D3 ir = qr * (1.0 + qr);
+  // normalized charges and the transconductance factor (ekv.va:519-557)
+  const D3 sif2 = 0.25 + if_, sir2 = 0.25 + ir;
+  const D3 sif = dsqrt(sif2), sir = dsqrt(sir2);
+  const D3 sif_sir_2 = (sif + sir) * (sif + sir);
+  const D3 sqrt_PHI_VP_2 = 2.0 * dsqrt(VP + PHI_T + 1.0e-6);
+  const D3 n_1 = GAMMA_S / sqrt_PHI_VP_2;
+  const D3 n_1_n = GAMMA_S / (sqrt_PHI_VP_2 + GAMMA_S);
+  const D3 qi = -((1.0 + n_1) * Vt *
+                  ((0.66666666 + 0.66666666) * (sir2 + sir * sif + sif2) / (sif + sir) - 1.0));
+  const D3 qb = -0.5 * GAMMA_S * sqrt_PHI_VP_2 - n_1_n * qi;
+  D3 beta;
+  if (E0 == 0.0) {
+    const D3 VPprime = 0.5 * (VP + dsqrt(VP * VP + Vt_Vt_2));
+    beta = KP_Weff / (Leq * (1.0 + THETA * VPprime));
+  } else {
+    const D3 e = qb + eta_qi * qi;
+    const D3 E0_Q_1 = e.v > 0.0 ? 1.0 + T0 * e : 1.0 - T0 * e;
+    beta = KP_Weff * (1.0 + T0 * GAMMA_sqrt_PHI) / (Leq * E0_Q_1);
+  }
+  // slope factor and drain current (ekv.va:560-565)
+  const D3 n = 1.0 + GAMMA_S / (2.0 * dsqrt(PHI_T + VP + Vt_4));
+  const D3 Id = Vt_Vt_2 * n * beta * (if_ - irprime);
+  // impact ionisation (ekv.va:709-725)
+  D3 Isub = dc(0.0);
+  const D3 Vib = VD - VS - (IBN + IBN) * VDSS;
+  if (Vib.v > 0.0 && IBA_IBB > 0.0) {
+    D3 x = -Lc_IBB / Vib;
+    if (x.v < -35.0) x = dc(-35.0);
+    Isub = IBA_IBB * Vib * dexp(x) * Id;
+  }
+  // node charges (ekv.va:727-740)
+  const D3 sif3 = sif * sif2, sir3 = sir * sir2;
+  const D3 sqrt_PHI_VP2_2 = 2.0 * dsqrt(PHI_T + 0.5 * VP);
+  const D3 n_Vt_COX = (1.0 + GAMMAprime / sqrt_PHI_VP2_2) * (Vt * WLCox);
+  const D3 QD = -(n_Vt_COX *
+                  (0.266666666 * (3.0 * sir3 + 6.0 * sir2 * sif + 4.0 * sir * sif2 + 2.0 * sif3) /
+                       sif_sir_2 -
+                   0.5));
+  const D3 QS = -(n_Vt_COX *
+                  (0.266666666 * (3.0 * sif3 + 6.0 * sif2 * sir + 4.0 * sif * sir2 + 2.0 * sir3) /
+                       sif_sir_2 -
+                   0.5));
+  const D3 QI = QS + QD;
+  const D3 QB = WLCox * (-0.5 * GAMMAprime * sqrt_PHI_VP_2 + VGprime - VGstar) -
+                QI * GAMMAprime / (GAMMAprime + sqrt_PHI_VP2_2);
+  const D3 QG = -QI - QB;
+  // junction diodes on the unswapped branches (ekv.va:790-847)
+  D3 ijd, qjd, ijs, qjs;
+  ekv_junction(vd, c[EK_IS_D], c[EK_TUN_PD], c[EK_TUN_AD], c[EK_CJA_D], c[EK_CJP_D], c, ijd, qjd);
+  ekv_junction(vs, c[EK_IS_S], c[EK_TUN_PS], c[EK_TUN_AS], c[EK_CJA_S], c[EK_CJP_S], c, ijs, qjs);
+  const double TM = TYPE * c[EK_M];
+  // terminal currents and charges (ekv.va:741-754, 803, 815, 832, 847)
+  const D3 ids = (TYPE * Mode) * Id;
+  const D3 isub = TYPE * Isub;
+  const D3 rd = (fwd ? ids + isub : ids) + TM * ijd;
+  const D3 rs = (fwd ? -ids : isub - ids) + TM * ijs;
+  const D3 rb = -(isub + TM * (ijd + ijs));
+  const D3 qd = TYPE * (fwd ? QD : QS) + TM * qjd;
+  const D3 qs = TYPE * (fwd ? QS : QD) + TM * qjs;
+  const D3 qg = TYPE * QG;
+  const D3 qbn = -(qd + qs + qg);
+  // Jacobian columns (d, g, s, b) from the branch derivatives (g, s, d)
+  auto put = [&](double* r, double* j, int row, const D3& x) {
+    r[row] = x.v;
+    j[4 * row + 0] = x.d2;
+    j[4 * row + 1] = x.d0;
+    j[4 * row + 2] = x.d1;
+    j[4 * row + 3] = -(x.d0 + x.d1 + x.d2);
+  };
+  put(rr, jr, 0, rd);
+  put(rr, jr, 1, dc(0.0));
+  put(rr, jr, 2, rs);
+  put(rr, jr, 3, rb);
+  put(rc, jc, 0, qd);
+  put(rc, jc, 1, qg);
+  put(rc, jc, 2, qs);
+  put(rc, jc, 3, qbn);
+}
+
 template <int MID>
 __device__ __forceinline__ void eval_model(const double* vb, const double* par, int i, int N,
                                            const double* li, double* rr, double* rc, double* jr,
@@ -270,6 +532,7 @@ __device__ __forceinline__ void eval_model(const double* vb, const double* par, 
   if constexpr (MID == VJX_MODEL_RESISTOR) eval_resistor(vb, par, i, N, rr, rc, jr, jc);
   if constexpr (MID == VJX_MODEL_CAPACITOR) eval_capacitor(vb, par, i, N, rr, rc, jr, jc);
   if constexpr (MID == VJX_MODEL_MOSFET) eval_mosfet(vb, par, i, N, rr, rc, jr, jc);
+  if constexpr (MID == VJX_MODEL_EKV26) eval_ekv26(vb, par, i, N, rr, rc, jr, jc);
   if constexpr (MID == VJX_MODEL_DIODE) eval_diode(vb, par, i, N, li, rr, rc, jr, jc, lr, lc, lo);
 }
 
@@ -372,9 +635,10 @@ __global__ void k_split_finish(int N, int n_res, int n_jac, int limited, const d
 
 // Raw eval (openvaf_jax contract), row-major outputs per instance.
 template <int MID>
-__global__ void k_model_eval(int N, const double* __restrict__ vbr, const double* __restrict__ par,
-                             const double* __restrict__ lio, double* rro, double* rco, double* jro,
-                             double* jco, double* lro, double* lco, double* loo) {
+__global__ void __launch_bounds__(256)
+    k_model_eval(int N, const double* __restrict__ vbr, const double* __restrict__ par,
+                 const double* __restrict__ lio, double* rro, double* rco, double* jro,
+                 double* jco, double* lro, double* lco, double* loo) {
   using Tr = ModelTraits<MID>;
   constexpr int NL = Tr::NL > 0 ? Tr::NL : 1;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1590,13 +1854,15 @@ inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
 
-int model_n_v(int id) {
-  return id == VJX_MODEL_MOSFET ? 3 : 1;
-}
-int model_n_res(int id) { return id == VJX_MODEL_MOSFET ? 4 : 2; }
-int model_n_jac(int id) { return id == VJX_MODEL_MOSFET ? 16 : 4; }
+static bool four_terminal(int id) { return id == VJX_MODEL_MOSFET || id == VJX_MODEL_EKV26; }
+int model_n_v(int id) { return four_terminal(id) ? 3 : 1; }
+int model_n_res(int id) { return four_terminal(id) ? 4 : 2; }
+int model_n_jac(int id) { return four_terminal(id) ? 16 : 4; }
 int model_n_par(int id) {
-  return id == VJX_MODEL_MOSFET ? 11 : id == VJX_MODEL_DIODE ? 4 : 1;
+  return id == VJX_MODEL_MOSFET  ? ModelTraits<VJX_MODEL_MOSFET>::NP
+         : id == VJX_MODEL_EKV26 ? ModelTraits<VJX_MODEL_EKV26>::NP
+         : id == VJX_MODEL_DIODE ? ModelTraits<VJX_MODEL_DIODE>::NP
+                                 : 1;
 }
 int model_n_lim(int id) { return id == VJX_MODEL_DIODE ? 1 : 0; }
 
@@ -1624,6 +1890,11 @@ void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalO
                                                m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
                                                out.rq, nullptr, nullptr, nullptr, nullptr);
       break;
+    case VJX_MODEL_EKV26:
+      k_eval<VJX_MODEL_EKV26><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
+                                              m.res_dst.p, m.par.p, X, sc, out.jac, out.rf,
+                                              out.rq, nullptr, nullptr, nullptr, nullptr);
+      break;
     case VJX_MODEL_DIODE:
       k_eval<VJX_MODEL_DIODE><<<G, B, 0, s>>>(m.count, m.vn1.p, m.vn2.p, m.jac_dst.p,
                                               m.res_dst.p, m.par.p, X, sc, out.jac, out.rf, out.rq,
@@ -1650,6 +1921,10 @@ void launch_model_eval_raw(int id, int count, const double* vb, const double* pa
     case VJX_MODEL_MOSFET:
       k_model_eval<VJX_MODEL_MOSFET><<<G, B, 0, s>>>(count, vb, par, li, rr, rc, jr, jc, lr, lc,
                                                      lo);
+      break;
+    case VJX_MODEL_EKV26:
+      k_model_eval<VJX_MODEL_EKV26><<<G, B, 0, s>>>(count, vb, par, li, rr, rc, jr, jc, lr, lc,
+                                                    lo);
       break;
     case VJX_MODEL_DIODE:
       k_model_eval<VJX_MODEL_DIODE><<<G, B, 0, s>>>(count, vb, par, li, rr, rc, jr, jc, lr, lc,

@@ -27,6 +27,7 @@ from typing import Dict
 import numpy as np
 
 from .netlist import (
+    MODEL_PARAMS,
     SOURCE_DEFAULTS,
     WAVE_DC,
     WAVE_PULSE,
@@ -48,6 +49,17 @@ NMOS_DEFAULTS: Dict[str, float] = {
     "cov": 2e-10,
     "cj": 5e-10,
 }
+# EKV 2.6 (``ekv26``) technology values for the same circuits: a short-channel
+# process at a 1 V supply (oxide 2.3 nm, junction depth 30 nm), with mobility
+# reduction, velocity saturation, channel-length modulation, short/narrow
+# channel effects, impact ionisation and the drain/source junctions.
+EKV26_NMOS: Dict[str, float] = {
+    "type": 1.0, "vto": 0.35, "gamma": 0.4, "phi": 0.8, "kp": 300e-6, "cox": 0.015,
+    "xj": 30e-9, "e0": 1.0e8, "ucrit": 5.0e6, "lambda": 0.5, "leta": 0.1, "weta": 0.2,
+    "q0": 0.0, "hdif": 0.1e-6, "xd_cj": 1.0e-3, "xd_cjsw": 1.0e-10, "xd_cjswg": 1.0e-10,
+}
+EKV26_PMOS: Dict[str, float] = dict(EKV26_NMOS, type=-1.0, vto=-0.35, kp=120e-6)
+
 PMOS_DEFAULTS: Dict[str, float] = {
     "type": -1.0,
     "vto": 0.35,
@@ -210,7 +222,7 @@ def _dag_inputs(rng: np.random.Generator, n_gates: int, n_pi: int, fanout_max: i
 
 def gate_dag(n_gates: int = 20000, seed: int = 0, n_pi: int = 64, fanout_max: int = 4,
              window: int = 96, mean_dist: float = 12.0, t_stop: float = 1e-9,
-             dt: float = 1e-12, vdd_rise: float = 100e-12) -> Netlist:
+             dt: float = 1e-12, vdd_rise: float = 100e-12, model: str = "mosfet") -> Netlist:
     """Random DAG of 2-input NAND/NOR gates (4 MOSFETs each).
 
     Gate g picks NAND or NOR with probability 1/2 and two distinct inputs
@@ -224,8 +236,12 @@ def gate_dag(n_gates: int = 20000, seed: int = 0, n_pi: int = 64, fanout_max: in
     supply ramped 0 -> 1 V over ``vdd_rise`` (a power-up): the DC point of a CMOS netlist
     whose cut-off stack nodes are held only by gmin is numerically singular
     for plain Newton from the mid-rail guess, while every transient Newton
-    matrix is regularised by c0*C.
+    matrix is regularised by c0*C.  ``model`` is "mosfet" (the long-channel
+    substitute) or "ekv26" (EKV 2.6 at :data:`EKV26_NMOS` / :data:`EKV26_PMOS`);
+    both draw the same random numbers, so they give the same netlist topology.
     """
+    if model not in ("mosfet", "ekv26"):
+        raise ValueError(f"gate_dag: model must be 'mosfet' or 'ekv26', not {model!r}")
     rng = np.random.default_rng(seed)
     is_nand = rng.random(n_gates) < 0.5
     inputs = _dag_inputs(rng, n_gates, n_pi, fanout_max, window, mean_dist)
@@ -274,9 +290,15 @@ def gate_dag(n_gates: int = 20000, seed: int = 0, n_pi: int = 64, fanout_max: in
     typ = typ.reshape(n_mos)
 
     params = {"type": typ, "w": w, "l": l}
-    for k in ("vto", "gamma", "phi", "kp", "theta", "cox", "cov", "cj"):
-        params[k] = np.where(typ > 0, NMOS_DEFAULTS[k], PMOS_DEFAULTS[k]).astype(np.float64)
-    mos = DeviceGroup(model="mosfet", terminals=terminals, params=params, names=None)
+    if model == "mosfet":
+        for k in ("vto", "gamma", "phi", "kp", "theta", "cox", "cov", "cj"):
+            params[k] = np.where(typ > 0, NMOS_DEFAULTS[k], PMOS_DEFAULTS[k]).astype(np.float64)
+    else:
+        for k, v in MODEL_PARAMS["ekv26"].items():
+            if k not in params:
+                params[k] = np.where(typ > 0, EKV26_NMOS.get(k, v),
+                                     EKV26_PMOS.get(k, v)).astype(np.float64)
+    mos = DeviceGroup(model=model, terminals=terminals, params=params, names=None)
 
     m_src = 1 + n_pi
     vs_terms = np.zeros((m_src, 2), dtype=np.int64)
@@ -303,10 +325,10 @@ def gate_dag(n_gates: int = 20000, seed: int = 0, n_pi: int = 64, fanout_max: in
     vs = SourceGroup(kind="vsource", names=["vdd"] + [f"vpi{i:0{pw}d}" for i in range(n_pi)],
                      terminals=vs_terms, wave=vs_wave, params=vsp)
     return Netlist(
-        title=f"gate_dag_{n_gates}_s{seed}",
+        title=f"gate_dag_{n_gates}_s{seed}" + ("" if model == "mosfet" else f"_{model}"),
         nodes=nodes,
         ground="0",
-        groups={"mosfet": mos},
+        groups={model: mos},
         vsources=vs,
         isources=empty_sources("isource"),
         analysis={"stop": t_stop, "step": dt},

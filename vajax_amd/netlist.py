@@ -36,6 +36,34 @@ MODEL_TERMINALS: Dict[str, List[str]] = {
     # junction diode, anode A, cathode K; its junction voltage goes through
     # $limit(..., "pnjlim", ...) (vajax/analysis/limiting.py:22-82)
     "diode": ["A", "K"],
+    # EKV 2.6 with its charges and junction diodes (vajax/devices/models/ekv/ekv.va)
+    "ekv26": ["d", "g", "s", "b"],
+}
+
+# EKV 2.6 instance parameters and their defaults (ekv.va:237-296).  "temp" and
+# "tnom" keep the file's not-given marker 1e21: the device then runs at the
+# simulation temperature plus "trise", with Tnom = 25 C.  "noise" is accepted
+# and ignored (no noise analysis here).
+EKV26_PARAMS: Dict[str, float] = {
+    "type": 1.0, "noise": 1.0, "trise": 0.0, "temp": 1e21, "tnom": 1e21,
+    "l": 10e-6, "w": 10e-6, "m": 1.0, "ns": 1.0,
+    "as": 0.0, "ad": 0.0, "ps": 0.0, "pd": 0.0,
+    "cox": 2.0e-3, "xj": 300e-9, "vto": 0.5, "tcv": 1.0e-3, "gamma": 0.7, "phi": 0.5,
+    "kp": 150e-6, "bex": -1.5, "theta": 0.0, "e0": 1.0e8, "ucrit": 2.0e6, "ucex": 0.8,
+    "lambda": 0.8, "dl": -0.01e-6, "dw": -0.01e-6, "weta": 0.2, "leta": 0.3,
+    "q0": 230e-6, "lk": 0.4e-6, "iba": 5.0e8, "ibb": 4.0e8, "ibbt": 9.0e-4, "ibn": 1.0,
+    "rsh": 0.0, "hdif": 0.5e-6, "avto": 1e-6, "akp": 1e-6, "agamma": 1e-6,
+    "af": 1.0, "kf": 0.0,
+    "xd_n": 1.0, "xd_js": 1.0e-09, "xd_jsw": 1.0e-12, "xd_jswg": 1.0e-12,
+    "xd_mj": 0.900, "xd_mjsw": 0.700, "xd_mjswg": 0.700,
+    "xd_pb": 0.800, "xd_pbsw": 0.600, "xd_pbswg": 0.600,
+    "xd_cj": 1.0e-09, "xd_cjsw": 1.0e-12, "xd_cjswg": 1.0e-12,
+    "xd_gmin": 0.0, "xd_xjbv": 0.0, "xd_bv": 10.0,
+    "xd_njts": 1.0, "xd_njtssw": 1.0, "xd_njtsswg": 1.0,
+    "xd_vts": 0.0, "xd_vtssw": 0.0, "xd_vtsswg": 0.0,
+    "tp_xti": 3.0, "tp_cj": 0.0, "tp_cjsw": 0.0, "tp_cjswg": 0.0,
+    "tp_pb": 0.0, "tp_pbsw": 0.0, "tp_pbswg": 0.0,
+    "tp_njts": 0.0, "tp_njtssw": 0.0, "tp_njtsswg": 0.0,
 }
 
 # Per-model instance parameters and defaults.
@@ -62,6 +90,7 @@ MODEL_PARAMS: Dict[str, Dict[str, float]] = {
     # SPICE diode defaults: saturation current, emission coefficient, a
     # linear junction capacitance
     "diode": {"is": 1e-14, "n": 1.0, "cj": 0.0},
+    "ekv26": EKV26_PARAMS,
 }
 
 SOURCE_DEFAULTS: Dict[str, float] = {

@@ -42,12 +42,13 @@ MODEL_VOLTAGES: Dict[str, List[Tuple[str, str]]] = {
     "resistor": [("A", "B")],
     "capacitor": [("A", "B")],
     "mosfet": [("g", "b"), ("s", "b"), ("d", "b")],
+    "ekv26": [("g", "b"), ("s", "b"), ("d", "b")],
     "diode": [("A", "K")],
 }
-MODEL_IDS = {"resistor": 0, "capacitor": 1, "mosfet": 2, "diode": 3}
+MODEL_IDS = {"resistor": 0, "capacitor": 1, "mosfet": 2, "diode": 3, "ekv26": 4}
 # limit states per instance (one per $limit call), flat over models in order,
 # instances, states (vajax/analysis/mna_builder.py:387-401)
-MODEL_LIMIT_STATES = {"resistor": 0, "capacitor": 0, "mosfet": 0, "diode": 1}
+MODEL_LIMIT_STATES = {"resistor": 0, "capacitor": 0, "mosfet": 0, "diode": 1, "ekv26": 0}
 
 
 @dataclass

@@ -29,7 +29,8 @@ vajax/utils/safe_eval.py:19-200):
 Device mapping onto the models built here: ``resistor``/``sp_resistor``,
 ``capacitor``/``sp_capacitor``, ``vsource``, ``isource``, ``diode``/``sp_diode``
 (junction diode: ``is``, ``n``, ``cj`` or ``cjo``), ``ekv`` (the EKV
-long-channel module, parameters by name) and ``psp103va`` (no PSP103 here: its
+long-channel module, parameters by name), ``ekv_va`` (the EKV 2.6 module of
+vajax/devices/models/ekv/ekv.va, parameters by name) and ``psp103va`` (no PSP103 here: its
 instances become the EKV substitute of the synthetic circuits, with the
 card's ``type`` polarity and the instance's ``w``/``l``).
 """
@@ -643,6 +644,7 @@ MODULE_KIND = {
     "vsource": "vsource", "isource": "isource",
     "diode": "diode", "sp_diode": "diode",
     "ekv": "mosfet", "ekv_longchannel": "mosfet", "psp103va": "mosfet_psp",
+    "ekv_va": "ekv26", "ekv26": "ekv26",
 }
 _ALIASES = {"diode": {"cjo": "cj", "cj0": "cj"}}
 # netlist options this build reads (options.py names)
