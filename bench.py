@@ -56,10 +56,10 @@ def build_circuit(gates: int, seed: int):
 
 
 def run_gpu(eng, steps: int, precision: str, use_graph: bool, band_width=None, autotune=True,
-            nd_leaf=None, window_from=0, replay=None):
+            nd_leaf=None, window_from=0, replay=None, t_stop=1e-9):
     from vajax_amd.engine import AdaptiveConfig
 
-    eng.prepare(t_stop=1e-9, dt=1e-12, adaptive_config=AdaptiveConfig(integration_method="gear2"),
+    eng.prepare(t_stop=t_stop, dt=1e-12, adaptive_config=AdaptiveConfig(integration_method="gear2"),
                 precision=precision, max_steps=steps + 1, use_graph=use_graph,
                 band_width=band_width, autotune=autotune, nd_leaf=nd_leaf,
                 window_from_step=window_from, replay_times=replay)
@@ -86,8 +86,8 @@ N_PAR = {"resistor": 1, "capacitor": 1, "mosfet": 11, "ekv26": 52}
 # FP64 VALU operations of one EKV 2.6 instance eval (value + three partials),
 # from rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 over the eval
 # kernel of the 65k-gate EKV 2.6 DAG (64 lanes per instruction, FMA = 2),
-# divided by its instances per launch
-EKV26_F64_OPS_PER_INSTANCE = None
+# divided by its instances per launch (profiles/eval_valu_ekv26_gate_dag_65000.md)
+EKV26_F64_OPS_PER_INSTANCE = 5595
 FP64_VALU_PEAK_TFLOPS = 78.6
 
 
@@ -214,6 +214,38 @@ def _counter_means(path):
     return {k: (len(v), sum(v.values()) / len(v)) for k, v in per.items()}
 
 
+VALU_F64 = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
+            "SQ_INSTS_VALU_TRANS_F64", "SQ_INSTS_VALU_FLOPS_FP64", "SQ_INSTS_VALU_FLOPS_FP64_TRANS")
+
+
+def summarize_valu(path, kernel, grid):
+    """FP64 VALU work of one kernel from a rocprofv3 --pmc CSV holding the
+    VALU_F64 counters: the mean per dispatch over the dispatches of
+    ``kernel`` with ``grid`` work-items, and the operations per work-item
+    (64 lanes per instruction, an FMA counting two)."""
+    import csv
+
+    per = {}
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            if _kernel_name(r["Kernel_Name"]) != kernel or int(r["Grid_Size"]) != grid:
+                continue
+            d = per.setdefault(r["Counter_Name"], {})
+            d[r["Dispatch_Id"]] = d.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    mean = {c: sum(v.values()) / len(v) for c, v in per.items()}
+    n = {c: len(v) for c, v in per.items()}
+    ops = 64.0 * (mean["SQ_INSTS_VALU_ADD_F64"] + mean["SQ_INSTS_VALU_MUL_F64"]
+                  + 2.0 * mean["SQ_INSTS_VALU_FMA_F64"] + mean["SQ_INSTS_VALU_TRANS_F64"])
+    print(f"* kernel `{kernel}`, grid {grid} work-items, {min(n.values())} dispatches\n")
+    print("| counter | mean per dispatch | per work-item |")
+    print("|---|---:|---:|")
+    for c in VALU_F64:
+        if c in mean:
+            print(f"| {c} | {mean[c]:.4g} | {mean[c] / grid:.2f} |")
+    print(f"\nFP64 operations per work-item, 64 (ADD + MUL + 2 FMA + TRANS) / grid: "
+          f"{ops / grid:.1f}")
+
+
 def summarize_pmc(paths, json_out):
     """HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE passes (KiB per
     dispatch), each scaled by what the same counter reads for the 1 GiB
@@ -306,7 +338,7 @@ def numpy_oracle_baseline(args):
             "sample": f"first {acc} accepted steps, numpy/SciPy oracle, one thread"}
 
 
-def run_batched(args):
+def run_batched(args, model="mosfet"):
     """Batched throughput: Monte Carlo instances of a 2k-gate DAG plus ragged
     topologies packed into one block-diagonal system (shared analysis per
     distinct block pattern, one launch per level for every instance), all
@@ -318,7 +350,7 @@ def run_batched(args):
 
     t0 = time.time()
     ragged = tuple(int(g) for g in args.batch_ragged.split(",")) if args.batch_ragged else ()
-    net = G.batch_mc(args.batch, args.batch_gates, ragged=ragged, seed=args.seed)
+    net = G.batch_mc(args.batch, args.batch_gates, ragged=ragged, seed=args.seed, model=model)
     n_inst = args.batch + len(ragged)
     eng = CircuitEngine(net)
     eng.parse()
@@ -326,7 +358,7 @@ def run_batched(args):
     log(f"batch setup {time.time() - t0:.1f}s: {n_inst} instances, "
         f"n_unknowns={topo.n_augmented} nse={topo.nse}")
     out = {"workload": net.title, "instances": n_inst, "unknowns": topo.n_augmented,
-           "mosfets": int(net.groups["mosfet"].count), "unit": "instance·steps/s"}
+           "mosfets": int(net.groups[model].count), "unit": "instance·steps/s"}
     wb = args.warmup
     for prec in ("f32", "f64"):
         t0 = time.time()
@@ -339,7 +371,7 @@ def run_batched(args):
         out[prec] = {"value": round(n_inst * w["accepted_steps"] / w["loop_s"], 3), **w}
         log(f"batch {prec}: {w['accepted_steps']} steps in {w['loop_s']:.3f}s")
     # the plain form: one instance at a time, FP64
-    single = CircuitEngine(G.gate_dag(args.batch_gates, seed=args.seed))
+    single = CircuitEngine(G.gate_dag(args.batch_gates, seed=args.seed, model=model))
     single.parse()
     run_gpu(single, 3, "f64", True)
     res = run_gpu(single, wb + args.batch_steps, "f64", True, window_from=wb + 1)
@@ -466,6 +498,7 @@ def run_ekv26(args):
     gbs = alg / (avg_ms * 1e-3) / 1e9
     out = {"workload": f"gate_dag_{args.gates}_seed{args.seed} with EKV 2.6 MOSFETs",
            "mosfets": n_dev, "unknowns": topo.n_augmented, **w,
+           "analysis_ms": [round(x, 1) for x in eng.device.lu.info()["analyze_ms"]],
            "phase_ms_per_newton_iter": {"build": round(ph["build_ms"] / iters, 4),
                                         "lu": round(ph["lu_ms"] / iters, 4),
                                         "solve": round(ph["solve_ms"] / iters, 4)},
@@ -478,6 +511,24 @@ def run_ekv26(args):
         out["eval"].update({"f64_ops_per_instance": EKV26_F64_OPS_PER_INSTANCE,
                             "achieved_f64_TFLOPs": round(tf, 3),
                             "f64_valu_frac": round(tf / FP64_VALU_PEAK_TFLOPS, 5)})
+    del eng
+    # the other synthetic workloads with the same model
+    small = {}
+    for name, netf, warm, steps, t_stop in (
+            ("dag20k", lambda: G.gate_dag(20000, seed=args.seed, model="ekv26"), args.warmup,
+             args.steps, 1e-9),
+            ("ring101", lambda: G.ring_oscillator(101, seed=args.seed, model="ekv26"), 200, 2000,
+             50e-9)):
+        e = CircuitEngine(netf())
+        e.parse()
+        run_gpu(e, 3, "f64", True, t_stop=t_stop)
+        r = run_gpu(e, warm + steps, "f64", True, window_from=warm + 1, t_stop=t_stop)
+        small[name] = {"unknowns": e.topology.n_augmented, **window_stats(r)}
+        log(f"ekv26 {name}: {small[name]['ms_per_step']} ms/step")
+        del e
+    out.update(small)
+    if args.batch > 0:
+        out["batch"] = run_batched(args, model="ekv26")
     return out
 
 
@@ -558,12 +609,21 @@ def main():
                     help="FETCH,WRITE,CAL_FETCH,CAL_WRITE: four rocprofv3 --pmc CSVs (the bench "
                          "and --copy-calibration); write --pmc-json, print a table and exit")
     ap.add_argument("--pmc-json", default="hbm_traffic.json")
+    ap.add_argument("--summarize-valu", default=None,
+                    help="rocprofv3 --pmc CSV of the FP64 VALU counters: print the FP64 "
+                         "operations per work-item of --valu-kernel and exit")
+    ap.add_argument("--valu-kernel", default="k_eval<4>")
+    ap.add_argument("--valu-grid", type=int, default=260096,
+                    help="work-items of the dispatches to average (260096: 260 000 devices)")
     args = ap.parse_args()
     if args.summarize_trace:
         summarize_trace(args.summarize_trace, args.window)
         return
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc.split(","), args.pmc_json)
+        return
+    if args.summarize_valu:
+        summarize_valu(args.summarize_valu, args.valu_kernel, args.valu_grid)
         return
     if args.copy_calibration:
         import torch

@@ -169,3 +169,46 @@ def _crossings(times, V, level=0.5):
         s = np.nonzero(np.sign(v[:-1]) * np.sign(v[1:]) < 0)[0]
         out.append(times[s] - v[s] * (times[s + 1] - times[s]) / (v[s + 1] - v[s]))
     return out
+
+
+def _period(times, v, level=0.5):
+    up = np.nonzero((v[:-1] < level) & (v[1:] >= level))[0]
+    t = times[up] + (level - v[up]) * (times[up + 1] - times[up]) / (v[up + 1] - v[up])
+    return np.diff(t)
+
+
+def test_ekv26_ring_oscillator():
+    """11-stage ring of EKV 2.6 inverters: on the oracle's grid to the FP64
+    bar, and free-running with the same period to 1 %."""
+    net = G.ring_oscillator(11, seed=2, t_stop=2e-9, model="ekv26")
+    _say("ekv26 ring 11: oracle adaptive 2 ns")
+    cir, ref = run_oracle(net, 2e-9, 1e-12)
+    _, rr = run_gpu(net, ref.times[-1], 1e-12, use_graph=True, replay_times=ref.times)
+    assert np.array_equal(rr.times, ref.times) and rr.stats["rejected_steps"] == 0
+    err = np.abs(rr.stats["V_out"] - ref.V)
+    _say(f"ekv26 ring 11: max |dV| on the oracle grid {float(err.max()):.3e} V")
+    assert float(np.max(err - 1e-6 * np.abs(ref.V))) <= 1e-9
+    _, res = run_gpu(net, 2e-9, 1e-12, use_graph=True)
+    j = cir.node_names["out003"]
+    pr = _period(ref.times, ref.V[:, j])
+    pg = _period(res.times, res.stats["V_out"][:, j])
+    _say(f"ekv26 ring 11: periods oracle {pr} gpu {pg}")
+    assert len(pr) >= 2 and len(pg) == len(pr)
+    assert np.all(np.abs(pg - pr) <= 1e-2 * pr)
+
+
+def test_ekv26_packed_batch_fixed_step_fp64_and_mixed():
+    """Monte Carlo copies and a ragged topology of EKV 2.6 DAGs packed into
+    one block-diagonal system, on a fixed 1 ps grid against the oracle run of
+    the same packed netlist."""
+    net = G.batch_mc(3, 60, ragged=(40, 90), seed=3, model="ekv26")
+    _say("ekv26 batch: oracle 100 fixed steps")
+    _, ref = run_oracle(net, 0.1e-9, 1e-12, fixed=True, tran_fs=1.0)
+    _, r64 = run_gpu(net, 0.1e-9, 1e-12, fixed=True, tran_fs=1.0, use_graph=True)
+    assert np.array_equal(r64.times, ref.times)
+    e64 = float(np.max(np.abs(r64.stats["V_out"] - ref.V)))
+    _, r32 = run_gpu(net, 0.1e-9, 1e-12, fixed=True, tran_fs=1.0, precision="f32",
+                     use_graph=True)
+    e32 = float(np.max(np.abs(r32.stats["V_out"] - ref.V)))
+    _say(f"ekv26 batch: max |dV| fp64 {e64:.3e} V, mixed {e32:.3e} V")
+    assert e64 <= 1e-9 and e32 <= 1e-6, (e64, e32)

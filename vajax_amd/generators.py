@@ -153,14 +153,17 @@ def diode_clamp_chain(stages: int = 6, vdc: float = 5.0, r: float = 50.0) -> Net
     return b.build(analysis={"stop": 1e-9, "step": 1e-12}, options={"icmode": "op"})
 
 
-def ring_oscillator(stages: int = 101, seed: int = 0, t_stop: float = 50e-9, dt: float = 1e-12) -> Netlist:
+def ring_oscillator(stages: int = 101, seed: int = 0, t_stop: float = 50e-9, dt: float = 1e-12,
+                    model: str = "mosfet") -> Netlist:
     """CMOS inverter ring with per-stage random W/L and load caps.
 
     Each stage: NMOS(out_i, in_i, 0, 0) and PMOS(out_i, in_i, vdd, vdd) with
     in_i = out_{i-1}; W ~ U[0.2, 2] um, L ~ U[45, 90] nm, C_load ~ U[1, 5] fF.
     Started from all-zero initial conditions (icmode uic) with a 100 uA,
-    50 ps current pulse drawn out of stage 0.
+    50 ps current pulse drawn out of stage 0.  ``model`` as for
+    :func:`gate_dag`.
     """
+    nmos, pmos = _technology(model)
     if stages % 2 == 0:
         raise ValueError("ring needs an odd number of stages")
     rng = np.random.default_rng(seed)
@@ -174,13 +177,22 @@ def ring_oscillator(stages: int = 101, seed: int = 0, t_stop: float = 50e-9, dt:
     out = [f"out{i:03d}" for i in range(stages)]
     for i in range(stages):
         src = out[i - 1]
-        b.device("mosfet", f"mn{i:03d}", [out[i], src, "0", "0"], w=wn[i], l=ln[i], **NMOS_DEFAULTS)
-        b.device("mosfet", f"mp{i:03d}", [out[i], src, "vdd", "vdd"], w=wp[i], l=lp[i], **PMOS_DEFAULTS)
+        b.device(model, f"mn{i:03d}", [out[i], src, "0", "0"], w=wn[i], l=ln[i], **nmos)
+        b.device(model, f"mp{i:03d}", [out[i], src, "vdd", "vdd"], w=wp[i], l=lp[i], **pmos)
         b.device("capacitor", f"cl{i:03d}", [out[i], "0"], c=cl[i])
     b.source("isource", "ikick", out[0], "0", type="pulse", val0=0.0, val1=100e-6, rise=10e-12,
              fall=10e-12, width=50e-12, delay=0.0, period=1.0)
     return b.build(analysis={"stop": t_stop, "step": dt},
                    options={"tran_method": "gear2", "icmode": "uic"})
+
+
+def _technology(model: str):
+    """(NMOS, PMOS) instance parameters of the synthetic circuits for a model."""
+    if model == "mosfet":
+        return NMOS_DEFAULTS, PMOS_DEFAULTS
+    if model == "ekv26":
+        return EKV26_NMOS, EKV26_PMOS
+    raise ValueError(f"model must be 'mosfet' or 'ekv26', not {model!r}")
 
 
 def _dag_inputs(rng: np.random.Generator, n_gates: int, n_pi: int, fanout_max: int,
@@ -240,8 +252,7 @@ def gate_dag(n_gates: int = 20000, seed: int = 0, n_pi: int = 64, fanout_max: in
     substitute) or "ekv26" (EKV 2.6 at :data:`EKV26_NMOS` / :data:`EKV26_PMOS`);
     both draw the same random numbers, so they give the same netlist topology.
     """
-    if model not in ("mosfet", "ekv26"):
-        raise ValueError(f"gate_dag: model must be 'mosfet' or 'ekv26', not {model!r}")
+    _technology(model)
     rng = np.random.default_rng(seed)
     is_nand = rng.random(n_gates) < 0.5
     inputs = _dag_inputs(rng, n_gates, n_pi, fanout_max, window, mean_dist)
@@ -339,12 +350,13 @@ def gate_dag(n_gates: int = 20000, seed: int = 0, n_pi: int = 64, fanout_max: in
 def monte_carlo(net: Netlist, seed: int, sigma: float = 0.03) -> Netlist:
     """Copy of ``net`` with per-MOSFET Vth and mobility variation.
 
-    dVth ~ N(0, sigma * vto) added to vto, kp scaled by N(1, sigma).
+    dVth ~ N(0, sigma * vto) added to vto, kp scaled by N(1, sigma), for the
+    MOSFETs of either model.
     """
     rng = np.random.default_rng(seed)
     out = copy.deepcopy(net)
     out.title = f"{net.title}_mc{seed}"
-    g = out.groups.get("mosfet")
+    g = out.groups.get("mosfet", out.groups.get("ekv26"))
     if g is not None:
         n = g.count
         g.params["vto"] = g.params["vto"] + rng.normal(0.0, 1.0, n) * sigma * g.params["vto"]
@@ -424,18 +436,19 @@ def pack(nets, title: str = "batch") -> Netlist:
 
 
 def batch_mc(n_instances: int = 256, gates: int = 2000, ragged=(500, 1000, 2000, 4000),
-             seed: int = 0, sigma: float = 0.03) -> Netlist:
+             seed: int = 0, sigma: float = 0.03, model: str = "mosfet") -> Netlist:
     """Monte Carlo batch plus ragged topologies, packed into one netlist.
 
     ``n_instances`` copies of the ``gates``-gate DAG (seed ``seed``) with
     per-device Vth / mobility variation (instance ``b`` drawn with seed
     ``seed + 1 + b``), followed by one DAG of each size in ``ragged`` (seeds
-    ``seed + 1000 + j``).
+    ``seed + 1000 + j``).  ``model`` as for :func:`gate_dag`.
     """
-    base = gate_dag(gates, seed=seed)
+    base = gate_dag(gates, seed=seed, model=model)
     nets = [monte_carlo(base, seed + 1 + b, sigma) for b in range(n_instances)]
-    nets += [gate_dag(g, seed=seed + 1000 + j) for j, g in enumerate(ragged)]
-    return pack(nets, title=f"batch_mc{n_instances}x{gates}_ragged{len(ragged)}_s{seed}")
+    nets += [gate_dag(g, seed=seed + 1000 + j, model=model) for j, g in enumerate(ragged)]
+    suffix = "" if model == "mosfet" else f"_{model}"
+    return pack(nets, title=f"batch_mc{n_instances}x{gates}_ragged{len(ragged)}_s{seed}{suffix}")
 
 
 def array_multiplier_netlist(n: int = 64) -> str:
