@@ -350,6 +350,13 @@ enum Ekv26Col {
   EK_CJG, EK_CJA_S, EK_CJP_S, EK_MJ, EK_MJSW, EK_MJSWG, EK_PB, EK_PBSW, EK_PBSWG, EK_NCOL
 };
 
+// Column k of instance i's cache, read where it is used.
+struct Ekv26Cols {
+  const double* p;
+  int N, i;
+  __device__ __forceinline__ double operator[](int k) const { return p[(size_t)k * N + i]; }
+};
+
 // Trap-assisted tunnelling term  exp(v k vt / max(vt + v, 1e-3)) - 1  (ekv.va:800-802).
 __device__ __forceinline__ D3 ekv_tunnel(D3 v, double k, double vt) {
   const D3 den = vt + v.v > 1.0e-3 ? vt + v : dc(1.0e-3);
@@ -359,7 +366,7 @@ __device__ __forceinline__ D3 ekv_tunnel(D3 v, double k, double vt) {
 // One junction diode between the drain (or source) and the bulk, v = TYPE V(x,b):
 // current (ekv.va:792-803) and charge (ekv.va:818-831), both without TYPE M.
 __device__ __forceinline__ void ekv_junction(D3 v, double is, double tun_p, double tun_a,
-                                             double cja, double cjp, const double* c, D3& cur,
+                                             double cja, double cjp, const Ekv26Cols& c, D3& cur,
                                              D3& q) {
   D3 arg = -v * c[EK_JK];
   if (arg.v < -40.0) arg = dc(-40.0);
@@ -385,9 +392,7 @@ __device__ __forceinline__ void ekv_junction(D3 v, double is, double tun_p, doub
 // order d, g, s, b; Jacobian keys row-major over (d, g, s, b).
 __device__ __forceinline__ void eval_ekv26(const double* vb, const double* par, int i, int N,
                                            double* rr, double* rc, double* jr, double* jc) {
-  double c[EK_NCOL];
-#pragma unroll
-  for (int k = 0; k < EK_NCOL; ++k) c[k] = par[(size_t)k * N + i];
+  const Ekv26Cols c{par, N, i};
   const double TYPE = c[EK_TYPE], VTO_S = c[EK_VTO_S], PHI_T = c[EK_PHI_T];
   const double GAMMA_S = c[EK_GAMMA_S], KP_Weff = c[EK_KP_WEFF], Leff = c[EK_LEFF];
   const double UCRIT_T = c[EK_UCRIT_T], LAMBDA = c[EK_LAMBDA], Lc = c[EK_LC];
