@@ -286,3 +286,39 @@ def test_tb_dp_benchmark_netlist_matches_fixture():
     assert sum(1 for w in net.vsources.wave if w == 3) == 28  # pwl clocks and inputs
     for i in range(net.vsources.count):
         assert np.array_equal(net.vsources.pwl_points(i), fx.vsources.pwl_points(i))
+
+
+EKV_SIM = """EKV 2.6 inverter
+load "ekv.osdi"
+
+model nch ekv_va type=1 vto=0.4 kp=300u cox=15m xj=30n
+model pch ekv_va type=-1 vto=-0.4 kp=120u cox=15m xj=30n
+model v vsource
+
+vdd (vdd 0) v dc=1
+vin (in 0) v dc=0.3
+mn (out in 0 0) nch w=1u l=0.1u
+mp (out in vdd vdd) pch w=2u l=0.1u
+
+control
+  analysis op1 op
+endc
+"""
+
+
+def test_ekv26_module_maps_onto_the_ekv26_model():
+    from oracle import ekv26 as E
+    from vajax_amd.models import EKV26_CACHE_COLUMNS, ekv26_cache
+
+    net = V.to_netlist(V.parse_netlist("", EKV_SIM))
+    g = net.groups["ekv26"]
+    assert g.count == 2
+    assert list(g.params["type"]) == [1.0, -1.0]
+    assert np.allclose(g.params["vto"], [0.4, -0.4]) and np.allclose(g.params["w"], [1e-6, 2e-6])
+    assert np.allclose(g.params["kp"], [300e-6, 120e-6]) and np.allclose(g.params["cox"], 15e-3)
+    assert g.params["lambda"][0] == 0.8  # the file's default where the card is silent
+    # the instance cache the eval kernel reads equals the oracle's
+    cache = dict(zip(EKV26_CACHE_COLUMNS, ekv26_cache(g.params)))
+    k = E.cache(g.params)
+    assert np.allclose(cache["vto_s"], k["VTO_S"], rtol=1e-15)
+    assert np.allclose(cache["kp_weff"], k["KP_Weff"], rtol=1e-15)
