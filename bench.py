@@ -393,6 +393,46 @@ MFMA_F64_PEAK_TFLOPS = 78.6
 MUL64_REFERENCE_MS_PER_STEP = 648.0  # JAX GPU, cuDSS f32 + refinement, Tesla T4, PSP103
 
 
+# the reference's published small-circuit step times (JAX GPU path, its CI GPU
+# runner; README.md:88-89): a 47-node PSP103 ring and a 4-node RC circuit
+REFERENCE_SMALL_MS_PER_STEP = {"ring": 1.49, "rc": 0.24}
+
+
+def run_small(args):
+    """The two small configurations, whose step time is launch- and
+    round-trip-bound: the 200-section RC ladder (fixed-step BDF2, 10 000
+    steps, the configuration's own length) and the 101-stage CMOS ring
+    (adaptive Gear2, 2 000 timed steps after 200), one HIP-graph Newton
+    iteration per launch."""
+    from vajax_amd import generators as G
+    from vajax_amd.engine import AdaptiveConfig, CircuitEngine
+
+    out = {}
+    eng = CircuitEngine(G.rc_ladder(200, seed=args.seed))
+    eng.parse()
+    eng.options.fixed_step_transient = True
+    cfg = AdaptiveConfig(integration_method="gear2")
+    cfg.tran_fs = 1.0
+    eng.prepare(t_stop=10e-9, dt=1e-12, adaptive_config=cfg, use_graph=True, max_steps=10002)
+    eng.run_transient()
+    eng.prepare(t_stop=10e-9, dt=1e-12, adaptive_config=cfg, use_graph=True, max_steps=10002,
+                window_from_step=1)
+    out["rc_ladder_200"] = {"unknowns": eng.topology.n_augmented,
+                            "integration": "gear2 fixed 1 ps", **window_stats(eng.run_transient())}
+    e = CircuitEngine(G.ring_oscillator(101, seed=args.seed))
+    e.parse()
+    run_gpu(e, 3, "f64", True, t_stop=50e-9)
+    r = run_gpu(e, 2200, "f64", True, window_from=201, t_stop=50e-9)
+    out["ring_101"] = {"unknowns": e.topology.n_augmented, "integration": "gear2 adaptive",
+                       **window_stats(r)}
+    out["reference_ms_per_step"] = REFERENCE_SMALL_MS_PER_STEP
+    out["reference_note"] = ("JAX GPU path on the reference's CI GPU runner: a 47-node PSP103 "
+                             "ring and a 4-node RC circuit -- other circuits, model and hardware")
+    for k in ("rc_ladder_200", "ring_101"):
+        log(f"small {k}: {out[k]['ms_per_step']} ms/step")
+    return out
+
+
 def run_mul64(args):
     """The reference's own benchmark circuit, mul64
     (vajax/benchmarks/data/mul64/runme.sim: the 64x64 array multiplier of its
@@ -595,6 +635,10 @@ def main():
                     help="skip the mul64 leg (the reference's 64x64 multiplier benchmark)")
     ap.add_argument("--mul64-only", action="store_true",
                     help="run only the mul64 leg and print its JSON")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the small-circuit leg (RC ladder, ring oscillator)")
+    ap.add_argument("--small-only", action="store_true",
+                    help="run only the small-circuit leg and print its JSON")
     ap.add_argument("--no-ekv26", action="store_true",
                     help="skip the EKV 2.6 leg (the DAG with the full EKV 2.6 MOSFET model)")
     ap.add_argument("--ekv26-only", action="store_true",
@@ -647,6 +691,9 @@ def main():
         return
     if args.ekv26_only:
         print(json.dumps(run_ekv26(args)), flush=True)
+        return
+    if args.small_only:
+        print(json.dumps(run_small(args)), flush=True)
         return
     from vajax_amd import _lib as L
     from vajax_amd.engine import CircuitEngine
@@ -768,6 +815,10 @@ def main():
     if not args.no_ekv26:
         ekv26 = run_ekv26(args)
 
+    small = None
+    if not args.no_small:
+        small = run_small(args)
+
     line = {
         "metric": METRIC,
         "value": win["newton_iters_per_s"],
@@ -811,6 +862,7 @@ def main():
         "batched": batched,
         "mul64": mul64,
         "ekv26": ekv26,
+        "small": small,
         "mixed_precision": mixed,
     }
     if cpu:

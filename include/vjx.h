@@ -145,6 +145,11 @@ typedef struct {
    * earlier pivots contributing to it and its FP64 operations per refactor */
   int32_t dense_k, dense_m;
   double dense_flops;
+  /* nested-dissection leaf of the ordering in use (orderings 2 and 3; 0
+   * otherwise); with ordering 3 and leaf 0 (automatic) systems of up to
+   * 4 096 rows take 256- or 4-row leaves, whichever program has fewer pivot
+   * levels for its updates, larger ones 256 */
+  int32_t nd_leaf;
 } vjx_lu_info;
 
 int vjx_lu_get_info(const vjx_pattern* pat, vjx_lu_info* out);
@@ -164,7 +169,9 @@ int vjx_pattern_set_band_width(vjx_pattern* pat, int32_t width);
  * 1 = natural, 2 or 3 = nested dissection with minimum-degree leaves of at
  * most nd_leaf rows, its vertex separators from BFS level structures (2) or
  * from multilevel bisection with node refinement (3); nd_leaf 0 = max(48,
- * n / 128) per diagonal block (2) or 256 (3); default 3, 0.  The
+ * n / 128) per diagonal block (2) or automatic (3: 256, or 4 on systems of
+ * up to 4 096 rows when that program is estimated faster, see
+ * vjx_lu_info.nd_leaf); default 3, 0.  The
  * role of UMFPACK's ordering choice (vajax/sparse/umfpack_ffi.cpp:140-179
  * caches one symbolic analysis per pattern); a new value drops the analysis
  * like a new band width. */

@@ -51,6 +51,7 @@ class LuInfo(C.Structure):
         ("tuned_solve_ms", C.c_double), ("tuned_factor_ms", C.c_double),
         ("lu_pipeline", C.c_int32),
         ("dense_k", C.c_int32), ("dense_m", C.c_int32), ("dense_flops", C.c_double),
+        ("nd_leaf", C.c_int32),
     ]
 
     def as_dict(self):

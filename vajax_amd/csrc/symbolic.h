@@ -191,6 +191,7 @@ struct LuSymbolic {
   double flops = 0.0;
   int n_dense = 0;
   int n_fact_levels = 0;     // pivot levels of the refactorisation
+  int nd_leaf = 0;           // nested-dissection leaf in use (0: other orderings)
   // analysis time (ms): ordering, pivoting, LU pattern, refactorisation
   // program, solve programs (single-block systems)
   double analyze_ms[5] = {0, 0, 0, 0, 0};
