@@ -35,15 +35,15 @@ struct Work {
   // limit states (limited devices): at the step's start, of the current
   // Newton iterate, written by the last build, saved for a re-pivot redo
   DBuf<double> ls_step, ls_iter, ls_out, ls_save;
-  DBuf<DevStatus> st;
-  // [0] Newton / LTE status, [1] LU status: mapped pinned host memory the
-  // iteration's last kernel writes (d_hst: its device address)
+  DBuf<DevStatus> st, st_lte;
+  // [0] Newton status, [1] LU status, [2] LTE status: mapped pinned host
+  // memory the iteration's last kernels write (d_hst: its device address)
   DevStatus* h_st = nullptr;
   DevStatus* d_hst = nullptr;
   hipError_t alloc_status() {
-    hipError_t e = hipHostMalloc((void**)&h_st, 2 * sizeof(DevStatus), hipHostMallocMapped);
+    hipError_t e = hipHostMalloc((void**)&h_st, 3 * sizeof(DevStatus), hipHostMallocMapped);
     if (e != hipSuccess) return e;
-    std::memset(h_st, 0, 2 * sizeof(DevStatus));
+    std::memset(h_st, 0, 3 * sizeof(DevStatus));
     return hipHostGetDevicePointer((void**)&d_hst, h_st, 0);
   }
   ~Work() {
@@ -65,8 +65,19 @@ struct Driver {
   int64_t fill_base = 0;
   double gmin_dev = -1.0;  // device simparam $gmin; < 0: the options' gmin
 
+  // Pinned staging of the scalar uploads, two records used in turn: the
+  // step loop does not wait for a step's last upload (its final build's)
+  // before the next step writes the next one; every Newton iteration
+  // synchronises, so a record is free again two uploads later.
+  DevScalars* h_sc2 = nullptr;
+  int sc_flip = 0;
   int upload_scalars(double c0, double c1, double d1, double c2, double gshunt, int it) {
-    DevScalars* h = c->h_sc;
+    if (!h_sc2) {
+      VJX_TRY_HIP(hipHostMalloc((void**)&h_sc2, 2 * sizeof(DevScalars), hipHostMallocDefault));
+      std::memset(h_sc2, 0, 2 * sizeof(DevScalars));
+    }
+    DevScalars* h = h_sc2 + sc_flip;
+    sc_flip ^= 1;
     h->c0 = c0;
     h->c1 = c1;
     h->c2 = c2;
@@ -92,11 +103,20 @@ struct Driver {
     return VJX_OK;
   }
 
-  int read_status() {
-    launch_status_export(w.st.p, nullptr, w.d_hst, s);
-    VJX_TRY_HIP(hipGetLastError());
-    VJX_TRY_HIP(hipStreamSynchronize(s));
-    return VJX_OK;
+  // LTE of the step (adaptive.py:221-339), queued behind every Newton
+  // iteration of a transient step and exported with its status: the
+  // iteration that converges (or the last one) leaves the step's LTE norm in
+  // h_st[2] at the same synchronisation, so the step needs no round trip of
+  // its own.  It reads only the iterate, the predictor and V_max, none of
+  // which the step's final build changes.
+  bool lte_hook = false;
+  double lte_scale = 0.0;
+  void enqueue_lte() {
+    if (!lte_hook) return;
+    launch_status_reset(w.st_lte.p, s);
+    launch_lte(n_total, w.Xw.p, w.Vp.p, w.Vmax.p, lte_scale, cfg->reltol, cfg->abstol,
+               w.st_lte.p, s);
+    launch_status_export(w.st_lte.p, nullptr, w.d_hst + 2, s);
   }
 
   hipGraph_t graph = nullptr;
@@ -112,6 +132,7 @@ struct Driver {
   }
   ~Driver() {
     drop_graph();
+    if (h_sc2) (void)hipHostFree(h_sc2);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -234,14 +255,16 @@ struct Driver {
     if (!cfg->use_graph) {
       int rc = enqueue_iteration(Qp, dQ, Q2, floor);
       if (rc < 0) return rc;
+      enqueue_lte();
       VJX_TRY_HIP(hipGetLastError());
       return VJX_OK;
     }
-    const double* args[4] = {Qp, dQ, Q2, floor};
     int rc = prepare_graph(Qp, dQ, Q2, floor);
     if (rc < 0) return rc;
     VJX_TRY_HIP(hipGraphLaunch(graph_exec, s));
     ++graph_launches;
+    enqueue_lte();
+    VJX_TRY_HIP(hipGetLastError());
     return VJX_OK;
   }
 
@@ -396,6 +419,7 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
   VJX_TRY_HIP(w.vout.alloc((size_t)out_rows * n_total));
   VJX_TRY_HIP(w.iout.alloc((size_t)out_rows * mo));
   VJX_TRY_HIP(w.st.alloc(1));
+  VJX_TRY_HIP(w.st_lte.alloc(1));
   VJX_TRY_HIP(w.alloc_status());
   const int nl = c->n_lim;
   D.n_lim = nl;
@@ -680,19 +704,18 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       bool conv;
       double maxf;
       if (nl) launch_copy(nl, w.ls_step.p, w.ls_iter.p, s);
+      // LTE (adaptive.py:221-339): computed behind each Newton iteration on
+      // its iterate, read with the last one's status
+      D.lte_hook = true;
+      D.lte_scale = e_int / (e_int - pred_err);
       rc = D.nr_solve(c0, c1, d1, c2, gshunt, w.Qp.p, w.dQ.p, w.Q2.p, w.floor.p, iters, conv, maxf);
+      D.lte_hook = false;
       if (rc < 0) return rc;
       total_nr += iters;
       const bool nr_failed = !conv;
       const bool at_min = dt_cur <= cfg->min_dt;
       bool fail_min = nr_failed && at_min;
-      // LTE (adaptive.py:221-339)
-      launch_status_reset(w.st.p, s);
-      launch_lte(n_total, w.Xw.p, w.Vp.p, w.Vmax.p, e_int / (e_int - pred_err), cfg->reltol,
-                 cfg->abstol, w.st.p, s);
-      rc = D.read_status();
-      if (rc < 0) return rc;
-      const double lte_norm = bits_to_double(w.h_st->maxf_bits);
+      const double lte_norm = bits_to_double(w.h_st[2].maxf_bits);
       const int lorder = std::min(hist_count - 1, cfg->max_order);
       double factor = std::pow(cfg->lte_ratio / std::max(lte_norm, 1e-10), 1.0 / (lorder + 1));
       factor = std::min(std::max(factor, 0.1), cfg->grow_factor);
@@ -819,6 +842,7 @@ extern "C" int vjx_newton_solve(vjx_circuit* c, const vjx_tran_config* cfg,
   VJX_TRY_HIP(w.isrc.alloc(std::max(k, 1)));
   VJX_TRY_HIP(w.Ivs.alloc(std::max(m, 1)));
   VJX_TRY_HIP(w.st.alloc(1));
+  VJX_TRY_HIP(w.st_lte.alloc(1));
   VJX_TRY_HIP(w.alloc_status());
   const int nl = c->n_lim;
   D.n_lim = nl;
