@@ -145,10 +145,11 @@ typedef struct {
    * earlier pivots contributing to it and its FP64 operations per refactor */
   int32_t dense_k, dense_m;
   double dense_flops;
-  /* nested-dissection leaf of the ordering in use (orderings 2 and 3; 0
-   * otherwise); with ordering 3 and leaf 0 (automatic) systems of up to
-   * 4 096 rows take 256- or 4-row leaves, whichever program has fewer pivot
-   * levels for its updates, larger ones 256 */
+  /* nested-dissection leaf of the ordering in use: ordering 3 reports the
+   * leaf taken (with leaf 0, automatic, systems of up to 4 096 rows take
+   * 256- or 4-row leaves, whichever program has fewer pivot levels for its
+   * updates, larger ones 256); ordering 2 the configured leaf (0: max(48,
+   * n / 128) per diagonal block); 0 for the other orderings */
   int32_t nd_leaf;
 } vjx_lu_info;
 

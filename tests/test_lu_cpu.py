@@ -287,3 +287,27 @@ def test_multilevel_dissection_beats_level_structures_and_is_deterministic():
                 assert np.array_equal(p1, p2) and np.array_equal(q1, q2)
         assert stats["mlnd"]["n_updates"] < 0.6 * stats["nd"]["n_updates"], stats
         assert stats["mlnd"]["fact_levels"] < stats["nd"]["fact_levels"], stats
+
+
+def test_automatic_leaf_breaks_chains_on_small_systems():
+    """Ordering 3 with the automatic leaf: a 200-section RC ladder (a chain,
+    one pivot level per row inside a 256-row minimum-degree leaf) takes
+    4-row leaves and a logarithmic level count; a 2k-gate DAG, whose levels
+    come from its separators, keeps 256-row leaves; an explicit leaf is
+    taken as given.  Every program solves its system."""
+    for net, leaf, max_levels in ((G.rc_ladder(200), 4, 16), (G.gate_dag(2000, seed=0), 256, None)):
+        cir, csr, A = _system(net)
+        lu = SparseLU(cir.indptr, cir.indices)
+        lu.analyze(csr)
+        lu.check_schedules()
+        info = lu.info()
+        assert info["nd_leaf"] == leaf, info["nd_leaf"]
+        if max_levels:
+            assert info["fact_levels"] <= max_levels, info["fact_levels"]
+            fixed = SparseLU(cir.indptr, cir.indices)
+            fixed.set_ordering("mlnd", 256)
+            fixed.analyze(csr)
+            assert fixed.info()["nd_leaf"] == 256 and fixed.info()["fact_levels"] >= cir.n_aug // 2
+        b = np.random.default_rng(1).standard_normal(cir.n_aug)
+        x = lu.solve_cpu(csr, b)
+        assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(b)

@@ -94,7 +94,7 @@ int vjx_lu_analyze(vjx_pattern* pat, const double* data_host) {
     if (err.find("fill limit") != std::string::npos) return VJX_ERR_FILL;
     return err.find("singular") != std::string::npos ? VJX_ERR_SINGULAR : VJX_ERR_ARG;
   }
-  fresh.nd_leaf = nd ? (o.nd_leaf > 0 ? o.nd_leaf : std::max(48, pat->n / 128)) : 0;
+  fresh.nd_leaf = nd ? o.nd_leaf : 0;
   if (auto_leaf) {
     auto cost = [](const LuSymbolic& s) { return 2.8 * s.n_fact_levels + 5e-5 * s.n_updates; };
     AnalyzeOptions o4 = o;
