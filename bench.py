@@ -539,6 +539,7 @@ def run_ekv26(args):
     out = {"workload": f"gate_dag_{args.gates}_seed{args.seed} with EKV 2.6 MOSFETs",
            "mosfets": n_dev, "unknowns": topo.n_augmented, **w,
            "analysis_ms": [round(x, 1) for x in eng.device.lu.info()["analyze_ms"]],
+           "eval_blocks": eng.device.eval_blocks(),
            "phase_ms_per_newton_iter": {"build": round(ph["build_ms"] / iters, 4),
                                         "lu": round(ph["lu_ms"] / iters, 4),
                                         "solve": round(ph["solve_ms"] / iters, 4)},
@@ -844,6 +845,7 @@ def main():
             "solve_bands": [lu_info["fwd_bands"], lu_info["bwd_bands"]],
             "band_width": lu_info["band_width"],
             "lu_pipeline": lu_info["lu_pipeline"],
+            "eval_blocks": eng.device.eval_blocks(),
             "solve_band_width": lu_info["solve_band_width"],
             "integration": "gear2 adaptive",
             "precision": args.precision,

@@ -316,6 +316,11 @@ vjx_pattern* vjx_circuit_pattern(vjx_circuit* c);
 /* Limit states of the circuit's limited devices (one per $limit call), flat
  * over model groups in order, instances, states (mna_builder.py:387-401). */
 int vjx_circuit_limit_states(const vjx_circuit* c);
+/* Eval workgroup size of model group `group` (desc order): 256 unless the
+ * transient autotuner (vjx_tran_config.autotune) found 128 or 64 lanes 10 %
+ * faster for that group; negative on a bad argument.  No counterpart in the
+ * reference (a launch-plan knob). */
+int vjx_circuit_eval_block(const vjx_circuit* c, int32_t group);
 
 /* build_system (mna_builder.py:528-596): evaluate all devices at X and
  * assemble csr_data, f (augmented), Q, I_vsource and max_res_contrib; with

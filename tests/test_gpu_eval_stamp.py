@@ -69,3 +69,19 @@ def test_eval_stamp_per_group_matches_oracle(torch, gen):
             assert _close(out[key], w), (name, key)
         if cir.n_lim and name in (None, "diode"):
             assert _close(out["limit_state_out"], want[5]), (name, "limit_state")
+
+
+@pytest.mark.parametrize("model", ["mosfet", "ekv26"])
+def test_eval_workgroup_autotuner_keeps_results(model):
+    """The transient autotuner also picks each large model group's eval
+    workgroup size (256, 128 or 64 lanes): one lane per device, so the run
+    with the tuned plan equals the run without tuning bit for bit."""
+    from .helpers import run_gpu
+
+    net = G.gate_dag(2500, seed=5, model=model)  # 10 000 MOSFETs: tuned
+    eng, a = run_gpu(net, 30e-12, 1e-12, autotune=True, use_graph=True)
+    blocks = eng.device.eval_blocks()
+    assert blocks[model] in (64, 128, 256), blocks
+    _, b = run_gpu(net, 30e-12, 1e-12, autotune=False, use_graph=True)
+    assert np.array_equal(a.times, b.times)
+    assert np.array_equal(a.stats["V_out"], b.stats["V_out"])

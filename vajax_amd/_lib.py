@@ -194,6 +194,7 @@ _PROTOS = {
     "vjx_circuit_destroy": (None, [vp]),
     "vjx_circuit_pattern": (vp, [vp]),
     "vjx_circuit_limit_states": (C.c_int, [vp]),
+    "vjx_circuit_eval_block": (C.c_int, [vp, C.c_int32]),
     "vjx_build_system": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp,
                                    vp, vp, vp, vp]),

@@ -71,6 +71,47 @@ void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, co
   launch_residual(a, s);
 }
 
+int circuit_autotune_eval(vjx_circuit* c, const double* X, hipStream_t s) {
+  EvalOut out{c->jac_contrib.p, c->rf.p, c->rq.p};
+  hipEvent_t ev[2];
+  VJX_TRY_HIP(hipEventCreate(&ev[0]));
+  VJX_TRY_HIP(hipEventCreate(&ev[1]));
+  int rc = VJX_OK;
+  for (auto& mp : c->models) {
+    DevModel& m = *mp;
+    if (m.count < 8192 || m.n_lim > 0) continue;
+    const int sizes[3] = {256, 128, 64};
+    float t[3] = {1e30f, 1e30f, 1e30f};  // best of the rounds per size
+    for (int round = 0; round < 3 && rc == VJX_OK; ++round) {
+      for (int k = 0; k < 3; ++k) {
+        m.block = sizes[k];
+        launch_eval(m, X, c->d_sc.p, out, s);  // warm
+        hipError_t e = hipEventRecord(ev[0], s);
+        for (int r = 0; r < 5 && e == hipSuccess; ++r) launch_eval(m, X, c->d_sc.p, out, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+        if (e != hipSuccess) {
+          set_error(std::string("circuit_autotune_eval: ") + hipGetErrorString(e));
+          rc = VJX_ERR_HIP;
+          break;
+        }
+        t[k] = std::min(t[k], ms / 5.f);
+      }
+    }
+    // the incumbent (256) stays unless another size is 10 % faster
+    int best = 0;
+    for (int k = 1; k < 3; ++k)
+      if (t[k] < 0.9f * t[0] && t[k] < t[best]) best = k;
+    m.block = sizes[best];
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return rc;
+}
+
+
 }  // namespace vjx
 
 using namespace vjx;
@@ -355,6 +396,16 @@ void vjx_circuit_destroy(vjx_circuit* c) { delete c; }
 vjx_pattern* vjx_circuit_pattern(vjx_circuit* c) { return c ? c->pat : nullptr; }
 
 int vjx_circuit_limit_states(const vjx_circuit* c) { return c ? c->n_lim : 0; }
+
+int vjx_circuit_eval_block(const vjx_circuit* c, int32_t group) {
+  if (!c || group < 0 || group >= (int32_t)c->models.size()) {
+    set_error("vjx_circuit_eval_block: null circuit or group out of range");
+    return VJX_ERR_ARG;
+  }
+  return c->models[group]->block;
+}
+
+
 
 int vjx_build_system(vjx_circuit* c, const double* X, const double* vsrc, const double* isrc,
                      const double* Q_prev, const double* dQdt_prev, const double* Q_prev2,

@@ -99,4 +99,12 @@ void circuit_build_async(vjx_circuit* c, const double* X, const double* vsrc, co
                          double* I_vs, vjx::DevStatus* st, const double* ls_in, double* ls_out,
                          hipStream_t s, int only_model = -1);
 
+// Autotuner of the eval launch plan: for every model group of at least
+// 8192 devices without limit states, times its eval kernel at workgroups of
+// 256, 128 and 64 lanes (HIP events, best of three rounds of five launches)
+// on the iterate X and keeps another size only when it is 10 % faster.
+// Results do not depend on the size (one lane per device); the contribution
+// buffers it writes are scratch that the next build overwrites.
+int circuit_autotune_eval(vjx_circuit* c, const double* X, hipStream_t s);
+
 }  // namespace vjx

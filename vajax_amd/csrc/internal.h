@@ -85,6 +85,7 @@ struct DevModel {
   int32_t n_v = 0, n_jac = 0, n_res = 0, n_par = 0;
   int32_t n_lim = 0;    // limit states per instance
   int64_t lim_off = 0;  // first limit state of the group in the flat array
+  int32_t block = 256;  // eval workgroup size (launch plan: 64, 128 or 256)
   DBuf<int32_t> vn1, vn2, jac_dst, res_dst;
   DBuf<double> par;
 };

@@ -1874,7 +1874,7 @@ int model_n_lim(int id) { return id == VJX_MODEL_DIODE ? 1 : 0; }
 void launch_eval(const DevModel& m, const double* X, const DevScalars* sc, EvalOut out,
                  hipStream_t s) {
   if (m.count == 0) return;
-  const int B = 256;
+  const int B = m.block;  // 64, 128 or 256 (the kernels' launch bound)
   const int G = nblk(m.count, B);
   const double* li = out.ls_in ? out.ls_in + m.lim_off : nullptr;
   double* lo = out.ls_out ? out.ls_out + m.lim_off : nullptr;

@@ -169,7 +169,9 @@ struct Driver {
     if (rc < 0 || !cfg->autotune) return rc;
     rc = pattern_autotune_factor(pat, w.csr.p);
     if (rc < 0) return rc;
-    return pattern_autotune_solves(pat, w.csr.p);
+    rc = pattern_autotune_solves(pat, w.csr.p);
+    if (rc < 0) return rc;
+    return circuit_autotune_eval(c, w.Xw.p, s);
   }
 
   // One Newton iteration (solver_factories.py:230-325) on Xw, kernels only

@@ -264,6 +264,16 @@ class DeviceCircuit:
     def n_limit_states(self) -> int:
         return int(L.lib.vjx_circuit_limit_states(self.h))
 
+    def eval_blocks(self) -> Dict[str, int]:
+        """Eval workgroup size per model group (256 unless the autotuner of
+        a transient run with ``autotune`` chose 128 or 64)."""
+        out = {}
+        for g, name in enumerate(self.topo.models):
+            b = int(L.lib.vjx_circuit_eval_block(self.h, g))
+            L.check(min(b, 0), "vjx_circuit_eval_block")
+            out[name] = b
+        return out
+
     def source_values(self, t: float, device="cuda"):
         """Source values at time t evaluated on the device (vjx_eval_sources):
         (vsource_vals, isource_vals) as torch tensors."""
