@@ -4,7 +4,8 @@ Bars: the eval within 1e-12 of each instance's largest output (device
 exp/log/sqrt against numpy's, through the model's cancellations); the
 assembled system within 1e-12 of its largest entry; node voltages within
 1e-9 V on a fixed grid (FP64) and 1e-6 V with the FP32 LU and one FP64
-refinement; on the oracle's adaptive grid (replayed) 1e-9 V + 1e-6 |V|.
+refinement, from a 150-gate DAG up to the 20k-gate configuration; on the
+oracle's adaptive grid (replayed) 1e-9 V + 1e-6 |V|.
 """
 
 import ctypes as C
@@ -211,4 +212,23 @@ def test_ekv26_packed_batch_fixed_step_fp64_and_mixed():
                      use_graph=True)
     e32 = float(np.max(np.abs(r32.stats["V_out"] - ref.V)))
     _say(f"ekv26 batch: max |dV| fp64 {e64:.3e} V, mixed {e32:.3e} V")
+    assert e64 <= 1e-9 and e32 <= 1e-6, (e64, e32)
+
+
+def test_ekv26_config_dag_20k_fixed_fp64_and_mixed():
+    """The 20k-gate DAG with every MOSFET the EKV 2.6 model (80 000 devices,
+    40 130 unknowns), 25 fixed steps through the supply ramp, against the
+    oracle: FP64 within 1e-9 V, FP32 LU + FP64 refinement within 1e-6 V."""
+    net = G.gate_dag(20000, seed=0, model="ekv26")
+    _say("ekv26 dag 20k: oracle 25 fixed steps (about a minute)")
+    _, ref = run_oracle(net, 25e-12, 1e-12, fixed=True, tran_fs=1.0)
+    _say("ekv26 dag 20k: gpu fp64")
+    _, r64 = run_gpu(net, 25e-12, 1e-12, fixed=True, tran_fs=1.0, use_graph=True)
+    assert np.array_equal(r64.times, ref.times)
+    e64 = float(np.max(np.abs(r64.stats["V_out"] - ref.V)))
+    _say("ekv26 dag 20k: gpu mixed")
+    _, r32 = run_gpu(net, 25e-12, 1e-12, fixed=True, tran_fs=1.0, precision="f32",
+                     use_graph=True)
+    e32 = float(np.max(np.abs(r32.stats["V_out"] - ref.V)))
+    _say(f"ekv26 dag 20k: max |dV| fp64 {e64:.3e} V, mixed {e32:.3e} V")
     assert e64 <= 1e-9 and e32 <= 1e-6, (e64, e32)
