@@ -88,6 +88,11 @@ void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const dou
                  const double* x, const double* base, double sgn, double* out,
                  const HubChunks& hub, hipStream_t s);
 void launch_status_reset(vjx::DevStatus* st, hipStream_t s);
+// Xsave[0, n) = X, ls_save[0, nl) = ls, and the status records a, b, c
+// (each may be null) cleared: one launch.
+void launch_iter_begin(int n, const double* X, double* Xsave, int nl, const double* ls,
+                       double* ls_save, vjx::DevStatus* a, vjx::DevStatus* b, vjx::DevStatus* c,
+                       hipStream_t s);
 
 // Enqueue one system build: eval, stamping reductions, residual.
 // ls_in (NULL: zeros) / ls_out: flat limit states of the limited devices.

@@ -1840,18 +1840,36 @@ __global__ void k_sources(SourceTable src, double t, double* out) {
   if (i < src.n) out[i] = wave_value(src, i, t);
 }
 
+__device__ __forceinline__ void status_clear(DevStatus* st) {
+  st->maxf_bits = 0;
+  st->maxdv_bits = 0;
+  st->maxdelta_bits = 0;
+  st->maxl_bits = 0;
+  st->limd_bits = 0;
+  st->limr_bits = 0;
+  st->res_fail = 0;
+  st->delta_fail = 0;
+  st->nonfinite = 0;
+  st->nonfinite_in = 0;
+}
+
 __global__ void k_status_reset(DevStatus* st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->maxf_bits = 0;
-    st->maxdv_bits = 0;
-    st->maxdelta_bits = 0;
-    st->maxl_bits = 0;
-    st->limd_bits = 0;
-    st->limr_bits = 0;
-    st->res_fail = 0;
-    st->delta_fail = 0;
-    st->nonfinite = 0;
-    st->nonfinite_in = 0;
+  if (threadIdx.x == 0 && blockIdx.x == 0) status_clear(st);
+}
+
+// The start of a Newton iteration in one launch: the iterate and the limit
+// states saved for a re-pivot redo, and up to three status records cleared.
+__global__ void __launch_bounds__(256)
+    k_iter_begin(int n, const double* __restrict__ X, double* __restrict__ Xsave, int nl,
+                 const double* __restrict__ ls, double* __restrict__ ls_save, DevStatus* a,
+                 DevStatus* b, DevStatus* c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Xsave[i] = X[i];
+  if (i < nl) ls_save[i] = ls[i];
+  if (i == 0) {
+    if (a) status_clear(a);
+    if (b) status_clear(b);
+    if (c) status_clear(c);
   }
 }
 
@@ -2192,6 +2210,11 @@ void launch_spmv(int n, const int32_t* indptr, const int32_t* indices, const dou
 }
 
 void launch_status_reset(DevStatus* st, hipStream_t s) { k_status_reset<<<1, 64, 0, s>>>(st); }
+void launch_iter_begin(int n, const double* X, double* Xsave, int nl, const double* ls,
+                       double* ls_save, DevStatus* a, DevStatus* b, DevStatus* c, hipStream_t s) {
+  k_iter_begin<<<nblk(std::max(std::max(n, nl), 1), 256), 256, 0, s>>>(n, X, Xsave, nl, ls,
+                                                                       ls_save, a, b, c);
+}
 
 void launch_axpy(int n, double* x, const double* d, hipStream_t s) {
   if (n == 0) return;

@@ -113,7 +113,7 @@ struct Driver {
   double lte_scale = 0.0;
   void enqueue_lte() {
     if (!lte_hook) return;
-    launch_status_reset(w.st_lte.p, s);
+    // the iteration's first kernel cleared st_lte
     launch_lte(n_total, w.Xw.p, w.Vp.p, w.Vmax.p, lte_scale, cfg->reltol, cfg->abstol,
                w.st_lte.p, s);
     launch_status_export(w.st_lte.p, nullptr, w.d_hst + 2, s);
@@ -188,10 +188,10 @@ struct Driver {
       for (auto& e : ev) VJX_TRY_HIP(hipEventCreate(&e));
     // per-launch events only in eager profiling passes (never under capture)
     ktimer_enable(prof);
-    launch_copy(n_total + m, w.Xw.p, w.Xsave.p, s);
-    if (n_lim) launch_copy(n_lim, w.ls_iter.p, w.ls_save.p, s);
-    launch_status_reset(pat->d_status.p, s);
-    launch_status_reset(w.st.p, s);
+    // save the iterate (and limit states) for a re-pivot redo; clear the LU,
+    // Newton and LTE status records
+    launch_iter_begin(n_total + m, w.Xw.p, w.Xsave.p, n_lim, n_lim ? w.ls_iter.p : nullptr,
+                      n_lim ? w.ls_save.p : nullptr, pat->d_status.p, w.st.p, w.st_lte.p, s);
     if (prof) VJX_TRY_HIP(hipEventRecord(ev[0], s));
     circuit_build_async(c, w.Xw.p, w.vsrc.p, w.isrc.p, Qp, dQ, Q2, floor, w.csr.p, w.f.p, w.Qn.p,
                         w.mrc.p, w.Ivs.p, w.st.p, n_lim ? w.ls_iter.p : nullptr,
