@@ -30,6 +30,17 @@ def _case(model, rng):
         vb = rng.uniform(-1.5, 1.5, (3, N))
         ref = omod.eval_mosfet(vb[0], vb[1], vb[2], omod.mosfet_cache(p))
         ref = tuple(ref) + (None, None, None)
+    elif model == "ekv26":
+        from oracle import ekv26 as E
+        from vajax_amd.netlist import EKV26_PARAMS
+
+        typ = np.where(rng.random(N) < 0.5, 1.0, -1.0)
+        p = {k: np.full(N, float(v)) for k, v in EKV26_PARAMS.items()}
+        p.update(type=typ, w=rng.uniform(0.2e-6, 2e-6, N), l=rng.uniform(45e-9, 1e-6, N),
+                 vto=typ * 0.35, cox=np.full(N, 0.015), xj=np.full(N, 30e-9),
+                 hdif=np.full(N, 0.1e-6), xd_cj=np.full(N, 1e-3))
+        vb = rng.uniform(-1.2, 1.2, (3, N))
+        ref = tuple(E.eval_ekv26(vb[0], vb[1], vb[2], E.cache(p))) + (None, None, None)
     elif model == "diode":
         p = {"is": rng.uniform(1e-15, 1e-13, N), "n": np.full(N, 1.2), "cj": rng.uniform(1e-15, 1e-14, N)}
         vb = rng.uniform(-1.0, 1.0, (1, N))
@@ -45,7 +56,7 @@ def _case(model, rng):
     return p, vb, None, ref
 
 
-@pytest.mark.parametrize("model", ["resistor", "capacitor", "mosfet", "diode"])
+@pytest.mark.parametrize("model", ["resistor", "capacitor", "mosfet", "diode", "ekv26"])
 def test_split_eval_nine_outputs(model):
     import torch
 
@@ -101,7 +112,10 @@ def test_split_eval_nine_outputs(model):
     for k in range(6):
         want = ref[k] * mf if ref[k] is not None else np.zeros_like(got[k])
         scale = np.max(np.abs(want), axis=1, keepdims=True) + 1e-300
-        assert np.all(np.abs(got[k] - want) <= 1e-13 * scale), (model, names[k])
+        # EKV 2.6: device exp/log/sqrt against numpy's through the model's
+        # cancellations (3e-14 of the row measured)
+        tol = 1e-12 if model == "ekv26" else 1e-13
+        assert np.all(np.abs(got[k] - want) <= tol * scale), (model, names[k])
     assert np.all(got[6] == 0.0) and np.all(got[7] == 0.0)
     if n_lim:
         np.testing.assert_allclose(lo.cpu().numpy(), ref[6], rtol=1e-14, atol=0)
