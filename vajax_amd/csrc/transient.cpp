@@ -600,12 +600,9 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
       // the limit state advances with a converged step (full_mna.py:1349-1353)
       if (nl && conv) launch_copy(nl, w.ls_iter.p, w.ls_step.p, s);
       times[i + 1] = t_next;
-      launch_copy(nx, w.Xw.p, w.Xa.p, s);
+      launch_fixed_accept(nx, n_unk, w.Xw.p, w.Xa.p, w.Qn.p, w.dQn.p, w.Qp.p, w.dQ.p, w.Q2.p, s);
       rc = emit(i + 1, 0);
       if (rc < 0) return rc;
-      launch_copy(n_unk, w.Qp.p, w.Q2.p, s);
-      launch_copy(n_unk, w.Qn.p, w.Qp.p, s);
-      launch_copy(n_unk, w.dQn.p, w.dQ.p, s);
     }
     step_idx = n_steps + 1;
     res->min_dt_used = dt;

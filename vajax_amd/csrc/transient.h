@@ -19,6 +19,9 @@ void launch_noi_apply(int nz, const int32_t* zero, int nd, const int32_t* diag,
                       const int32_t* rows, double* csr, double* f, hipStream_t s);
 void launch_noi_clamp(int nd, const int32_t* rows, double* X, hipStream_t s);
 void launch_copy(int n, const double* a, double* b, hipStream_t s);
+// Xa = Xw over nx; Q2 = Qp, Qp = Qn, dQ = dQn over n_unk (one launch)
+void launch_fixed_accept(int nx, int n_unk, const double* Xw, double* Xa, const double* Qn,
+                         const double* dQn, double* Qp, double* dQ, double* Q2, hipStream_t s);
 // out[0] = *a and, when b is given, out[1] = *b (out: mapped pinned host memory)
 void launch_status_export(const DevStatus* a, const DevStatus* b, DevStatus* out, hipStream_t s);
 void launch_window_mark(int which, hipStream_t s);

@@ -99,6 +99,22 @@ __global__ void k_copy(int n, const double* __restrict__ a, double* b) {
   if (i < n) b[i] = a[i];
 }
 
+// The fixed-step loop's accepted step in one launch (full_mna.py:1355-1365):
+// X_accepted = X_new, and the charge history shifts, Q_prev2 = Q_prev,
+// Q_prev = Q_new, dQ/dt_prev = dQ/dt_new (each element read before written).
+__global__ void k_fixed_accept(int nx, int n_unk, const double* __restrict__ Xw,
+                               double* __restrict__ Xa, const double* __restrict__ Qn,
+                               const double* __restrict__ dQn, double* __restrict__ Qp,
+                               double* __restrict__ dQ, double* __restrict__ Q2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nx) Xa[i] = Xw[i];
+  if (i < n_unk) {
+    Q2[i] = Qp[i];
+    Qp[i] = Qn[i];
+    dQ[i] = dQn[i];
+  }
+}
+
 // The Newton and LU status records written by one thread straight into
 // mapped pinned host memory at the end of an iteration (no copy launches on
 // the critical path).
@@ -232,6 +248,11 @@ void launch_fill(int n, double v, double* x, hipStream_t s) {
 }
 void launch_copy(int n, const double* a, double* b, hipStream_t s) {
   if (n) k_copy<<<nb(n), 256, 0, s>>>(n, a, b);
+}
+void launch_fixed_accept(int nx, int n_unk, const double* Xw, double* Xa, const double* Qn,
+                         const double* dQn, double* Qp, double* dQ, double* Q2, hipStream_t s) {
+  const int n = nx > n_unk ? nx : n_unk;
+  if (n) k_fixed_accept<<<nb(n), 256, 0, s>>>(nx, n_unk, Xw, Xa, Qn, dQn, Qp, dQ, Q2);
 }
 void launch_window_mark(int which, hipStream_t s) { k_window_mark<<<1, 64, 0, s>>>(which); }
 void launch_status_export(const DevStatus* a, const DevStatus* b, DevStatus* out, hipStream_t s) {
