@@ -28,6 +28,9 @@ CASES = {
     "rc_ladder_20_s1_fixed": (lambda: G.rc_ladder(20, seed=1), 50e-12, 1e-12, True),
     "diode_ladder_20_s2_adaptive": (lambda: G.diode_ladder(20, seed=2), 0.8e-9, 1e-12, False),
     "gate_dag_40_s3_fixed": (lambda: G.gate_dag(40, seed=3), 20e-12, 1e-12, True),
+    # the EKV 2.6 model (vajax/devices/models/ekv/ekv.va) on the same kind of circuit
+    "gate_dag_ekv26_40_s4_fixed": (lambda: G.gate_dag(40, seed=4, model="ekv26"), 20e-12, 1e-12,
+                                   True),
 }
 
 
