@@ -288,6 +288,9 @@ struct SourceTable {
   const double* pwl_v;
 };
 void launch_sources(const SourceTable& src, double t, double* out, hipStream_t s);
+// both tables at t in one launch (either may be empty)
+void launch_sources2(const SourceTable& a, double* out_a, const SourceTable& b, double* out_b,
+                     double t, hipStream_t s);
 
 int model_n_v(int model_id);
 int model_n_lim(int model_id);

@@ -1840,6 +1840,20 @@ __global__ void k_sources(SourceTable src, double t, double* out) {
   if (i < src.n) out[i] = wave_value(src, i, t);
 }
 
+// Both source tables at t in one launch: workgroups [0, nba) the first,
+// the rest the second.
+__global__ void k_sources2(SourceTable a, double* out_a, int nba, SourceTable b, double* out_b,
+                           double t) {
+  const bool first = (int)blockIdx.x < nba;
+  const int blk = first ? (int)blockIdx.x : (int)blockIdx.x - nba;
+  const int i = blk * blockDim.x + threadIdx.x;
+  if (first) {
+    if (i < a.n) out_a[i] = wave_value(a, i, t);
+  } else if (i < b.n) {
+    out_b[i] = wave_value(b, i, t);
+  }
+}
+
 __device__ __forceinline__ void status_clear(DevStatus* st) {
   st->maxf_bits = 0;
   st->maxdv_bits = 0;
@@ -2224,6 +2238,13 @@ void launch_axpy(int n, double* x, const double* d, hipStream_t s) {
 void launch_sources(const SourceTable& src, double t, double* out, hipStream_t s) {
   if (src.n == 0) return;
   k_sources<<<nblk(src.n, 64), 64, 0, s>>>(src, t, out);
+}
+
+void launch_sources2(const SourceTable& a, double* out_a, const SourceTable& b, double* out_b,
+                     double t, hipStream_t s) {
+  const int nba = nblk(a.n, 64), nbb = nblk(b.n, 64);
+  if (nba + nbb == 0) return;
+  k_sources2<<<nba + nbb, 64, 0, s>>>(a, out_a, nba, b, out_b, t);
 }
 
 }  // namespace vjx

@@ -108,8 +108,7 @@ struct Driver {
   }
   // source values at t, evaluated on the device (no host round trip)
   int set_sources(double t) {
-    launch_sources(c->vsrc_table(), t, w.vsrc.p, s);
-    launch_sources(c->isrc_table(), t, w.isrc.p, s);
+    launch_sources2(c->vsrc_table(), w.vsrc.p, c->isrc_table(), w.isrc.p, t, s);
     VJX_TRY_HIP(hipGetLastError());
     return VJX_OK;
   }
@@ -703,13 +702,14 @@ extern "C" int vjx_transient_run(vjx_circuit* c, const vjx_tran_config* cfg, vjx
         a[0] = l0; a[1] = l1; a[2] = l2; pred_err = err2;
       }
       for (int j = order + 1; j < 3; ++j) a[j] = 0.0;
+      // predictor, and the step's residual floor from the history maxima
       launch_predict(n_total, m, w.Xa.p, hist_ptr(0), hist_ptr(1), hist_ptr(2), a[0], a[1], a[2],
-                     can_predict ? 1 : 0, w.Vp.p, w.Xw.p, s);
+                     can_predict ? 1 : 0, w.Vp.p, w.Xw.p, n_unk, w.hmrc.p, cfg->reltol,
+                     cfg->abstol, w.floor.p, s);
       double ramp = cfg->gshunt_steps > 0
                         ? std::min(std::max((double)step_idx / cfg->gshunt_steps, 0.0), 1.0)
                         : 1.0;
       const double gshunt = cfg->gshunt_init + ramp * (cfg->gshunt_target - cfg->gshunt_init);
-      launch_floor(n_unk, w.hmrc.p, cfg->reltol, cfg->abstol, w.floor.p, s);
       int iters;
       bool conv;
       double maxf;

@@ -9,11 +9,12 @@ void launch_delta(int n_unk, int m, int n_total, const double* delta, const doub
                   const DevScalars* sc, DevStatus* st, hipStream_t s);
 void launch_update(int n_unk, int m, int n_total, const double* delta, double* X,
                    const DevScalars* sc, const DevStatus* st, hipStream_t s);
+// predictor over the n_total + m unknowns; with floor given, also
+// floor[i] = max(hmrc[i] * reltol, abstol) for i < n_unk
 void launch_predict(int n_total, int m, const double* Xa, const double* h0, const double* h1,
                     const double* h2, double a0, double a1, double a2, int can_predict, double* Vp,
-                    double* Xw, hipStream_t s);
-void launch_floor(int n, const double* hmrc, double reltol, double abstol, double* floor,
-                  hipStream_t s);
+                    double* Xw, int n_unk, const double* hmrc, double reltol, double abstol,
+                    double* floor, hipStream_t s);
 void launch_fill(int n, double v, double* x, hipStream_t s);
 void launch_noi_apply(int nz, const int32_t* zero, int nd, const int32_t* diag,
                       const int32_t* rows, double* csr, double* f, hipStream_t s);

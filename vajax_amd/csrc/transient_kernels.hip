@@ -69,11 +69,17 @@ __global__ void __launch_bounds__(256) k_update(int n_unk, int m, int n_total,
   }
 }
 
+// Predictor (adaptive.py:94-218) and, for the node rows, the residual
+// tolerance floor max(reltol * history max |contribution|, abstol)
+// (full_mna.py:1922-1928) of the step, in one launch.
 __global__ void k_predict(int n_total, int m, const double* __restrict__ Xa,
                           const double* __restrict__ h0, const double* __restrict__ h1,
                           const double* __restrict__ h2, double a0, double a1, double a2,
-                          int can_predict, double* Vp, double* Xw) {
+                          int can_predict, double* Vp, double* Xw, int n_unk,
+                          const double* __restrict__ hmrc, double reltol, double abstol,
+                          double* floor) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (floor && i < n_unk) floor[i] = jmax(hmrc[i] * reltol, abstol);
   if (i < n_total) {
     const double v = a0 * h0[i] + a1 * h1[i] + a2 * h2[i];
     Vp[i] = v;
@@ -81,12 +87,6 @@ __global__ void k_predict(int n_total, int m, const double* __restrict__ Xa,
   } else if (i < n_total + m) {
     Xw[i] = Xa[i];
   }
-}
-
-__global__ void k_floor(int n, const double* __restrict__ hmrc, double reltol, double abstol,
-                        double* floor) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) floor[i] = jmax(hmrc[i] * reltol, abstol);
 }
 
 __global__ void k_fill(int n, double v, double* x) {
@@ -203,13 +203,11 @@ void launch_update(int n_unk, int m, int n_total, const double* delta, double* X
 }
 void launch_predict(int n_total, int m, const double* Xa, const double* h0, const double* h1,
                     const double* h2, double a0, double a1, double a2, int can_predict, double* Vp,
-                    double* Xw, hipStream_t s) {
+                    double* Xw, int n_unk, const double* hmrc, double reltol, double abstol,
+                    double* floor, hipStream_t s) {
+  // n_unk < n_total: the grid covers the floor rows too
   k_predict<<<nb(n_total + m), 256, 0, s>>>(n_total, m, Xa, h0, h1, h2, a0, a1, a2, can_predict,
-                                            Vp, Xw);
-}
-void launch_floor(int n, const double* hmrc, double reltol, double abstol, double* floor,
-                  hipStream_t s) {
-  if (n) k_floor<<<nb(n), 256, 0, s>>>(n, hmrc, reltol, abstol, floor);
+                                            Vp, Xw, n_unk, hmrc, reltol, abstol, floor);
 }
 // Noise-only internal nodes in the Newton solve (solver_factories.py:1268-1276):
 // zero their rows and their columns in node rows, put 1 on their diagonals,
