@@ -25,7 +25,12 @@ differences of the residuals; KCL and charge conservation; source/drain
 symmetry; the normalized-charge interpolation against the exact root of
 2 q + ln q = x; the long-channel limit against the textbook EKV drain
 current and its 40/60 saturation charge partition and Cgg = W L Cox in
-strong inversion; the junction diode against its closed form.  No golden
+strong inversion; the junction diode against its closed form; and the
+short-channel terms (LETA roll-off and DIBL, WETA, the Q0 flat-band shift,
+E0 mobility reduction, LAMBDA channel-length modulation, UCRIT velocity
+saturation, impact ionisation) by the direction each must move the device
+in, against the same device with the term off.  Those terms are pinned in
+direction, not in value: no closed form fixes their magnitudes.  No golden
 vectors for this model exist in the reference tree (its comparison test,
 tests/test_osdi_jax_comparison.py:304-320, needs compiled OSDI).
 """

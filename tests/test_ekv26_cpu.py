@@ -230,3 +230,65 @@ def test_series_resistance_is_refused():
         ekv26_cache(p)
     with pytest.raises(ValueError, match="RSH"):
         E.cache(p)
+
+
+def _core(n, **kw):
+    """Intrinsic device only (no junctions, no impact ionisation), every
+    short-channel term off unless given."""
+    d = dict(w=1e-6, l=0.2e-6, vto=0.4, q0=0.0, leta=0.0, weta=0.0, iba=0.0, hdif=0.0,
+             xd_jswg=0.0, xd_cjswg=0.0, e0=0.0, theta=0.0)
+    d["lambda"] = 0.0
+    d.update(kw)
+    return _params(n, **d)
+
+
+def _vp(p, vg=0.8, vd=0.05):
+    n = len(p["w"])
+    r = E.intrinsic(E.D(np.full(n, vg)), E.D(np.zeros(n)), E.D(np.full(n, vd)), E.cache(p))
+    return r
+
+
+def test_short_channel_terms_act_in_their_physical_direction():
+    """What the short-channel terms must do, each switched on against the
+    same device with it off (a dropped term or a flipped sign fails)."""
+    # LETA: threshold roll-off (VP rises as L shrinks) and drain-induced
+    # barrier lowering (VP rises with V_D); none without it
+    L = np.array([0.1e-6, 0.2e-6, 1e-6])
+    off, on = _core(3, l=L), _core(3, l=L, leta=0.3)
+    assert np.ptp(_vp(off)["VP"].v) == 0.0
+    vp = _vp(on)["VP"].v
+    assert np.all(np.diff(vp) < 0)
+    dibl = _vp(on, vd=1.0)["VP"].v - vp
+    assert np.all(dibl > 0) and np.all(np.diff(dibl) < 0)
+    assert np.all(_vp(off, vd=1.0)["VP"].v == _vp(off)["VP"].v)
+    # WETA: narrow-channel effect (VP falls as W narrows)
+    W = np.array([0.2e-6, 1e-6, 10e-6])
+    assert np.all(np.diff(_vp(_core(3, w=W, weta=0.2))["VP"].v) > 0)
+    assert np.ptp(_vp(_core(3, w=W))["VP"].v) == 0.0
+    # Q0: reverse short-channel flat-band shift, larger for shorter L
+    dvfb = E.cache(_core(3, l=L, q0=230e-6))["deltaVFB"]
+    assert np.all(dvfb > 0) and np.all(np.diff(dvfb) < 0)
+    # E0: vertical-field mobility reduction, beta falls with V_G
+    vg = np.array([0.8, 1.2, 1.6, 2.0])
+    n = len(vg)
+    k0, k1 = E.cache(_core(n, l=1e-6)), E.cache(_core(n, l=1e-6, e0=1e8))
+    b0 = E.intrinsic(E.D(vg), E.D(np.zeros(n)), E.D(np.full(n, 0.05)), k0)["beta"].v
+    b1 = E.intrinsic(E.D(vg), E.D(np.zeros(n)), E.D(np.full(n, 0.05)), k1)["beta"].v
+    assert np.all(b1 < b0) and np.all(np.diff(b1) < 0) and np.all(np.diff(b1 / b0) < 0)
+    # LAMBDA: channel-length modulation, the saturation current rises with V_D
+    vd = np.linspace(0.6, 1.2, 7)
+    n = len(vd)
+    i0 = E.eval_ekv26(np.ones(n), np.zeros(n), vd, E.cache(_core(n, ucrit=2e6)))[0][:, 0]
+    i1 = E.eval_ekv26(np.ones(n), np.zeros(n), vd, E.cache(_core(n, ucrit=2e6, **{"lambda": 0.5})))[0][:, 0]
+    assert np.all(np.diff(i1) > 0)
+    assert (i1[-1] - i1[0]) / i1[0] > 100 * (i0[-1] - i0[0]) / i0[0]
+    # UCRIT: velocity saturation lowers the short-channel current
+    cur = [E.eval_ekv26(np.full(1, 1.2), np.zeros(1), np.full(1, 1.2),
+                        E.cache(_core(1, l=0.1e-6, ucrit=u)))[0][0, 0] for u in (1e30, 5e6, 1e6)]
+    assert cur[0] > cur[1] > cur[2] > 0
+    # impact ionisation: no substrate current in the linear region, then
+    # rising with V_D in saturation
+    vd = np.array([0.05, 0.5, 1.5, 3.0])
+    r = E.intrinsic(E.D(np.full(4, 1.2)), E.D(np.zeros(4)), E.D(vd), E.cache(_params(4, w=1e-6, l=0.2e-6)))
+    ratio = r["Isub"].v / r["Id"].v
+    assert ratio[0] == 0.0 and np.all(np.diff(ratio[1:]) > 0)
