@@ -523,6 +523,16 @@ def run_ekv26(args):
     res = run_gpu(eng, total, "f64", True, window_from=args.warmup + 1)
     w = window_stats(res)
     log(f"ekv26: {w['ms_per_step']} ms/step, {w['newton_iters_per_s']} Newton iters/s")
+    mixed = None
+    if not args.no_mixed:
+        # FP32 LU + one FP64 refinement over the same window, and on the FP64
+        # run's grid for the voltage difference
+        r32 = run_gpu(eng, total, "f32", True, window_from=args.warmup + 1)
+        mixed = {"window": window_stats(r32), **mixed_error(eng, res, True)}
+        mixed["mixed_vs_fp64_ms_per_step"] = round(
+            w["ms_per_step"] / mixed["window"]["ms_per_step"], 3)
+        log(f"ekv26 mixed: {mixed['window']['ms_per_step']} ms/step, max |dV| vs FP64 "
+            f"{mixed['max_abs_dV_vs_fp64']:.2e} V")
     L.lib.vjx_profile_enable(1)
     L.lib.vjx_profile_reset()
     run_gpu(eng, min(total, 60), "f64", False)
@@ -540,6 +550,7 @@ def run_ekv26(args):
            "mosfets": n_dev, "unknowns": topo.n_augmented, **w,
            "analysis_ms": [round(x, 1) for x in eng.device.lu.info()["analyze_ms"]],
            "eval_blocks": eng.device.eval_blocks(),
+           "mixed_precision": mixed,
            "phase_ms_per_newton_iter": {"build": round(ph["build_ms"] / iters, 4),
                                         "lu": round(ph["lu_ms"] / iters, 4),
                                         "solve": round(ph["solve_ms"] / iters, 4)},
