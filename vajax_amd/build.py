@@ -3,6 +3,13 @@
 Usage: ``python -m vajax_amd.build`` or ``vajax_amd.build.build()``.  Objects
 go to ``_build/`` at the repository root; the shared library is written next
 to this file so it travels with the source tree.
+
+``python -m vajax_amd.build --asan-host DIR`` builds, into DIR, a variant whose
+host C++ (analysis, orderings, C ABI, drivers) is compiled with
+AddressSanitizer, the device code unchanged (build the normal library first:
+its kernel objects are reused).  The CPU tests run against it with
+``VJX_LIB=DIR/libvjx.so`` and the sanitizer runtime preloaded
+(``LD_PRELOAD=<clang resource dir>/lib/linux/libclang_rt.asan-x86_64.so``).
 """
 
 from __future__ import annotations
@@ -61,5 +68,31 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+def build_asan_host(out_dir: str) -> Path:
+    """The host sources under AddressSanitizer (-Xarch_host: no device code is
+    instrumented), linked with the kernel objects of the last normal build
+    into ``out_dir/libvjx.so``."""
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    flags = [f for f in CXX_FLAGS if f != "-O3"] + [
+        "-O1", "-g", "-fno-omit-frame-pointer", "-Xarch_host", "-fsanitize=address"]
+    objs = [str(OBJ / (s + ".o")) for s in HIP_SOURCES if (CSRC / s).exists()]
+    missing = [o for o in objs if not Path(o).exists()]
+    if missing:
+        raise RuntimeError(f"build the library first (missing {missing[0]})")
+    jobs = [[HIPCC, *flags, "-c", str(CSRC / src), "-o", str(out / (src + ".o"))]
+            for src in CXX_SOURCES]
+    with ThreadPoolExecutor(max_workers=min(len(jobs), 8)) as ex:
+        list(ex.map(_run, jobs))
+    objs += [str(out / (s + ".o")) for s in CXX_SOURCES]
+    lib = out / "libvjx.so"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Xarch_host",
+          "-fsanitize=address", "-shared-libsan", *objs, "-o", str(lib)])
+    return lib
+
+
 if __name__ == "__main__":
-    print(build(verbose=True))
+    if len(sys.argv) == 3 and sys.argv[1] == "--asan-host":
+        print(build_asan_host(sys.argv[2]))
+    else:
+        print(build(verbose=True))
