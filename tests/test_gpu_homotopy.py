@@ -65,3 +65,19 @@ def test_engine_dc_operating_point():
     cir, X0, vs, isv = _clamp_start(net)
     direct = oracle_nr_solve(cir, 100)(X0, vs, isv, np.zeros(cir.n_unk), 0.0)
     assert float(np.max(np.abs(V - direct[0][:cir.n_total]))) <= 1e-9
+
+
+@pytest.mark.parametrize("which", ["ring", "dag"])
+def test_ekv26_dc_operating_points_match_oracle(which):
+    """DC operating points of EKV 2.6 circuits: an 11-stage ring (plain
+    Newton from the mid-rail guess) and a 30-gate DAG (its stack nodes held
+    only by gmin: the chain's gshunt stepping), the device Newton solve
+    against the oracle's under the same host step control."""
+    net = (G.ring_oscillator(11, seed=2, model="ekv26") if which == "ring"
+           else G.gate_dag(30, seed=3, model="ekv26"))
+    cir, (Xo, ro), (Xg, rg) = _both(net, 50, SimulationOptions())
+    assert ro.converged and ro.method == ("direct" if which == "ring" else "chain_gshunt_stepping")
+    assert (rg.method, rg.converged) == (ro.method, ro.converged)
+    assert (rg.iterations, rg.homotopy_steps) == (ro.iterations, ro.homotopy_steps)
+    err = float(np.max(np.abs(Xg[:cir.n_total] - Xo[:cir.n_total])))
+    assert err <= 1e-9, err
