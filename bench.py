@@ -489,6 +489,19 @@ def run_mul64(args):
            "reference_note": "JAX GPU (cuDSS FP32 factor + FP64 refinement) on a Tesla T4 with "
                              "PSP103 (666 409 unknowns); this run: EKV substitute, 133 592 "
                              "unknowns, MI355X -- different model and hardware"}
+    # the same circuit with every MOSFET the EKV 2.6 model (charges, junctions,
+    # short-channel effects): the whole transient at a PSP-class eval cost
+    eng_e = CircuitEngine(G.with_ekv26(net))
+    eng_e.parse()
+    eng_e.prepare(t_stop=2e-9, dt=2e-12, precision="f64", use_graph=True, autotune=True,
+                  max_steps=3)
+    eng_e.run_transient()
+    eng_e.prepare(t_stop=2e-9, dt=2e-12, precision="f64", use_graph=True, autotune=True,
+                  window_from_step=1)
+    we = window_stats(eng_e.run_transient())
+    out["ekv26"] = {"workload": "mul64 with EKV 2.6 MOSFETs", **we}
+    log(f"mul64 ekv26: {we['accepted_steps']} steps, {we['ms_per_step']} ms/step")
+    del eng_e
     dense = ph["kernels"].get("lu_dense")
     if dense and dense["launches"]:
         ms = dense["ms"] / iters  # one refactorisation per Newton iteration

@@ -232,3 +232,27 @@ def test_ekv26_config_dag_20k_fixed_fp64_and_mixed():
     e32 = float(np.max(np.abs(r32.stats["V_out"] - ref.V)))
     _say(f"ekv26 dag 20k: max |dV| fp64 {e64:.3e} V, mixed {e32:.3e} V")
     assert e64 <= 1e-9 and e32 <= 1e-6, (e64, e32)
+
+
+def test_ekv26_array_multiplier_4x4():
+    """The reference's multiplier testbench topology at 4x4 (the generator of
+    its mul64 benchmark) with EKV 2.6 MOSFETs: the oracle's adaptive grid
+    replayed to the FP64 bar, and the free-running run's node voltages
+    within the rails."""
+    from vajax_amd import vacask
+
+    net = G.with_ekv26(vacask.load("", text=G.array_multiplier_testbench(
+        4, [1, 1, 0, 1], [0, 1, 1, 0], t_stop=2e-9)))
+    assert net.groups["ekv26"].count > 0 and "mosfet" not in net.groups
+    _say("ekv26 mul4: oracle adaptive 2 ns")
+    _, ref = run_oracle(net, 2e-9, 2e-12, lte_ratio=1.5)
+    _, rr = run_gpu(net, ref.times[-1], 2e-12, use_graph=True, replay_times=ref.times,
+                    lte_ratio=1.5)
+    assert np.array_equal(rr.times, ref.times) and rr.stats["rejected_steps"] == 0
+    err = np.abs(rr.stats["V_out"] - ref.V)
+    _say(f"ekv26 mul4: max |dV| on the oracle grid {float(err.max()):.3e} V")
+    assert float(np.max(err - 1e-6 * np.abs(ref.V))) <= 1e-9
+    _, res = run_gpu(net, 2e-9, 2e-12, use_graph=True, lte_ratio=1.5)
+    V = res.stats["V_out"]
+    assert abs(res.times[-1] - 2e-9) <= 1e-15 and np.all(np.isfinite(V))
+    assert float(V.min()) >= -0.5 and float(V.max()) <= 1.2 + 0.5

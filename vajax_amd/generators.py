@@ -14,6 +14,7 @@ vajax/analysis/engine.py:459-468) equals construction order.
   locality of a placed design instead of the expander structure of a uniform
   random graph.
 * :func:`monte_carlo` -- per-device threshold/mobility variation of a netlist.
+* :func:`with_ekv26` -- the same netlist with its MOSFETs as EKV 2.6 devices.
 
 MOSFETs use the EKV long-channel substitute model (``mosfet`` model type) at
 the technology values in :data:`NMOS_DEFAULTS` / :data:`PMOS_DEFAULTS`.
@@ -361,6 +362,34 @@ def monte_carlo(net: Netlist, seed: int, sigma: float = 0.03) -> Netlist:
         n = g.count
         g.params["vto"] = g.params["vto"] + rng.normal(0.0, 1.0, n) * sigma * g.params["vto"]
         g.params["kp"] = g.params["kp"] * rng.normal(1.0, sigma, n)
+    return out
+
+
+def with_ekv26(net: Netlist) -> Netlist:
+    """Copy of ``net`` whose MOSFETs (the long-channel ``mosfet`` group) are
+    EKV 2.6 devices: same terminals, polarity and W/L, the technology values
+    of :data:`EKV26_NMOS` / :data:`EKV26_PMOS`.  The group keeps its place in
+    the netlist's group order."""
+    out = copy.deepcopy(net)
+    g = out.groups.get("mosfet")
+    if g is None:
+        return out
+    typ = np.asarray(g.params["type"], dtype=np.float64)
+    params = {"type": typ, "w": np.asarray(g.params["w"], np.float64),
+              "l": np.asarray(g.params["l"], np.float64)}
+    for k, v in MODEL_PARAMS["ekv26"].items():
+        if k not in params:
+            params[k] = np.where(typ > 0, EKV26_NMOS.get(k, v), EKV26_PMOS.get(k, v)).astype(
+                np.float64)
+    groups = {}
+    for name, grp in out.groups.items():
+        if name == "mosfet":
+            groups["ekv26"] = DeviceGroup(model="ekv26", terminals=grp.terminals, params=params,
+                                          names=grp.names)
+        else:
+            groups[name] = grp
+    out.groups = groups
+    out.title = f"{net.title}_ekv26"
     return out
 
 
