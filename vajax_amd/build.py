@@ -4,12 +4,13 @@ Usage: ``python -m vajax_amd.build`` or ``vajax_amd.build.build()``.  Objects
 go to ``_build/`` at the repository root; the shared library is written next
 to this file so it travels with the source tree.
 
-``python -m vajax_amd.build --asan-host DIR`` builds, into DIR, a variant whose
-host C++ (analysis, orderings, C ABI, drivers) is compiled with
-AddressSanitizer, the device code unchanged (build the normal library first:
-its kernel objects are reused).  The CPU tests run against it with
-``VJX_LIB=DIR/libvjx.so`` and the sanitizer runtime preloaded
-(``LD_PRELOAD=<clang resource dir>/lib/linux/libclang_rt.asan-x86_64.so``).
+``python -m vajax_amd.build --asan-host DIR`` (``--ubsan-host DIR``) builds, into
+DIR, a variant whose host C++ (analysis, orderings, C ABI, drivers) is
+compiled with AddressSanitizer (UndefinedBehaviorSanitizer), the device code
+unchanged (build the normal library first: its kernel objects are reused).
+The CPU tests run against it with ``VJX_LIB=DIR/libvjx.so`` and the
+sanitizer runtime preloaded (``LD_PRELOAD=<clang resource
+dir>/lib/linux/libclang_rt.asan-x86_64.so``, or ``ubsan_standalone``).
 """
 
 from __future__ import annotations
@@ -68,14 +69,16 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
-def build_asan_host(out_dir: str) -> Path:
-    """The host sources under AddressSanitizer (-Xarch_host: no device code is
-    instrumented), linked with the kernel objects of the last normal build
-    into ``out_dir/libvjx.so``."""
+def build_asan_host(out_dir: str, sanitizer: str = "address") -> Path:
+    """The host sources under a sanitizer ("address" or "undefined";
+    -Xarch_host: no device code is instrumented), linked with the kernel
+    objects of the last normal build into ``out_dir/libvjx.so``."""
+    if sanitizer not in ("address", "undefined"):
+        raise ValueError("sanitizer must be 'address' or 'undefined'")
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     flags = [f for f in CXX_FLAGS if f != "-O3"] + [
-        "-O1", "-g", "-fno-omit-frame-pointer", "-Xarch_host", "-fsanitize=address"]
+        "-O1", "-g", "-fno-omit-frame-pointer", "-Xarch_host", f"-fsanitize={sanitizer}"]
     objs = [str(OBJ / (s + ".o")) for s in HIP_SOURCES if (CSRC / s).exists()]
     missing = [o for o in objs if not Path(o).exists()]
     if missing:
@@ -87,12 +90,13 @@ def build_asan_host(out_dir: str) -> Path:
     objs += [str(out / (s + ".o")) for s in CXX_SOURCES]
     lib = out / "libvjx.so"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Xarch_host",
-          "-fsanitize=address", "-shared-libsan", *objs, "-o", str(lib)])
+          f"-fsanitize={sanitizer}", "-shared-libsan", *objs, "-o", str(lib)])
     return lib
 
 
 if __name__ == "__main__":
-    if len(sys.argv) == 3 and sys.argv[1] == "--asan-host":
-        print(build_asan_host(sys.argv[2]))
+    if len(sys.argv) == 3 and sys.argv[1] in ("--asan-host", "--ubsan-host"):
+        kind = "address" if sys.argv[1] == "--asan-host" else "undefined"
+        print(build_asan_host(sys.argv[2], kind))
     else:
         print(build(verbose=True))
